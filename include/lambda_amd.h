@@ -1,0 +1,250 @@
+/* lambda_amd.h -- C ABI of the MI355X render loop of lambda.
+ *
+ * This is the drop-in boundary under `Renderer::render`
+ * (include/renderer.hpp:39 of the reference).  Host C++ (the .nls loader,
+ * the Renderer class and the CLI, declared in lambda_host.h) flattens an
+ * nl::cg::scene into a lam_scene_desc, uploads it once, and calls lam_render.
+ * All pointers crossing this ABI are HOST pointers unless a function name
+ * ends in `_dev`; device memory is owned by the context and never escapes.
+ *
+ * Error behaviour: every entry point returns LAM_OK (0) or a negative
+ * LAM_E_* code and never throws; lam_last_error() gives the text.  The
+ * reference's `render` is void and traps on FP errors (src/main.cpp:15);
+ * here device code never traps, errors come back as codes.
+ *
+ * Threading: one lam_ctx is used by one host thread at a time; every call
+ * is synchronous on return (the context owns one HIP stream).
+ */
+#ifndef LAMBDA_AMD_H
+#define LAMBDA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define LAM_OK 0
+#define LAM_E_INVALID (-1)  /* bad argument / malformed scene description */
+#define LAM_E_DEVICE (-2)   /* HIP runtime error (allocation, launch, fault) */
+#define LAM_E_NOSCENE (-3)  /* call needs an uploaded scene */
+#define LAM_E_LIMIT (-4)    /* a size exceeds a documented maximum */
+#define LAM_E_IO (-5)       /* file could not be read or written */
+#define LAM_E_PARSE (-6)    /* .nls / OBJ / CSV syntax or schema error */
+#define LAM_E_BVH (-7)      /* BVH exceeds the traversal stack bound */
+
+#define LAM_NONE 0xFFFFFFFFu /* "no object" / miss marker */
+
+/* ---- enums (values mirror the reference's variant alternatives) ------ */
+/* Object = std::variant<sphere, plane, trimesh>   (lib/numli/cg.hpp:857) */
+enum { LAM_OBJ_SPHERE = 1, LAM_OBJ_PLANE = 2, LAM_OBJ_TRIMESH = 3 };
+/* Material = variant<lambertian, blinn, thinfilm, microfacet, emitter,
+ * diremitter>                                      (lib/numli/cg.hpp:1084) */
+enum {
+  LAM_MAT_NONE = 0,
+  LAM_MAT_LAMBERTIAN = 1,
+  LAM_MAT_BLINN = 2,
+  LAM_MAT_THINFILM = 3,
+  LAM_MAT_MICROFACET = 4,
+  LAM_MAT_EMITTER = 5,
+  LAM_MAT_DIREMITTER = 6
+};
+/* Light = variant<ambient, point, dir, sphere, quaddir> (cg.hpp:904) */
+enum {
+  LAM_LIGHT_AMBIENT = 0,
+  LAM_LIGHT_POINT = 1,
+  LAM_LIGHT_DIR = 2,
+  LAM_LIGHT_SPHERE = 3,
+  LAM_LIGHT_QUADDIR = 4
+};
+
+/* ---- scene description (flat, SoA where it is large) ---------------- */
+
+/* Coefficient-spectrum grid: n values at inf + i*delta, delta =
+ * (sup-inf)/(n-1) in float.  The reference fixes n=80 on [400,720]
+ * (coefficientλ, cg.hpp:241-246); hero wavelengths are drawn on [inf,sup]
+ * (sample::spectrum, cg.hpp:1517-1531). */
+typedef struct {
+  uint32_t n;
+  float inf, sup;
+} lam_grid;
+
+/* Rows 0..2 of a 4x4 affine matrix, row-major (transform, cg.hpp:422-487).
+ * Row 3 is implicitly (0,0,0,1). */
+typedef struct {
+  float M[12];
+  float Minv[12];
+} lam_xform;
+
+typedef struct {
+  uint32_t type;     /* LAM_OBJ_* */
+  uint32_t material; /* index into materials */
+  uint32_t mesh;     /* trimesh: mesh id (informational) */
+  uint32_t pad;
+  lam_xform T;
+} lam_object;
+
+/* Spectrum slots (indices into the spectra table, -1 = unused):
+ *   lambertian: [0]=albedo                       (cg.hpp:921-926)
+ *   blinn:      [0]=Kd [1]=Ks [2]=Kt [3]=Le [4]=ior (cg.hpp:927-993)
+ *   thinfilm:   [0]=film_ior [1]=base_ior         (cg.hpp:995-1052)
+ *   emitter / diremitter: [0]=radiance            (cg.hpp:1069-1081)   */
+typedef struct {
+  uint32_t type;
+  int32_t spec[5];
+  float alpha;    /* blinn / thinfilm glossiness α */
+  float d;        /* thinfilm thickness in nm */
+  float p_d, p_spec, w_r, w_t; /* blinn::init() results (cg.hpp:931-940) */
+  float specular; /* diremitter exponent */
+  float pad;
+} lam_material;
+
+typedef struct {
+  uint32_t type;   /* LAM_LIGHT_* */
+  uint32_t object; /* sphere / quaddir: the emitting object; else LAM_NONE */
+  int32_t spec;    /* radiance or radiant intensity spectrum */
+  float size;      /* sphere light radius (cg.hpp:2121) */
+  float inv_area;  /* quaddir 1/(|col0||col1|) (cg.hpp:2098-2099) */
+  float specular;  /* quaddir exponent (cg.hpp:2087-2088) */
+  float pos[3];    /* point light position */
+  float dir[3];    /* directional light direction (as loaded) */
+} lam_light;
+
+/* camera after camera::init (cg.hpp:1151-1165, 2039-2063) */
+typedef struct {
+  float pos[3], x[3], y[3], z[3];
+  float D, delta, w, h, dof, fov;
+  uint32_t width, height;
+} lam_camera;
+
+typedef struct {
+  lam_grid grid;
+  uint32_t n_spectra;
+  const float *spectra; /* n_spectra * grid.n coefficients */
+  const float *cmf;     /* 3 * grid.n: CIE 1931 x̄,ȳ,z̄ box-filtered onto grid */
+  uint32_t n_objects;
+  const lam_object *objects;
+  uint32_t n_materials;
+  const lam_material *materials;
+  uint32_t n_lights;
+  const lam_light *lights;
+  /* every trimesh instance flattened to world space, in object order */
+  uint32_t n_tris;
+  const float *tri_v;       /* 9 per tri: v0,v1,v2 world space            */
+  const float *tri_n;       /* 9 per tri: vertex normals, normalToWorld()   */
+  const float *tri_gn;      /* 3 per tri: geometric normal, world, unit     */
+  const uint32_t *tri_obj;  /* object index of the instance                */
+  const uint32_t *tri_prim; /* face index inside the instanced mesh        */
+  lam_camera camera;
+} lam_scene_desc;
+
+/* ---- ray batches for the extend / shadow test hooks ------------------ */
+/* 32 B.  A hit is accepted for t in [BIAS, tmax]; `skip` is an object id
+ * to ignore (LAM_NONE for none), as intersect::scene's `skip`
+ * (cg.hpp:1469-1489). */
+typedef struct {
+  float o[3];
+  float tmax;
+  float d[3];
+  uint32_t skip;
+} lam_ray;
+
+/* 16 B.  Closest hit; ties in t go to the lowest (obj, prim).  Miss:
+ * obj = LAM_NONE, t = FLT_MAX.  `tri` is the index into the desc's flat
+ * triangle arrays (LAM_NONE for spheres/planes). */
+typedef struct {
+  float t;
+  uint32_t tri;
+  uint32_t obj;
+  uint32_t prim;
+} lam_hit;
+
+/* ---- render parameters ---------------------------------------------- */
+#define LAM_FLAG_MEGAKERNEL 0x1u /* one thread per path instead of queues */
+#define LAM_FLAG_NO_GRAPH 0x2u   /* launch the bounce loop without a hipGraph */
+#define LAM_FLAG_PROFILE 0x4u    /* time every kernel with HIP events */
+
+typedef struct {
+  uint32_t spp;             /* Renderer::SPP            (renderer.hpp:36) */
+  uint32_t max_scatterings; /* Renderer::MAX_SCATTERINGS (renderer.hpp:35) */
+  float sample_p;           /* Renderer::SAMPLE_P       (renderer.hpp:37) */
+  uint32_t hero;            /* hero wavelengths per path: 16 (reference) or 4 */
+  uint64_t seed;            /* counter-RNG key */
+  uint32_t spp_offset;      /* first sample index (progressive / resume) */
+  uint32_t flags;           /* LAM_FLAG_* */
+} lam_render_params;
+
+typedef struct {
+  uint32_t n_nodes, n_leaves, n_tris, depth;
+  uint32_t stack_bound; /* worst-case traversal stack entries */
+  uint32_t n_analytic;  /* spheres + planes tested outside the BVH */
+  uint64_t device_bytes;
+  double sah_cost;
+} lam_scene_stats;
+
+/* Kernel timing accumulated since the last reset (LAM_FLAG_PROFILE, or the
+ * extend/occlusion hooks which are always timed). */
+typedef struct {
+  double extend_ms, shadow_ms, shade_ms, raygen_ms, film_ms, other_ms;
+  uint64_t extend_launches, shadow_launches, shade_launches;
+  uint64_t extend_rays, shadow_rays;
+  double render_ms; /* wall time of the last lam_render on the stream */
+} lam_timing;
+
+/* ---- context ---------------------------------------------------------- */
+typedef struct lam_ctx lam_ctx;
+
+int lam_create(lam_ctx **out, int device);
+void lam_destroy(lam_ctx *ctx);
+const char *lam_last_error(const lam_ctx *ctx);
+/* Library version string. */
+const char *lam_version(void);
+
+/* Copies the scene to HBM: quantized BVH8 (128-B nodes), 48-B triangle
+ * records, shading records, material/spectra tables.  Replaces the scene
+ * build of loadTriMesh/buildBVH (cg.hpp:732-749, 2287-2304) and the
+ * per-render read of nl::cg::scene (cg.hpp:1167-1180). */
+int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *desc);
+int lam_get_scene_stats(const lam_ctx *ctx, lam_scene_stats *out);
+
+/* Renderer::render (src/renderer.cpp:14-61).  linrgb: w*h*3 floats, top
+ * row first (the reference's (h-i-1)*w+j flip).  xyz (optional, may be
+ * NULL): w*h*3 floats, compensated per-pixel XYZ sums before the
+ * 1/(SPP*hero) normalisation, same orientation. */
+int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb,
+               float *xyz);
+
+/* Renderer::tracePath (src/renderer.cpp:64-146) for a batch of
+ * (pixel, sample) pairs: pixel = i*w+j with row i counted from the bottom,
+ * as in render.  out: n*hero floats, the spectral radiance Li of each path
+ * (before the 1/prob_λ film weight). */
+int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p,
+                    const uint32_t *pixel, const uint32_t *sample, size_t n,
+                    float *radiance_out);
+
+/* Closest-hit query, intersect::scene (cg.hpp:1469-1489). */
+int lam_extend(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit *hits);
+/* Shadow query: occluded[i] = 1 iff a hit exists in [BIAS, tmax]
+ * excluding rays[i].skip (the shadow calls in cg.hpp:1596-1698). */
+int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n,
+                 uint32_t *occluded);
+/* Device-pointer variants for rays already resident in HBM; `counts`
+ * (optional, device, 2*n uint32) receives interior nodes fetched and
+ * triangles tested per ray. */
+int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
+                   lam_hit *d_hits, uint32_t *d_counts);
+
+/* Test hook: the device's elementary functions on n inputs (fn 0 sinf,
+ * 1 cosf, 2 powf(a,b), 3 atan2f(a,b), 4 sqrtf(a), 5 a/b), for bitwise
+ * comparison with the CPU oracle. */
+int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out);
+
+int lam_get_timing(const lam_ctx *ctx, lam_timing *out);
+int lam_reset_timing(lam_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAMBDA_AMD_H */

@@ -1,0 +1,51 @@
+/* lambda_host.h -- C ABI of the host side above lambda_amd.h: the .nls
+ * loader, tone mapping and PNG output, for callers that are not C++ (the
+ * Python binding, tests).  C++ callers can use the Renderer class in
+ * lambda_amd/csrc/host/renderer.hpp instead, which mirrors the reference's
+ * include/renderer.hpp.
+ */
+#ifndef LAMBDA_HOST_H
+#define LAMBDA_HOST_H
+
+#include "lambda_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lamh_scene lamh_scene;
+
+/* nl::cg::load::loadNLS (lib/numli/cg.hpp:2455-2470).  Relative paths in
+ * the scene (spectra CSV, OBJ meshes) resolve against the current working
+ * directory, as in the reference.  data_dir (may be NULL) holds the CIE
+ * tables.  Returns LAM_OK, LAM_E_IO or LAM_E_PARSE; err receives the text. */
+int lamh_load_nls(const char *path, const char *data_dir, lamh_scene **out, char *err, size_t errlen);
+int lamh_load_nls_text(const char *text, const char *data_dir, lamh_scene **out, char *err, size_t errlen);
+void lamh_scene_free(lamh_scene *s);
+/* Flattened description, valid until lamh_scene_free. */
+const lam_scene_desc *lamh_scene_desc(const lamh_scene *s);
+/* Hero wavelengths requested by the scene ("hero_samples", default 16). */
+uint32_t lamh_scene_hero(const lamh_scene *s);
+/* Names (for tests): object i / material i; NULL if out of range. */
+const char *lamh_object_name(const lamh_scene *s, uint32_t i);
+const char *lamh_material_name(const lamh_scene *s, uint32_t i);
+
+/* Renderer::toneMap (src/renderer.cpp:149-176): log-average exposure
+ * Y_MID/Ȳ then the koiFilmic curve per channel.  rgb_in/out: w*h*3. */
+int lamh_tone_map(const float *rgb_in, uint32_t w, uint32_t h, float y_mid, float *rgb_out);
+/* rendering::rgb24 + lodepng::encode (include/renderer.hpp:20-29,
+ * src/renderer.cpp:187-190): sRGB OETF, 8-bit quantisation, RGB8 PNG. */
+int lamh_write_png(const char *path, const float *rgb, uint32_t w, uint32_t h);
+/* Renderer::saveImage file name rule (src/renderer.cpp:182-185). */
+int lamh_image_name(const char *scene_path, uint32_t spp, uint32_t bounces, float p, const char *suffix,
+                    char *out, size_t outlen);
+
+/* BVH8 statistics without a GPU (tests): nodes, leaves, depth, stack
+ * bound, and the conservativeness of every quantized child box (returns 1
+ * in *conservative when every decoded box contains its exact box). */
+int lamh_bvh8_check(const float *tri_v, uint32_t n_tris, lam_scene_stats *stats, int *conservative);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAMBDA_HOST_H */

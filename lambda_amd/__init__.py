@@ -1,0 +1,467 @@
+"""Python binding of the MI355X render loop of lambda (ctypes over the C ABI).
+
+The C ABI is declared in include/lambda_amd.h (device side) and
+include/lambda_host.h (loader, tone map, PNG).  This module mirrors the
+reference's Renderer interface (include/renderer.hpp:32-52):
+
+    r = Renderer(); r.SPP = 16; r.MAX_SCATTERINGS = 8; r.SAMPLE_P = 0.8
+    scene = load_nls("scenes/box.nls")
+    raw = r.render(scene)          # Renderer::render  -> linear RGB image
+    img = r.toneMap(raw, 0.12)     # Renderer::toneMap
+    r.saveImage(raw, "scenes/box.nls", "-raw")
+
+There is no CPU fallback: every call goes to liblambda_amd.so, which must be
+built (lambda_amd/Makefile, or __graft_entry__.build()); a missing library
+raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblambda_amd.so")
+DATA_DIR = os.path.join(_HERE, "data")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+LAM_OK = 0
+LAM_E_INVALID = -1
+LAM_E_DEVICE = -2
+LAM_E_NOSCENE = -3
+LAM_E_LIMIT = -4
+LAM_E_IO = -5
+LAM_E_PARSE = -6
+LAM_E_BVH = -7
+LAM_NONE = 0xFFFFFFFF
+
+LAM_FLAG_MEGAKERNEL = 0x1
+LAM_FLAG_NO_GRAPH = 0x2
+LAM_FLAG_PROFILE = 0x4
+
+OBJ_SPHERE, OBJ_PLANE, OBJ_TRIMESH = 1, 2, 3
+(MAT_NONE, MAT_LAMBERTIAN, MAT_BLINN, MAT_THINFILM, MAT_MICROFACET, MAT_EMITTER,
+ MAT_DIREMITTER) = range(7)
+LIGHT_AMBIENT, LIGHT_POINT, LIGHT_DIR, LIGHT_SPHERE, LIGHT_QUADDIR = range(5)
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("tmax", "<f4"), ("d", "<f4", 3), ("skip", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("tri", "<u4"), ("obj", "<u4"), ("prim", "<u4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+
+
+class LambdaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+# ---- C structures (layout of include/lambda_amd.h) -------------------------
+class Grid(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("inf", C.c_float), ("sup", C.c_float)]
+
+
+class Xform(C.Structure):
+    _fields_ = [("M", C.c_float * 12), ("Minv", C.c_float * 12)]
+
+
+class Object(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("material", C.c_uint32), ("mesh", C.c_uint32),
+                ("pad", C.c_uint32), ("T", Xform)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("spec", C.c_int32 * 5), ("alpha", C.c_float),
+                ("d", C.c_float), ("p_d", C.c_float), ("p_spec", C.c_float), ("w_r", C.c_float),
+                ("w_t", C.c_float), ("specular", C.c_float), ("pad", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("object", C.c_uint32), ("spec", C.c_int32),
+                ("size", C.c_float), ("inv_area", C.c_float), ("specular", C.c_float),
+                ("pos", C.c_float * 3), ("dir", C.c_float * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("x", C.c_float * 3), ("y", C.c_float * 3),
+                ("z", C.c_float * 3), ("D", C.c_float), ("delta", C.c_float), ("w", C.c_float),
+                ("h", C.c_float), ("dof", C.c_float), ("fov", C.c_float),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("grid", Grid), ("n_spectra", C.c_uint32), ("spectra", C.POINTER(C.c_float)),
+                ("cmf", C.POINTER(C.c_float)), ("n_objects", C.c_uint32),
+                ("objects", C.POINTER(Object)), ("n_materials", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("n_lights", C.c_uint32),
+                ("lights", C.POINTER(Light)), ("n_tris", C.c_uint32),
+                ("tri_v", C.POINTER(C.c_float)), ("tri_n", C.POINTER(C.c_float)),
+                ("tri_gn", C.POINTER(C.c_float)), ("tri_obj", C.POINTER(C.c_uint32)),
+                ("tri_prim", C.POINTER(C.c_uint32)), ("camera", Camera)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_scatterings", C.c_uint32), ("sample_p", C.c_float),
+                ("hero", C.c_uint32), ("seed", C.c_uint64), ("spp_offset", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("n_leaves", C.c_uint32), ("n_tris", C.c_uint32),
+                ("depth", C.c_uint32), ("stack_bound", C.c_uint32), ("n_analytic", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("sah_cost", C.c_double)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("extend_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
+                ("raygen_ms", C.c_double), ("film_ms", C.c_double), ("other_ms", C.c_double),
+                ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
+                ("shade_launches", C.c_uint64), ("extend_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("render_ms", C.c_double)]
+
+
+def params(spp=4, max_scatterings=4, sample_p=0.5, hero=16, seed=0, spp_offset=0, flags=0):
+    """lam_render_params with the reference's Renderer defaults
+    (include/renderer.hpp:35-37)."""
+    return RenderParams(int(spp), int(max_scatterings), float(sample_p), int(hero), int(seed),
+                        int(spp_offset), int(flags))
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded liblambda_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C lambda_amd` "
+                          "or __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u32p, fp = C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_float)
+    sig = {
+        "lam_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+        "lam_destroy": (None, [vp]),
+        "lam_last_error": (C.c_char_p, [vp]),
+        "lam_version": (C.c_char_p, []),
+        "lam_scene_upload": (C.c_int, [vp, vp]),
+        "lam_get_scene_stats": (C.c_int, [vp, C.POINTER(SceneStats)]),
+        "lam_render": (C.c_int, [vp, C.POINTER(RenderParams), fp, fp]),
+        "lam_trace_paths": (C.c_int, [vp, C.POINTER(RenderParams), u32p, u32p, sz, fp]),
+        "lam_extend": (C.c_int, [vp, vp, sz, vp]),
+        "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
+        "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
+        "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
+        "lam_reset_timing": (C.c_int, [vp]),
+        "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
+        "lamh_load_nls": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
+        "lamh_load_nls_text": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
+        "lamh_scene_free": (None, [vp]),
+        "lamh_scene_desc": (C.POINTER(SceneDesc), [vp]),
+        "lamh_scene_hero": (C.c_uint32, [vp]),
+        "lamh_object_name": (C.c_char_p, [vp, C.c_uint32]),
+        "lamh_material_name": (C.c_char_p, [vp, C.c_uint32]),
+        "lamh_tone_map": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_float, fp]),
+        "lamh_write_png": (C.c_int, [C.c_char_p, fp, C.c_uint32, C.c_uint32]),
+        "lamh_image_name": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p,
+                                      C.c_char_p, sz]),
+        "lamh_bvh8_check": (C.c_int, [fp, C.c_uint32, C.POINTER(SceneStats), C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def declared_functions():
+    """Names of every function declared in include/*.h (for the export test)."""
+    names = []
+    for h in sorted(os.listdir(INCLUDE_DIR)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE_DIR, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(lamh?_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _carr(ptr, n, dtype):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+
+class Scene:
+    """A loaded .nls scene (nl::cg::scene) and its flattened description."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self._lib = lib()
+
+    @classmethod
+    def load(cls, path, data_dir=None):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.lamh_load_nls(str(path).encode(), (data_dir or DATA_DIR).encode(), C.byref(h), err, 1024)
+        if rc != LAM_OK:
+            raise LambdaError(rc, err.value.decode())
+        return cls(h.value)
+
+    @classmethod
+    def from_text(cls, text, data_dir=None):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.lamh_load_nls_text(text.encode(), (data_dir or DATA_DIR).encode(), C.byref(h), err, 1024)
+        if rc != LAM_OK:
+            raise LambdaError(rc, err.value.decode())
+        return cls(h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.lamh_scene_free(h)
+            self._h = C.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def desc_ptr(self):
+        return self._lib.lamh_scene_desc(self._h)
+
+    @property
+    def desc(self):
+        return self.desc_ptr.contents
+
+    @property
+    def hero(self):
+        return int(self._lib.lamh_scene_hero(self._h))
+
+    @property
+    def width(self):
+        return int(self.desc.camera.width)
+
+    @property
+    def height(self):
+        return int(self.desc.camera.height)
+
+    @property
+    def n_tris(self):
+        return int(self.desc.n_tris)
+
+    def tri_v(self):
+        d = self.desc
+        return _carr(d.tri_v, 9 * d.n_tris, np.float32).reshape(-1, 3, 3)
+
+    def tri_obj(self):
+        d = self.desc
+        return _carr(d.tri_obj, d.n_tris, np.uint32)
+
+    def tri_prim(self):
+        d = self.desc
+        return _carr(d.tri_prim, d.n_tris, np.uint32)
+
+    def objects(self):
+        d = self.desc
+        return [d.objects[i] for i in range(d.n_objects)]
+
+    def materials(self):
+        d = self.desc
+        return [d.materials[i] for i in range(d.n_materials)]
+
+    def lights(self):
+        d = self.desc
+        return [d.lights[i] for i in range(d.n_lights)]
+
+    def spectra(self):
+        d = self.desc
+        return _carr(d.spectra, d.n_spectra * d.grid.n, np.float32).reshape(d.n_spectra, d.grid.n)
+
+    def cmf(self):
+        d = self.desc
+        return _carr(d.cmf, 3 * d.grid.n, np.float32).reshape(3, d.grid.n)
+
+    def object_name(self, i):
+        v = self._lib.lamh_object_name(self._h, i)
+        return None if v is None else v.decode()
+
+    def material_name(self, i):
+        v = self._lib.lamh_material_name(self._h, i)
+        return None if v is None else v.decode()
+
+
+def load_nls(path, data_dir=None):
+    """nl::cg::load::loadNLS (lib/numli/cg.hpp:2455-2470)."""
+    return Scene.load(path, data_dir)
+
+
+class Context:
+    """One lam_ctx: a HIP stream and the HBM copy of one scene."""
+
+    def __init__(self, device=0):
+        self._lib = lib()
+        h = C.c_void_p()
+        rc = self._lib.lam_create(C.byref(h), int(device))
+        if rc != LAM_OK:
+            raise LambdaError(rc, "lam_create failed (no GPU or HIP runtime error)")
+        self._h = h
+        self.scene = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lam_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc):
+        if rc != LAM_OK:
+            raise LambdaError(rc, self._lib.lam_last_error(self._h).decode())
+
+    def upload(self, scene):
+        self._check(self._lib.lam_scene_upload(self._h, C.cast(scene.desc_ptr, C.c_void_p)))
+        self.scene = scene
+
+    def stats(self):
+        s = SceneStats()
+        self._check(self._lib.lam_get_scene_stats(self._h, C.byref(s)))
+        return s
+
+    def extend(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        self._check(self._lib.lam_extend(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
+        return hits
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        out = np.zeros(len(rays), dtype=np.uint32)
+        self._check(self._lib.lam_occluded(self._h, rays.ctypes.data, len(rays), _u32ptr(out)))
+        return out
+
+    def extend_dev(self, rays_ptr, n, hits_ptr, counts_ptr=None):
+        self._check(self._lib.lam_extend_dev(self._h, rays_ptr, n, hits_ptr, counts_ptr))
+
+    def trace_paths(self, p, pixel, sample):
+        pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
+        sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        out = np.zeros((len(pixel), p.hero), dtype=np.float32)
+        self._check(self._lib.lam_trace_paths(self._h, C.byref(p), _u32ptr(pixel), _u32ptr(sample),
+                                              len(pixel), _fptr(out)))
+        return out
+
+    def render(self, p, want_xyz=False):
+        d = self.scene.desc
+        w, h = d.camera.width, d.camera.height
+        rgb = np.zeros((h, w, 3), dtype=np.float32)
+        xyz = np.zeros((h, w, 3), dtype=np.float32) if want_xyz else None
+        self._check(self._lib.lam_render(self._h, C.byref(p), _fptr(rgb),
+                                         _fptr(xyz) if want_xyz else None))
+        return (rgb, xyz) if want_xyz else rgb
+
+    def timing(self):
+        t = Timing()
+        self._check(self._lib.lam_get_timing(self._h, C.byref(t)))
+        return t
+
+    def reset_timing(self):
+        self._check(self._lib.lam_reset_timing(self._h))
+
+    def debug_math(self, fn, a, b=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        b = np.ascontiguousarray(a if b is None else b, dtype=np.float32)
+        out = np.zeros_like(a)
+        self._check(self._lib.lam_debug_math(self._h, int(fn), _fptr(a), _fptr(b), len(a), _fptr(out)))
+        return out
+
+
+def tone_map(rgb, y_mid=0.12):
+    """Renderer::toneMap (src/renderer.cpp:149-176)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros_like(rgb)
+    h, w = rgb.shape[:2]
+    rc = lib().lamh_tone_map(_fptr(rgb), w, h, float(y_mid), _fptr(out))
+    if rc != LAM_OK:
+        raise LambdaError(rc, "tone map failed")
+    return out
+
+
+def write_png(path, rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    rc = lib().lamh_write_png(str(path).encode(), _fptr(rgb), w, h)
+    if rc != LAM_OK:
+        raise LambdaError(rc, f"could not write {path}")
+
+
+def image_name(scene_path, spp, bounces, p, suffix=""):
+    out = C.create_string_buffer(4096)
+    rc = lib().lamh_image_name(str(scene_path).encode(), spp, bounces, float(p), suffix.encode(), out, 4096)
+    if rc != LAM_OK:
+        raise LambdaError(rc, f"no image name for {scene_path}")
+    return out.value.decode()
+
+
+class Renderer:
+    """The reference's Renderer (include/renderer.hpp:32-52) over the GPU.
+
+    tracePath is keyed by (pixel, sample) instead of (λ, ray, RNG&): every
+    path draws its random numbers from a counter-based RNG addressed by
+    (seed, pixel, sample, dimension)."""
+
+    def __init__(self, device=0):
+        self.MAX_SCATTERINGS = 4
+        self.SPP = 4
+        self.SAMPLE_P = 0.5
+        self.SEED = 0
+        self.FLAGS = 0
+        self._ctx = Context(device)
+        self._scene = None
+
+    def _params(self, scene):
+        return params(self.SPP, self.MAX_SCATTERINGS, self.SAMPLE_P, scene.hero, self.SEED, 0, self.FLAGS)
+
+    def _upload(self, scene):
+        if self._scene is not scene:
+            self._ctx.upload(scene)
+            self._scene = scene
+
+    def render(self, scene):
+        self._upload(scene)
+        return self._ctx.render(self._params(scene))
+
+    def toneMap(self, buffer, Y_MID=0.12):
+        return tone_map(buffer, Y_MID)
+
+    def tracePath(self, scene, pixel, sample):
+        self._upload(scene)
+        return self._ctx.trace_paths(self._params(scene), [pixel], [sample])[0]
+
+    def saveImage(self, buffer, fpath, suffix=""):
+        write_png(image_name(fpath, self.SPP, self.MAX_SCATTERINGS, self.SAMPLE_P, suffix), buffer)

@@ -1,0 +1,418 @@
+// C-ABI runtime: context, scene upload (BVH8 build + HBM layout), render,
+// path and ray test hooks, timing.  See include/lambda_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host/bvh8.hpp"
+#include "lam_launch.h"
+#include "lambda_amd.h"
+
+struct lam_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool has_scene = false;
+  lam::DevScene sc{};
+  lam_scene_stats stats{};
+  std::vector<void *> scene_bufs;
+  uint32_t *d_err = nullptr;  // 16 B: [0] traversal-stack overflow flag
+  // grow-only scratch for the test hooks and the film
+  void *scratch[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t scratch_size[6] = {0, 0, 0, 0, 0, 0};
+  lam_timing timing{};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+int fail(lam_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPC(call)                                                                             \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(ctx, LAM_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+int ensure_scratch(lam_ctx *ctx, int slot, size_t bytes) {
+  if (ctx->scratch_size[slot] >= bytes) return LAM_OK;
+  if (ctx->scratch[slot]) HIPC(hipFree(ctx->scratch[slot]));
+  ctx->scratch[slot] = nullptr;
+  ctx->scratch_size[slot] = 0;
+  HIPC(hipMalloc(&ctx->scratch[slot], bytes));
+  ctx->scratch_size[slot] = bytes;
+  return LAM_OK;
+}
+
+void free_scene(lam_ctx *ctx) {
+  for (void *p : ctx->scene_bufs) (void)hipFree(p);
+  ctx->scene_bufs.clear();
+  ctx->has_scene = false;
+  ctx->sc = lam::DevScene{};
+}
+
+template <typename T>
+int upload(lam_ctx *ctx, const T *host, size_t count, const T **dev) {
+  *dev = nullptr;
+  if (count == 0) return LAM_OK;
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, count * sizeof(T)));
+  ctx->scene_bufs.push_back(p);
+  HIPC(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  *dev = static_cast<const T *>(p);
+  ctx->stats.device_bytes += count * sizeof(T);
+  return LAM_OK;
+}
+
+int check_err_flag(lam_ctx *ctx) {
+  uint32_t flag = 0;
+  HIPC(hipMemcpy(&flag, ctx->d_err, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(ctx, LAM_E_BVH, "BVH traversal stack overflow");
+  return LAM_OK;
+}
+
+int clear_err_flag(lam_ctx *ctx) {
+  HIPC(lam::launch_zero(ctx->d_err, 16, ctx->stream));
+  return LAM_OK;
+}
+
+double event_ms(lam_ctx *ctx) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) != hipSuccess) return 0.0;
+  return double(ms);
+}
+
+lam::PathArgs path_args(const lam_render_params *p) {
+  lam::PathArgs pa;
+  pa.seed = p->seed;
+  pa.max_scat = p->max_scatterings;
+  pa.sample_p = p->sample_p;
+  return pa;
+}
+
+int check_params(lam_ctx *ctx, const lam_render_params *p) {
+  if (!p) return fail(ctx, LAM_E_INVALID, "params is NULL");
+  if (p->hero != 4 && p->hero != 16) return fail(ctx, LAM_E_INVALID, "hero must be 4 or 16");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (ctx->sc.n_lights == 0) return fail(ctx, LAM_E_INVALID, "scene has no lights");
+  return LAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lam_version(void) { return "lambda_amd 0.1 (gfx950)"; }
+
+int lam_create(lam_ctx **out, int device) {
+  if (!out) return LAM_E_INVALID;
+  *out = nullptr;
+  lam_ctx *ctx = new lam_ctx();
+  ctx->device = device;
+  int rc = LAM_OK;
+  do {
+    if (hipSetDevice(device) != hipSuccess) { rc = LAM_E_DEVICE; break; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = LAM_E_DEVICE; break; }
+    if (hipMalloc(reinterpret_cast<void **>(&ctx->d_err), 16) != hipSuccess) { rc = LAM_E_DEVICE; break; }
+    if (hipMemset(ctx->d_err, 0, 16) != hipSuccess) { rc = LAM_E_DEVICE; break; }
+    if (hipEventCreate(&ctx->ev[0]) != hipSuccess || hipEventCreate(&ctx->ev[1]) != hipSuccess) {
+      rc = LAM_E_DEVICE;
+      break;
+    }
+  } while (false);
+  if (rc != LAM_OK) {
+    lam_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return LAM_OK;
+}
+
+void lam_destroy(lam_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_scene(ctx);
+  for (int i = 0; i < 6; i++)
+    if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  for (int i = 0; i < 2; i++)
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *lam_last_error(const lam_ctx *ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!d) return fail(ctx, LAM_E_INVALID, "scene description is NULL");
+  HIPC(hipSetDevice(ctx->device));
+  const uint32_t gn = d->grid.n;
+  if (gn < 2 || !(d->grid.sup > d->grid.inf)) return fail(ctx, LAM_E_INVALID, "bad spectral grid");
+  if (!d->cmf) return fail(ctx, LAM_E_INVALID, "cmf table missing");
+  if (d->n_tris && (!d->tri_v || !d->tri_n || !d->tri_gn || !d->tri_obj || !d->tri_prim))
+    return fail(ctx, LAM_E_INVALID, "triangle arrays missing");
+  if (d->n_objects == 0 || !d->objects) return fail(ctx, LAM_E_INVALID, "scene has no objects");
+  if (d->n_materials == 0 || !d->materials) return fail(ctx, LAM_E_INVALID, "scene has no materials");
+  if (d->n_lights == 0 || !d->lights) return fail(ctx, LAM_E_INVALID, "scene has no lights");
+  if (d->camera.width == 0 || d->camera.height == 0) return fail(ctx, LAM_E_INVALID, "camera has zero size");
+  for (uint32_t i = 0; i < d->n_objects; i++) {
+    const lam_object &o = d->objects[i];
+    if (o.type < LAM_OBJ_SPHERE || o.type > LAM_OBJ_TRIMESH) return fail(ctx, LAM_E_INVALID, "bad object type");
+    if (o.material >= d->n_materials) return fail(ctx, LAM_E_INVALID, "object material out of range");
+  }
+  auto spec_ok = [&](int32_t s) { return s >= 0 && uint32_t(s) < d->n_spectra; };
+  for (uint32_t i = 0; i < d->n_materials; i++) {
+    const lam_material &m = d->materials[i];
+    int need = 0;
+    switch (m.type) {
+      case LAM_MAT_LAMBERTIAN: need = 1; break;
+      case LAM_MAT_BLINN: need = 5; break;
+      case LAM_MAT_THINFILM: need = 2; break;
+      case LAM_MAT_EMITTER: case LAM_MAT_DIREMITTER: need = 1; break;
+      case LAM_MAT_NONE: case LAM_MAT_MICROFACET: need = 0; break;
+      default: return fail(ctx, LAM_E_INVALID, "bad material type");
+    }
+    for (int k = 0; k < need; k++)
+      if (!spec_ok(m.spec[k])) return fail(ctx, LAM_E_INVALID, "material spectrum index out of range");
+  }
+  for (uint32_t i = 0; i < d->n_lights; i++) {
+    const lam_light &l = d->lights[i];
+    if (l.type > LAM_LIGHT_QUADDIR) return fail(ctx, LAM_E_INVALID, "bad light type");
+    if (!spec_ok(l.spec)) return fail(ctx, LAM_E_INVALID, "light spectrum index out of range");
+    if (l.type == LAM_LIGHT_SPHERE || l.type == LAM_LIGHT_QUADDIR) {
+      if (l.object >= d->n_objects) return fail(ctx, LAM_E_INVALID, "light object out of range");
+      const uint32_t want = l.type == LAM_LIGHT_SPHERE ? LAM_OBJ_SPHERE : LAM_OBJ_PLANE;
+      if (d->objects[l.object].type != want) return fail(ctx, LAM_E_INVALID, "light object has the wrong type");
+    }
+  }
+  for (uint32_t t = 0; t < d->n_tris; t++)
+    if (d->tri_obj[t] >= d->n_objects || d->objects[d->tri_obj[t]].type != LAM_OBJ_TRIMESH)
+      return fail(ctx, LAM_E_INVALID, "triangle object id invalid");
+
+  free_scene(ctx);
+  ctx->stats = lam_scene_stats{};
+
+  lamh::Bvh8 bvh;
+  if (!lamh::build_bvh8(d->tri_v, d->n_tris, bvh)) return fail(ctx, LAM_E_LIMIT, "too many triangles");
+  if (bvh.stack_bound > uint32_t(lam::kStack))
+    return fail(ctx, LAM_E_BVH, "BVH needs a traversal stack of " + std::to_string(bvh.stack_bound) +
+                                    " entries (limit " + std::to_string(lam::kStack) + ")");
+  const uint32_t nt = d->n_tris;
+  std::vector<float> tri(size_t(nt) * 12), shd(size_t(nt) * 12);
+  for (uint32_t s = 0; s < nt; s++) {
+    const uint32_t o = bvh.order[s];
+    std::memcpy(&tri[size_t(s) * 12], d->tri_v + size_t(o) * 9, 36);
+    uint32_t ids[3] = {d->tri_obj[o], d->tri_prim[o], o};
+    std::memcpy(&tri[size_t(s) * 12 + 9], ids, 12);
+    std::memcpy(&shd[size_t(s) * 12], d->tri_n + size_t(o) * 9, 36);
+    std::memcpy(&shd[size_t(s) * 12 + 9], d->tri_gn + size_t(o) * 3, 12);
+  }
+  std::vector<uint32_t> analytic;
+  for (uint32_t i = 0; i < d->n_objects; i++)
+    if (d->objects[i].type != LAM_OBJ_TRIMESH) analytic.push_back(i);
+
+  lam::DevScene &sc = ctx->sc;
+  int rc;
+  static_assert(sizeof(lamh::Node8) == sizeof(lam::Node8d), "node layout");
+  const lamh::Node8 *nodes_dev = nullptr;
+  if ((rc = upload(ctx, bvh.nodes.data(), bvh.nodes.size(), &nodes_dev))) return rc;
+  sc.nodes = reinterpret_cast<const lam::Node8d *>(nodes_dev);
+  const float *tri_dev = nullptr, *shd_dev = nullptr;
+  if ((rc = upload(ctx, tri.data(), tri.size(), &tri_dev))) return rc;
+  if ((rc = upload(ctx, shd.data(), shd.size(), &shd_dev))) return rc;
+  sc.tris = reinterpret_cast<const float4 *>(tri_dev);
+  sc.shade = reinterpret_cast<const float4 *>(shd_dev);
+  if ((rc = upload(ctx, analytic.data(), analytic.size(), &sc.analytic))) return rc;
+  if ((rc = upload(ctx, d->objects, d->n_objects, &sc.objects))) return rc;
+  if ((rc = upload(ctx, d->materials, d->n_materials, &sc.materials))) return rc;
+  if ((rc = upload(ctx, d->lights, d->n_lights, &sc.lights))) return rc;
+  if ((rc = upload(ctx, d->spectra, size_t(d->n_spectra) * gn, &sc.spectra))) return rc;
+  if ((rc = upload(ctx, d->cmf, size_t(3) * gn, &sc.cmf))) return rc;
+  sc.n_nodes = uint32_t(bvh.nodes.size());
+  sc.n_tris = nt;
+  sc.n_analytic = uint32_t(analytic.size());
+  sc.n_objects = d->n_objects;
+  sc.n_materials = d->n_materials;
+  sc.n_lights = d->n_lights;
+  sc.grid_n = gn;
+  sc.grid_inf = d->grid.inf;
+  sc.grid_sup = d->grid.sup;
+  sc.grid_delta = (d->grid.sup - d->grid.inf) / float(gn - 1);
+  sc.cam = d->camera;
+  ctx->stats.n_nodes = sc.n_nodes;
+  ctx->stats.n_leaves = bvh.n_leaves;
+  ctx->stats.n_tris = nt;
+  ctx->stats.depth = bvh.depth;
+  ctx->stats.stack_bound = bvh.stack_bound;
+  ctx->stats.n_analytic = sc.n_analytic;
+  ctx->stats.sah_cost = bvh.sah_cost;
+  ctx->has_scene = true;
+  return LAM_OK;
+}
+
+int lam_get_scene_stats(const lam_ctx *ctx, lam_scene_stats *out) {
+  if (!ctx || !out) return LAM_E_INVALID;
+  if (!ctx->has_scene) return LAM_E_NOSCENE;
+  *out = ctx->stats;
+  return LAM_OK;
+}
+
+static int trace_rays_host(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit *hits, uint32_t *occl) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n && (!rays || (!hits && !occl))) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (n == 0) return LAM_OK;
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_scratch(ctx, 0, n * sizeof(lam_ray)))) return rc;
+  if ((rc = ensure_scratch(ctx, 1, n * sizeof(lam_hit)))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[0], rays, n * sizeof(lam_ray), hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPC(lam::launch_trace_rays(ctx->sc, static_cast<lam_ray *>(ctx->scratch[0]), n,
+                              occl ? nullptr : static_cast<lam_hit *>(ctx->scratch[1]),
+                              occl ? static_cast<uint32_t *>(ctx->scratch[1]) : nullptr, nullptr, ctx->d_err,
+                              ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  if (occl)
+    HIPC(hipMemcpyAsync(occl, ctx->scratch[1], n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  else
+    HIPC(hipMemcpyAsync(hits, ctx->scratch[1], n * sizeof(lam_hit), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  const double ms = event_ms(ctx);
+  if (occl) { ctx->timing.shadow_ms += ms; ctx->timing.shadow_launches++; ctx->timing.shadow_rays += n; }
+  else { ctx->timing.extend_ms += ms; ctx->timing.extend_launches++; ctx->timing.extend_rays += n; }
+  return check_err_flag(ctx);
+}
+
+int lam_extend(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit *hits) {
+  return trace_rays_host(ctx, rays, n, hits, nullptr);
+}
+
+int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n, uint32_t *occluded) {
+  if (ctx && n && !occluded) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  return trace_rays_host(ctx, rays, n, nullptr, occluded);
+}
+
+int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hits, uint32_t *d_counts) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n && (!d_rays || !d_hits)) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (n == 0) return LAM_OK;
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, d_counts, ctx->d_err, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->timing.extend_ms += event_ms(ctx);
+  ctx->timing.extend_launches++;
+  ctx->timing.extend_rays += n;
+  return check_err_flag(ctx);
+}
+
+int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pixel, const uint32_t *sample,
+                    size_t n, float *out) {
+  if (!ctx) return LAM_E_INVALID;
+  int rc;
+  if ((rc = check_params(ctx, p))) return rc;
+  if (n == 0) return LAM_OK;
+  if (!pixel || !sample || !out) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (n > (size_t(1) << 28)) return fail(ctx, LAM_E_LIMIT, "path batch too large");
+  const uint64_t npix = uint64_t(ctx->sc.cam.width) * ctx->sc.cam.height;
+  for (size_t i = 0; i < n; i++)
+    if (pixel[i] >= npix) return fail(ctx, LAM_E_INVALID, "pixel index out of range");
+  HIPC(hipSetDevice(ctx->device));
+  const size_t H = p->hero;
+  if ((rc = ensure_scratch(ctx, 0, n * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 1, n * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 2, n * H * 4))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[0], pixel, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->scratch[1], sample, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(lam::launch_trace_paths(ctx->sc, path_args(p), int(H), static_cast<uint32_t *>(ctx->scratch[0]),
+                               static_cast<uint32_t *>(ctx->scratch[1]), n, static_cast<float *>(ctx->scratch[2]),
+                               ctx->d_err, ctx->stream));
+  HIPC(hipMemcpyAsync(out, ctx->scratch[2], n * H * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return check_err_flag(ctx);
+}
+
+int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *xyz) {
+  if (!ctx) return LAM_E_INVALID;
+  int rc;
+  if ((rc = check_params(ctx, p))) return rc;
+  if (!linrgb) return fail(ctx, LAM_E_INVALID, "NULL output buffer");
+  HIPC(hipSetDevice(ctx->device));
+  const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
+  const size_t npix = size_t(w) * h;
+  if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
+  if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;  // film: sum + compensation
+  if ((rc = ensure_scratch(ctx, 4, npix * 12))) return rc;  // linRGB
+  if ((rc = ensure_scratch(ctx, 5, npix * 12))) return rc;  // XYZ sums
+  float *film = static_cast<float *>(ctx->scratch[3]);
+  HIPC(lam::launch_zero(film, npix * 32, ctx->stream));
+  if ((rc = clear_err_flag(ctx))) return rc;
+  const lam::PathArgs pa = path_args(p);
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t s = 0; s < p->spp; s++)
+    HIPC(lam::launch_render_mega(ctx->sc, pa, int(p->hero), p->spp_offset + s, uint32_t(npix), film, ctx->d_err,
+                                 ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
+  HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
+                           xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
+  HIPC(hipMemcpyAsync(linrgb, ctx->scratch[4], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
+  if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->timing.render_ms = event_ms(ctx);
+  return check_err_flag(ctx);
+}
+
+int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n == 0) return LAM_OK;
+  if (!a || !b || !out) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (n > (size_t(1) << 28)) return fail(ctx, LAM_E_LIMIT, "batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_scratch(ctx, 0, n * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 1, n * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 2, n * 4))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[0], a, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->scratch[1], b, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(lam::launch_math(fn, static_cast<float *>(ctx->scratch[0]), static_cast<float *>(ctx->scratch[1]), n,
+                        static_cast<float *>(ctx->scratch[2]), ctx->stream));
+  HIPC(hipMemcpyAsync(out, ctx->scratch[2], n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return LAM_OK;
+}
+
+int lam_get_timing(const lam_ctx *ctx, lam_timing *out) {
+  if (!ctx || !out) return LAM_E_INVALID;
+  *out = ctx->timing;
+  return LAM_OK;
+}
+
+int lam_reset_timing(lam_ctx *ctx) {
+  if (!ctx) return LAM_E_INVALID;
+  ctx->timing = lam_timing{};
+  return LAM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,348 @@
+// Device-side scene view, vector helpers and the closest-hit / any-hit
+// traversal shared by the extend, shadow and path kernels.
+//
+// Evaluation order of every float expression restates the reference
+// (lib/numli/cg.hpp, bra.hpp) and the whole library is compiled with
+// -ffp-contract=off, so the CPU oracle computes the same bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lambda_amd.h"
+#include "lam_math.h"
+
+namespace lam {
+
+// ---- constants ------------------------------------------------------------
+// intersect::BIAS = constexprSqrt(ε<float>) (cg.hpp:1189, numli.hpp:69-70):
+// the Newton iteration converges to this float.
+constexpr float kBias = 0x1.6a09e6p-12f;
+constexpr float kEps = 1.1920928955078125e-07f;  // ε<float>
+constexpr float kFltMax = 3.40282346638528859812e+38f;
+constexpr float kInvPi = 0.318309886183790671538f;  // inv_π<float>
+constexpr float kPi = 3.14159265358979323846f;
+constexpr int kStack = 64;  // traversal stack entries per lane (LDS)
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// ---- small vector type with the reference's operator semantics -----------
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float a, float b, float c) { return f3{a, b, c}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 neg(f3 a) { return f3{a.x * -1.f, a.y * -1.f, a.z * -1.f}; }
+__device__ __forceinline__ f3 mul(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len2(f3 a) {
+  float s = 0.f;
+  s += a.x * a.x;
+  s += a.y * a.y;
+  s += a.z * a.z;
+  return s;
+}
+__device__ __forceinline__ f3 normalized(f3 a) {
+  const float l = sqrtf(len2(a));
+  return f3{a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ f3 divs(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float fmax_ref(float a, float b) { return a >= b ? a : b; }
+__device__ __forceinline__ float fmin_ref(float a, float b) { return a <= b ? a : b; }
+
+// M (3x4 rows) * (v, w) with a sequential k-sum (bra.hpp:325-336)
+__device__ __forceinline__ f3 xmul(const float *M, f3 v, float w) {
+  float r[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float s = 0.f;
+    s += M[i * 4 + 0] * v.x;
+    s += M[i * 4 + 1] * v.y;
+    s += M[i * 4 + 2] * v.z;
+    s += M[i * 4 + 3] * w;
+    r[i] = s;
+  }
+  return f3{r[0], r[1], r[2]};
+}
+// subMatT<3,3>(M_inv) * n (cg.hpp:442-443)
+__device__ __forceinline__ f3 normal_to_world(const float *Minv, f3 n) {
+  float r[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float s = 0.f;
+    s += Minv[0 * 4 + i] * n.x;
+    s += Minv[1 * 4 + i] * n.y;
+    s += Minv[2 * 4 + i] * n.z;
+    r[i] = s;
+  }
+  return f3{r[0], r[1], r[2]};
+}
+
+// ---- device scene -----------------------------------------------------------
+struct Node8d {  // 128 B, see host/bvh8.hpp
+  float ox, oy, oz;
+  uint32_t e_n;  // ex | ey<<8 | ez<<16 | nchild<<24
+  uint32_t child[8];
+  uint32_t qlo[6];  // 24 B: x[8], y[8], z[8]
+  uint32_t qhi[6];
+  uint32_t pad[8];
+};
+static_assert(sizeof(Node8d) == 128, "node record");
+
+struct DevScene {
+  const Node8d *nodes;
+  const float4 *tris;   // 3 float4 per triangle slot: v0 v1 v2 | obj prim orig
+  const float4 *shade;  // 3 float4 per slot: n0 n1 n2 | gn
+  const uint32_t *analytic;  // object ids of spheres and planes
+  const lam_object *objects;
+  const lam_material *materials;
+  const lam_light *lights;
+  const float *spectra;
+  const float *cmf;
+  uint32_t n_nodes, n_tris, n_analytic, n_objects, n_materials, n_lights;
+  uint32_t grid_n;
+  float grid_inf, grid_sup, grid_delta;
+  lam_camera cam;
+};
+
+struct Ray {
+  f3 o, d;
+  float tmax;
+  uint32_t skip;
+};
+
+struct Hit {
+  float t;
+  uint32_t slot;  // triangle slot (BVH order) or LAM_NONE
+  uint32_t obj;
+  uint32_t prim;
+};
+
+__device__ __forceinline__ bool better(float t, uint32_t obj, uint32_t prim, const Hit &b) {
+  return t < b.t || (t == b.t && (obj < b.obj || (obj == b.obj && prim < b.prim)));
+}
+
+// ---- primitive tests -------------------------------------------------------
+
+// intersect::triangle, Plücker form (cg.hpp:1341-1379).  Returns t or a
+// negative value on a miss; the t-window test is done by the caller.
+__device__ __forceinline__ float tri_t(f3 v0, f3 v1, f3 v2, f3 o, f3 u, f3 mr) {
+  const f3 e0 = sub(v1, v0);
+  const f3 e2 = sub(v0, v2);
+  const f3 n = cross(e2, e0);
+  const float D = dot(u, n);
+  if (fabsf(D) < kEps) return -1.f;
+  const f3 m0 = cross(v0, e0);
+  const float s0 = dot(u, m0) + dot(e0, mr);
+  if (s0 * D < 0.f) return -1.f;
+  const f3 e1 = sub(v2, v1);
+  const f3 m1 = cross(v1, e1);
+  const float s1 = dot(u, m1) + dot(e1, mr);
+  if (s1 * D < 0.f) return -1.f;
+  const f3 m2 = cross(v2, e2);
+  const float s2 = dot(u, m2) + dot(e2, mr);
+  if (s2 * D < 0.f) return -1.f;
+  if (fabsf(s0 + s1 + s2) < kEps) return -1.f;
+  const float inv_D = 1.f / D;
+  const float t = dot(sub(v0, o), n) * inv_D;
+  return t;
+}
+// barycentrics of the same test, b = (s1, s2, s0)/D (cg.hpp:1371)
+__device__ __forceinline__ f3 tri_bary(f3 v0, f3 v1, f3 v2, f3 o, f3 u) {
+  const f3 mr = cross(o, u);
+  const f3 e0 = sub(v1, v0);
+  const f3 e2 = sub(v0, v2);
+  const f3 n = cross(e2, e0);
+  const float D = dot(u, n);
+  const float s0 = dot(u, cross(v0, e0)) + dot(e0, mr);
+  const f3 e1 = sub(v2, v1);
+  const float s1 = dot(u, cross(v1, e1)) + dot(e1, mr);
+  const float s2 = dot(u, cross(v2, e2)) + dot(e2, mr);
+  const float inv_D = 1.f / D;
+  return f3{s1 * inv_D, s2 * inv_D, s0 * inv_D};
+}
+
+// ray.toLocal (cg.hpp:438-439)
+__device__ __forceinline__ void to_local(const lam_object &ob, f3 o, f3 u, f3 &lo, f3 &lu) {
+  lo = xmul(ob.T.Minv, o, 1.f);
+  lu = xmul(ob.T.Minv, u, 0.f);
+}
+
+// intersect::sphere t (cg.hpp:1196-1214); negative on miss
+__device__ __forceinline__ float sphere_t(f3 lo, f3 lu) {
+  const float a = dot(lu, lu);
+  const float b = 2 * dot(lo, lu);
+  const float c = dot(lo, lo) - 1.f;
+  const float disc = b * b - 4 * a * c;
+  if (disc < .1f * kBias) return -1.f;
+  const float inv_2a = .5f / a;
+  const float tp = (-b + sqrtf(disc)) * inv_2a;
+  const float tm = (-b - sqrtf(disc)) * inv_2a;
+  float t = tm;
+  if (tm < kBias) t = tp;
+  if (t < kBias) return -1.f;
+  return t;
+}
+// intersect::plane t (cg.hpp:1245-1252); negative on miss
+__device__ __forceinline__ float plane_t(f3 lo, f3 lu) {
+  if (lu.z == 0.f) return -1.f;
+  const float t = -lo.z / lu.z;
+  const f3 x = add(lo, mul(lu, t));
+  if (x.x < -1.f || x.x > 1.f || x.y < -1.f || x.y > 1.f || t < kBias) return -1.f;
+  return t;
+}
+
+__device__ __forceinline__ float analytic_t(const lam_object &ob, f3 o, f3 u) {
+  f3 lo, lu;
+  to_local(ob, o, u, lo, lu);
+  return ob.type == LAM_OBJ_SPHERE ? sphere_t(lo, lu) : plane_t(lo, lu);
+}
+
+// ---- BVH8 traversal ---------------------------------------------------------
+
+__device__ __forceinline__ float scale_of(uint32_t e) { return __uint_as_float(e << 23); }
+
+// robust slab-test widening (Ize 2013): 1 + 2*gamma(3), gamma(n)=n*2^-24/(1-n*2^-24)
+constexpr float kRobust = 1.0000003576278687f;
+
+__device__ __forceinline__ void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb) {
+  const bool s = tb < ta;
+  const float t0 = s ? tb : ta, t1 = s ? ta : tb;
+  const uint32_t c0 = s ? cb : ca, c1 = s ? ca : cb;
+  ta = t0; tb = t1; ca = c0; cb = c1;
+}
+
+struct TravCounts {
+  uint32_t nodes, tris;
+};
+
+// Closest hit (ANY=false) or any hit (ANY=true) for t in [BIAS, tmax],
+// objects `skip` excluded.  Ties go to the lowest (obj, prim).
+// `stk` points at this lane's LDS stack column: entry k at stk[k*64].
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, uint32_t *stk,
+                                     TravCounts &cnt, bool &overflow) {
+  Hit best;
+  best.t = r.tmax;
+  best.slot = LAM_NONE;
+  best.obj = LAM_NONE;
+  best.prim = LAM_NONE;
+  // spheres and planes: linear loop as intersect::scene (cg.hpp:1477-1487)
+  for (uint32_t a = 0; a < sc.n_analytic; a++) {
+    const uint32_t oi = sc.analytic[a];
+    if (oi == r.skip) continue;
+    const float t = analytic_t(sc.objects[oi], r.o, r.d);
+    if (t >= kBias && better(t, oi, 0u, best)) {
+      best.t = t;
+      best.slot = LAM_NONE;
+      best.obj = oi;
+      best.prim = 0;
+      if (ANY) return best;
+    }
+  }
+  if (sc.n_nodes == 0) return best;
+
+  // ray constructor inv_u (cg.hpp:357-361)
+  const float ix = r.d.x != 0.f ? 1.f / r.d.x : kFltMax;
+  const float iy = r.d.y != 0.f ? 1.f / r.d.y : kFltMax;
+  const float iz = r.d.z != 0.f ? 1.f / r.d.z : kFltMax;
+  const f3 mr = cross(r.o, r.d);
+  int sp = 0;
+  uint32_t cur = 0;  // root is always an interior record
+  while (true) {
+    if (!(cur & kLeafBit)) {
+      const Node8d *nd = sc.nodes + cur;
+      const float4 h0 = *reinterpret_cast<const float4 *>(&nd->ox);
+      const uint4 c0 = *reinterpret_cast<const uint4 *>(&nd->child[0]);
+      const uint4 c1 = *reinterpret_cast<const uint4 *>(&nd->child[4]);
+      const uint4 q0 = *reinterpret_cast<const uint4 *>(&nd->qlo[0]);
+      const uint4 q1 = *reinterpret_cast<const uint4 *>(&nd->qlo[4]);  // qlo[4..5], qhi[0..1]
+      const uint4 q2 = *reinterpret_cast<const uint4 *>(&nd->qhi[2]);  // qhi[2..5]
+      if (COUNT) cnt.nodes++;
+      const uint32_t en = __float_as_uint(h0.w);
+      const float sx = scale_of(en & 0xffu), sy = scale_of((en >> 8) & 0xffu),
+                  sz = scale_of((en >> 16) & 0xffu);
+      const uint32_t nchild = en >> 24;
+      const uint32_t qlo[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+      const uint32_t qhi[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+      const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      const float kInf = __builtin_inff();
+      float tk[8];
+      uint32_t ck[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int w = k >> 2, sh = (k & 3) * 8;
+        const float lx = h0.x + float((qlo[0 + w] >> sh) & 0xffu) * sx;
+        const float ly = h0.y + float((qlo[2 + w] >> sh) & 0xffu) * sy;
+        const float lz = h0.z + float((qlo[4 + w] >> sh) & 0xffu) * sz;
+        const float hx = h0.x + float((qhi[0 + w] >> sh) & 0xffu) * sx;
+        const float hy = h0.y + float((qhi[2 + w] >> sh) & 0xffu) * sy;
+        const float hz = h0.z + float((qhi[4 + w] >> sh) & 0xffu) * sz;
+        const float tx0 = (lx - r.o.x) * ix, tx1 = (hx - r.o.x) * ix;
+        const float ty0 = (ly - r.o.y) * iy, ty1 = (hy - r.o.y) * iy;
+        const float tz0 = (lz - r.o.z) * iz, tz1 = (hz - r.o.z) * iz;
+        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+        const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
+        const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobust);
+        tk[k] = hit ? fminf(tn, kFltMax) : kInf;
+        ck[k] = ch[k];
+      }
+      // sorting network for 8 (19 comparators), nearest first
+      cswap(tk[0], ck[0], tk[1], ck[1]); cswap(tk[2], ck[2], tk[3], ck[3]);
+      cswap(tk[4], ck[4], tk[5], ck[5]); cswap(tk[6], ck[6], tk[7], ck[7]);
+      cswap(tk[0], ck[0], tk[2], ck[2]); cswap(tk[1], ck[1], tk[3], ck[3]);
+      cswap(tk[4], ck[4], tk[6], ck[6]); cswap(tk[5], ck[5], tk[7], ck[7]);
+      cswap(tk[1], ck[1], tk[2], ck[2]); cswap(tk[5], ck[5], tk[6], ck[6]);
+      cswap(tk[0], ck[0], tk[4], ck[4]); cswap(tk[3], ck[3], tk[7], ck[7]);
+      cswap(tk[1], ck[1], tk[5], ck[5]); cswap(tk[2], ck[2], tk[6], ck[6]);
+      cswap(tk[1], ck[1], tk[4], ck[4]); cswap(tk[3], ck[3], tk[6], ck[6]);
+      cswap(tk[2], ck[2], tk[4], ck[4]); cswap(tk[3], ck[3], tk[5], ck[5]);
+      cswap(tk[3], ck[3], tk[4], ck[4]);
+      if (tk[0] != kInf) {
+        // push the hit children except the nearest, farthest first
+#pragma unroll
+        for (int k = 7; k >= 1; k--) {
+          if (tk[k] != kInf) {
+            if (sp >= kStack) { overflow = true; return best; }
+            stk[sp * 64] = ck[k];
+            sp++;
+          }
+        }
+        cur = ck[0];
+        continue;
+      }
+    } else {
+      const uint32_t first = cur & 0x07FFFFFFu;
+      const uint32_t count = ((cur >> 27) & 7u) + 1u;
+      for (uint32_t i = 0; i < count; i++) {
+        const uint32_t s = first + i;
+        const float4 a = sc.tris[3 * s + 0];
+        const float4 b = sc.tris[3 * s + 1];
+        const float4 c = sc.tris[3 * s + 2];
+        if (COUNT) cnt.tris++;
+        const uint32_t obj = __float_as_uint(c.y);
+        if (obj == r.skip) continue;
+        const f3 v0 = mk(a.x, a.y, a.z), v1 = mk(a.w, b.x, b.y), v2 = mk(b.z, b.w, c.x);
+        const float t = tri_t(v0, v1, v2, r.o, r.d, mr);
+        if (t >= kBias) {
+          const uint32_t prim = __float_as_uint(c.z);
+          if (better(t, obj, prim, best)) {
+            best.t = t;
+            best.slot = s;
+            best.obj = obj;
+            best.prim = prim;
+            if (ANY) return best;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    sp--;
+    cur = stk[sp * 64];
+  }
+  return best;
+}
+
+}  // namespace lam

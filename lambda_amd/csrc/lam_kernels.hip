@@ -1,0 +1,331 @@
+// gfx950 kernels: extend / occlusion, path tracing (one path per lane),
+// film accumulation and resolve.  Launch wrappers are in lam_launch.h.
+#include "lam_launch.h"
+#include "lam_shade.h"
+
+namespace lam {
+
+// ---------------------------------------------------------------------------
+// extend / occlusion over a ray batch
+// ---------------------------------------------------------------------------
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *__restrict__ rays, uint32_t n,
+                                                   lam_hit *__restrict__ hits, uint32_t *__restrict__ occl,
+                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ err) {
+  __shared__ uint32_t stack[kStack * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = blockIdx.x * 64u + lane;
+  if (i >= n) return;
+  const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+  const float4 a = rp[0], b = rp[1];
+  Ray r;
+  r.o = mk(a.x, a.y, a.z);
+  r.tmax = a.w;
+  r.d = mk(b.x, b.y, b.z);
+  r.skip = __float_as_uint(b.w);
+  TravCounts c{0, 0};
+  bool ovf = false;
+  const Hit h = trace<ANY, COUNT>(sc, r, stack + lane, c, ovf);
+  if (ovf) atomicOr(err, 1u);
+  if (ANY) {
+    occl[i] = h.obj != LAM_NONE ? 1u : 0u;
+  } else {
+    uint32_t tri = LAM_NONE;
+    if (h.slot != LAM_NONE) tri = __float_as_uint(sc.tris[3 * h.slot + 2].w);
+    float4 o;
+    o.x = h.obj == LAM_NONE ? kFltMax : h.t;
+    o.y = __uint_as_float(tri);
+    o.z = __uint_as_float(h.obj);
+    o.w = __uint_as_float(h.obj == LAM_NONE ? LAM_NONE : h.prim);
+    reinterpret_cast<float4 *>(hits)[i] = o;
+  }
+  if (COUNT) {
+    counts[2 * i + 0] = c.nodes;
+    counts[2 * i + 1] = c.tris;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// one path per lane: Renderer::tracePath (src/renderer.cpp:64-146) unrolled
+// into a forward loop.  The recursion is linear, so
+//   Li_s = M_IS,s * w_mat,s + L_IS,s * w_L,s
+// becomes L += β_s ⊙ (w_L,s L_IS,s) and β_{s+1} = β_s ⊙ mult_s / prob_s * w_mat,s,
+// equal to the reference up to floating-point reassociation.
+// ---------------------------------------------------------------------------
+template <int H>
+__device__ void trace_path(const DevScene &sc, const PathArgs &pa, uint32_t pixel, uint32_t sample, float *Lout,
+                           float &inv_omega, Lam<H> &lam, uint32_t *stk, bool &ovf) {
+  const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+  const uint32_t w = sc.cam.width;
+  const uint32_t row = pixel / w, colj = pixel - row * w;
+  // render (src/renderer.cpp:35-40)
+  const float u = float(colj) + lam_draw_f(key, LAM_DIM_U);
+  const float v = float(row) + lam_draw_f(key, LAM_DIM_V);
+  Ray ray;
+  camera_ray(sc.cam, u, v, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), ray.o, ray.d);
+  ray.tmax = kFltMax;
+  ray.skip = LAM_NONE;
+  inv_omega = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
+  float beta[H], L[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) { beta[k] = 1.f; L[k] = 0.f; }
+  TravCounts tc{0, 0};
+  for (uint32_t s = 0;; s++) {
+    const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
+    if (hit.obj == LAM_NONE || ovf) break;
+    const Surf sf = surface(sc, hit, ray.o, ray.d);
+    const f3 o = neg(normalized(ray.d));
+    const f3 n = sf.n();
+    const lam_material &mat = sc.materials[sf.mat];
+    if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
+      float le[H];
+      spec_eval<H>(sc, mat.spec[0], lam, le);
+      float f = 1.f;
+      if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+#pragma unroll
+      for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+      break;
+    }
+    if (s > pa.max_scat) break;
+    // light IS (src/renderer.cpp:91-106)
+    const uint32_t nl = sc.n_lights;
+    const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+    const float pl = 1.f / float(nl);
+    const lam_light &lt = sc.lights[li];
+    LightSample<H> ls;
+    sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                    lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+    float coef[H];
+    bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
+    bool vis = true;
+    if (ls.shadow) {
+      const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
+      vis = sh.obj == LAM_NONE;
+    }
+    float lis[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+    // material IS (src/renderer.cpp:116-120)
+    MatSample<H> ms;
+    sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
+                       lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
+                       pa.sample_p, ms);
+    if (!ms.ok) {
+      if (vis) {
+#pragma unroll
+        for (int k = 0; k < H; k++) L[k] += beta[k] * lis[k];
+      }
+      break;
+    }
+    // MIS (src/renderer.cpp:135-145), power heuristic β=2
+    const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+    const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], pa.sample_p);
+    const float p_L_i = ls.prob * pl;
+    const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+    const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+    if (vis) {
+#pragma unroll
+      for (int k = 0; k < H; k++) L[k] += beta[k] * (lis[k] * w_L);
+    }
+#pragma unroll
+    for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
+    ray.o = sf.p;
+    ray.d = ms.dir;
+    ray.tmax = kFltMax;
+    ray.skip = LAM_NONE;
+  }
+#pragma unroll
+  for (int k = 0; k < H; k++) Lout[k] = L[k];
+}
+
+template <int H>
+__global__ __launch_bounds__(64) void k_trace_paths(DevScene sc, PathArgs pa, const uint32_t *__restrict__ pixel,
+                                                    const uint32_t *__restrict__ sample, uint32_t n,
+                                                    float *__restrict__ out, uint32_t *__restrict__ err) {
+  __shared__ uint32_t stack[kStack * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = blockIdx.x * 64u + lane;
+  if (i >= n) return;
+  float L[H];
+  float inv_om;
+  Lam<H> lam;
+  bool ovf = false;
+  trace_path<H>(sc, pa, pixel[i], sample[i], L, inv_om, lam, stack + lane, ovf);
+  if (ovf) atomicOr(err, 1u);
+#pragma unroll
+  for (int k = 0; k < H; k++) out[size_t(i) * H + k] = L[k];
+}
+
+// addHeroλ (cg.hpp:306-320) projected onto the CMFs: the splat weights of
+// wavelength l on the coefficient grid, with the reference's ordering of
+// (1-t) and t, folded into XYZ.
+__device__ __forceinline__ void splat_xyz(const DevScene &sc, float l, float s, float &X, float &Y, float &Z) {
+  const uint32_t n = sc.grid_n;
+  const float inf = sc.grid_inf, D = sc.grid_delta;
+  // i = number of grid points inf+iΔ <= l, capped at n
+  float g = (l - inf) / D;
+  int ci = g >= 0.f ? int(fminf(g, float(n))) + 1 : 0;
+  if (ci > int(n)) ci = int(n);
+  while (ci > 0 && !(inf + float(ci - 1) * D <= l)) ci--;
+  while (ci < int(n) && inf + float(ci) * D <= l) ci++;
+  const uint32_t i = uint32_t(ci);
+  const float l1 = inf + float(i) * D;
+  float t = (l1 - l) / D;
+  if (i == 0) t = 1.f;
+  if (i == n) t = 0.f;
+  if (i > 0) {
+    const float a = (1.f - t) * s;
+    X += a * sc.cmf[i - 1];
+    Y += a * sc.cmf[n + i - 1];
+    Z += a * sc.cmf[2 * n + i - 1];
+  }
+  if (i < n) {
+    const float b = t * s;
+    X += b * sc.cmf[i];
+    Y += b * sc.cmf[n + i];
+    Z += b * sc.cmf[2 * n + i];
+  }
+}
+
+// Kahan-compensated accumulation of one sample per pixel (6 floats/px)
+__device__ __forceinline__ void film_add(float *__restrict__ film, uint32_t pixel, float X, float Y, float Z) {
+  float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;  // 8 floats: s.xyz c.xyz pad
+  float4 s = f[0], c = f[1];
+  float v[3] = {X, Y, Z};
+  float *sp[3] = {&s.x, &s.y, &s.z};
+  float *cp[3] = {&c.x, &c.y, &c.z};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float y = v[q] - *cp[q];
+    const float t = *sp[q] + y;
+    *cp[q] = (t - *sp[q]) - y;
+    *sp[q] = t;
+  }
+  f[0] = s;
+  f[1] = c;
+}
+
+template <int H>
+__global__ __launch_bounds__(64) void k_render_mega(DevScene sc, PathArgs pa, uint32_t sample, uint32_t npix,
+                                                    float *__restrict__ film, uint32_t *__restrict__ err) {
+  __shared__ uint32_t stack[kStack * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t pixel = blockIdx.x * 64u + lane;
+  if (pixel >= npix) return;
+  float L[H];
+  float inv_om;
+  Lam<H> lam;
+  bool ovf = false;
+  trace_path<H>(sc, pa, pixel, sample, L, inv_om, lam, stack + lane, ovf);
+  if (ovf) atomicOr(err, 1u);
+  float X = 0.f, Y = 0.f, Z = 0.f;
+#pragma unroll
+  for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z);
+  film_add(film, pixel, X, Y, Z);
+}
+
+// coefλ2linRGB of the normalised film (src/renderer.cpp:52-56,
+// cg.hpp:144-148, 340-341), written top row first
+__global__ void k_resolve(DevScene sc, const float *__restrict__ film, uint32_t w, uint32_t h, float inv_n,
+                          float *__restrict__ rgb, float *__restrict__ xyz) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= w * h) return;
+  const uint32_t i = p / w, j = p - i * w;
+  const float4 s = reinterpret_cast<const float4 *>(film)[size_t(p) * 2];
+  const float X = s.x * inv_n * sc.grid_delta;
+  const float Y = s.y * inv_n * sc.grid_delta;
+  const float Z = s.z * inv_n * sc.grid_delta;
+  const size_t o = (size_t(h - i - 1) * w + j) * 3;
+  rgb[o + 0] = 3.2404542f * X - 1.5371385f * Y - 0.4985314f * Z;
+  rgb[o + 1] = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+  rgb[o + 2] = 0.0556434f * X - 0.2040259f * Y + 1.0572252f * Z;
+  if (xyz) {
+    xyz[o + 0] = s.x;
+    xyz[o + 1] = s.y;
+    xyz[o + 2] = s.z;
+  }
+}
+
+__global__ void k_math(int fn, const float *__restrict__ a, const float *__restrict__ b, uint32_t n,
+                       float *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i], y = b[i];
+  float r;
+  switch (fn) {
+    case 0: r = lam_sinf(x); break;
+    case 1: r = lam_cosf(x); break;
+    case 2: r = lam_powf(x, y); break;
+    case 3: r = lam_atan2f(x, y); break;
+    case 4: r = sqrtf(x); break;
+    default: r = x / y; break;
+  }
+  out[i] = r;
+}
+
+__global__ void k_zero(float4 *__restrict__ p, size_t n4) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n4) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t blocks64(size_t n) { return uint32_t((n + 63) / 64); }
+
+hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, lam_hit *hits, uint32_t *occl,
+                             uint32_t *counts, uint32_t *err, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const dim3 g(blocks64(n)), b(64);
+  if (occl) {
+    hipLaunchKernelGGL((k_trace_rays<true, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+  } else if (counts) {
+    hipLaunchKernelGGL((k_trace_rays<false, true>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+  } else {
+    hipLaunchKernelGGL((k_trace_rays<false, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_paths(const DevScene &sc, const PathArgs &pa, int hero, const uint32_t *pixel,
+                              const uint32_t *sample, size_t n, float *out, uint32_t *err, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const dim3 g(blocks64(n)), b(64);
+  if (hero == 16)
+    hipLaunchKernelGGL((k_trace_paths<16>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err);
+  else
+    hipLaunchKernelGGL((k_trace_paths<4>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, uint32_t sample, uint32_t npix,
+                              float *film, uint32_t *err, hipStream_t st) {
+  const dim3 g(blocks64(npix)), b(64);
+  if (hero == 16)
+    hipLaunchKernelGGL((k_render_mega<16>), g, b, 0, st, sc, pa, sample, npix, film, err);
+  else
+    hipLaunchKernelGGL((k_render_mega<4>), g, b, 0, st, sc, pa, sample, npix, film, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const DevScene &sc, const float *film, uint32_t w, uint32_t h, float inv_n, float *rgb,
+                          float *xyz, hipStream_t st) {
+  const size_t n = size_t(w) * h;
+  hipLaunchKernelGGL(k_resolve, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, sc, film, w, h, inv_n, rgb, xyz);
+  return hipGetLastError();
+}
+
+hipError_t launch_math(int fn, const float *a, const float *b, size_t n, float *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_math, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, fn, a, b, uint32_t(n), out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st) {
+  const size_t n4 = bytes / 16;
+  if (n4 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(uint32_t((n4 + 255) / 256)), dim3(256), 0, st, reinterpret_cast<float4 *>(p), n4);
+  return hipGetLastError();
+}
+
+}  // namespace lam

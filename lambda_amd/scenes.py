@@ -1,0 +1,281 @@
+"""Seeded procedural scenes for the benchmark workloads, written as .nls +
+OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
+
+  box_room(dir)      closed room of 10 quads (20 tris) + two axis-aligned
+                     blocks (24 tris), one 0.5 m x 0.5 m ceiling light with
+                     a 6500 K blackbody spectrum, Lambertian surfaces with
+                     seeded smooth reflectance spectra (sum of 3 Gaussians,
+                     clipped to [0.05, 0.9]); 256x256, hero 4 on 360-830 nm.
+  icosphere_field(dir)
+                     64 instances of one noise-displaced geodesic sphere
+                     (frequency 28: 15,680 tris, ~1.0 M in total) with random
+                     rotation, scale U[0.5,2] and position in a 20 m cube,
+                     over a ground plane; materials uniform over Lambertian,
+                     Blinn (α log-uniform 10-1000) and thin film (d U[200,800]
+                     nm, film IOR U[1.33,1.6] over base 1.5); two area
+                     lights; 1920x1080.
+
+Each function returns the path of the .nls file.  Same seed, same bytes.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+GRID = {"n": 95, "inf": 360.0, "sup": 830.0}
+
+
+def blackbody(T, lam_nm):
+    """Planck's law, relative spectral radiance."""
+    h, c, k = 6.62607015e-34, 2.99792458e8, 1.380649e-23
+    lam = lam_nm * 1e-9
+    return (2 * h * c * c / lam ** 5) / np.expm1(h * c / (lam * k * T))
+
+
+def write_csv(path, lam, val):
+    with open(path, "w") as f:
+        for l, v in zip(lam, val):
+            f.write(f"{l:.6g},{v:.9g}\n")
+
+
+def smooth_reflectance(rng, lam):
+    """Sum of 3 Gaussians (μ~U[380,780], σ~U[20,80], a~U[0,1]), clipped."""
+    v = np.zeros_like(lam, dtype=np.float64)
+    for _ in range(3):
+        mu, sig, a = rng.uniform(380, 780), rng.uniform(20, 80), rng.uniform(0, 1)
+        v += a * np.exp(-0.5 * ((lam - mu) / sig) ** 2)
+    return np.clip(v, 0.05, 0.9)
+
+
+def write_obj(path, V, F, N=None):
+    """OBJ with 1-based indices; vn per vertex when N is given."""
+    with open(path, "w") as f:
+        for v in V:
+            f.write(f"v {v[0]:.7g} {v[1]:.7g} {v[2]:.7g}\n")
+        if N is not None:
+            for n in N:
+                f.write(f"vn {n[0]:.7g} {n[1]:.7g} {n[2]:.7g}\n")
+        for t in F:
+            a, b, c = (int(x) + 1 for x in t)
+            if N is not None:
+                f.write(f"f {a}//{a} {b}//{b} {c}//{c}\n")
+            else:
+                f.write(f"f {a} {b} {c}\n")
+
+
+def quad(p0, p1, p2, p3):
+    """two triangles of the quad p0 p1 p2 p3 (counter-clockwise seen from the
+    side the normal points to)"""
+    return [p0, p1, p2, p3], [(0, 1, 2), (0, 2, 3)]
+
+
+def _mesh_from_quads(quads):
+    V, F = [], []
+    for q in quads:
+        vs, fs = quad(*q)
+        base = len(V)
+        V += vs
+        F += [(a + base, b + base, c + base) for a, b, c in fs]
+    V = np.array(V, dtype=np.float64)
+    # flat per-face normals via duplicated vertices
+    N = np.zeros_like(V)
+    for a, b, c in F:
+        n = np.cross(V[b] - V[a], V[c] - V[a])
+        n /= np.linalg.norm(n)
+        N[a] = N[b] = N[c] = n
+    return V, np.array(F), N
+
+
+def _block(x0, y0, x1, y1, h):
+    z0, z1 = 0.0, h
+    P = lambda x, y, z: (x, y, z)
+    return [
+        (P(x0, y0, z1), P(x1, y0, z1), P(x1, y1, z1), P(x0, y1, z1)),  # top (+z)
+        (P(x0, y0, z0), P(x1, y0, z0), P(x1, y0, z1), P(x0, y0, z1)),  # -y
+        (P(x1, y1, z0), P(x0, y1, z0), P(x0, y1, z1), P(x1, y1, z1)),  # +y
+        (P(x0, y1, z0), P(x0, y0, z0), P(x0, y0, z1), P(x0, y1, z1)),  # -x
+        (P(x1, y0, z0), P(x1, y1, z0), P(x1, y1, z1), P(x1, y0, z1)),  # +x
+        (P(x0, y1, z0), P(x1, y1, z0), P(x1, y0, z0), P(x0, y0, z0)),  # bottom (-z)
+    ]
+
+
+def box_room(out_dir, seed=1, width=256, hero=4):
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    lam_ref = np.arange(355.0, 836.0, 5.0)
+    lam_bb = np.arange(360.0, 831.0, 1.0)
+    bb = blackbody(6500.0, lam_bb)
+    write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
+    # room x,y in [-1,1], z in [0,2]; normals point into the room
+    P = lambda x, y, z: (x, y, z)
+    h = 0.25
+    surfaces = {
+        "floor": [(P(-1, -1, 0), P(1, -1, 0), P(1, 1, 0), P(-1, 1, 0))],
+        "ceiling": [  # four strips around the light opening
+            (P(-1, -1, 2), P(-1, 1, 2), P(-h, 1, 2), P(-h, -1, 2)),
+            (P(h, -1, 2), P(h, 1, 2), P(1, 1, 2), P(1, -1, 2)),
+            (P(-h, -1, 2), P(-h, -h, 2), P(h, -h, 2), P(h, -1, 2)),
+            (P(-h, h, 2), P(-h, 1, 2), P(h, 1, 2), P(h, h, 2)),
+        ],
+        "back": [(P(-1, 1, 0), P(0, 1, 0), P(0, 1, 2), P(-1, 1, 2)),
+                 (P(0, 1, 0), P(1, 1, 0), P(1, 1, 2), P(0, 1, 2))],
+        "left": [(P(-1, -1, 0), P(-1, 1, 0), P(-1, 1, 2), P(-1, -1, 2))],
+        "right": [(P(1, 1, 0), P(1, -1, 0), P(1, -1, 2), P(1, 1, 2))],
+        "front": [(P(1, -1, 0), P(-1, -1, 0), P(-1, -1, 2), P(1, -1, 2))],
+    }
+    # fix winding so normals point inward: flip quads whose normal points out
+    def inward(q):
+        a, b, c = (np.array(x, dtype=float) for x in q[:3])
+        n = np.cross(b - a, c - a)
+        center = np.mean(np.array(q, dtype=float), axis=0)
+        to_in = np.array([0, 0, 1.0]) - center
+        return q if np.dot(n, to_in) > 0 else tuple(reversed(q))
+    objects, materials = [], []
+    names = list(surfaces) + ["block_tall", "block_short"]
+    for nm in names:
+        lam_spec = smooth_reflectance(rng, lam_ref)
+        write_csv(os.path.join(out_dir, f"albedo_{nm}.csv"), lam_ref, lam_spec)
+        materials.append({"name": f"mat_{nm}", "type": "lambertian",
+                          "albedo": {"val": os.path.join(out_dir, f"albedo_{nm}.csv")}})
+    for nm, qs in surfaces.items():
+        V, F, N = _mesh_from_quads([inward(q) for q in qs])
+        write_obj(os.path.join(out_dir, f"{nm}.obj"), V, F, N)
+    V, F, N = _mesh_from_quads(_block(-0.7, 0.0, -0.1, 0.6, 1.2))
+    write_obj(os.path.join(out_dir, "block_tall.obj"), V, F, N)
+    V, F, N = _mesh_from_quads(_block(0.1, -0.5, 0.7, 0.1, 0.6))
+    write_obj(os.path.join(out_dir, "block_short.obj"), V, F, N)
+    for nm in names:
+        objects.append({"name": nm, "type": "trimesh", "transform": {},
+                        "material": f"mat_{nm}", "source": os.path.join(out_dir, f"{nm}.obj")})
+    scene = {
+        "spectral_grid": GRID,
+        "hero_samples": hero,
+        "camera": {"pos": [0, -0.95, 1.0], "look_at": [0, 1, 1.0], "up": [0, 0, 1],
+                   "fov": 70, "ar": 1.0, "width": width},
+        "lights": [{"name": "ceiling-light", "type": "quaddir", "specular": 1,
+                    "transform": {"scale": [h, h, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
+                                  "translate": [0, 0, 2.0]},
+                    "radiance": {"val": os.path.join(out_dir, "blackbody6500.csv"), "mult": 6.0}}],
+        "objects": objects,
+        "materials": materials,
+    }
+    path = os.path.join(out_dir, "box_room.nls")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
+
+
+def geodesic_sphere(freq, seed=2, amp=0.08):
+    """Icosahedron faces subdivided freq x freq, projected to the unit sphere,
+    displaced radially by a seeded sum of sinusoids; smooth vertex normals."""
+    t = (1 + 5 ** 0.5) / 2
+    ico_v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                      [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], float)
+    ico_f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts, index, faces = [], {}, []
+
+    def vid(p):
+        p = p / np.linalg.norm(p)
+        key = tuple(np.round(p, 9))
+        if key not in index:
+            index[key] = len(verts)
+            verts.append(p)
+        return index[key]
+
+    for a, b, c in ico_f:
+        A, B, Cc = ico_v[a], ico_v[b], ico_v[c]
+        grid = {}
+        for i in range(freq + 1):
+            for j in range(freq + 1 - i):
+                grid[i, j] = vid(A + (B - A) * (i / freq) + (Cc - A) * (j / freq))
+        for i in range(freq):
+            for j in range(freq - i):
+                faces.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                if i + j + 1 < freq:
+                    faces.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+    V = np.array(verts)
+    rng = np.random.default_rng(seed)
+    disp = np.zeros(len(V))
+    for _ in range(6):
+        k = rng.normal(size=3) * rng.uniform(2, 6)
+        disp += np.sin(V @ k + rng.uniform(0, 2 * np.pi)) / 6
+    V = V * (1 + amp * disp)[:, None]
+    F = np.array(faces)
+    # orient faces outward
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    fn = np.cross(b - a, c - a)
+    flip = np.einsum("ij,ij->i", fn, a + b + c) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    N = np.zeros_like(V)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    return V, F, N
+
+
+def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778, hero=4):
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    V, F, N = geodesic_sphere(freq, seed)
+    write_obj(os.path.join(out_dir, "icosphere.obj"), V, F, N)
+    lam_ref = np.arange(355.0, 836.0, 5.0)
+    lam_bb = np.arange(360.0, 831.0, 1.0)
+    bb = blackbody(6500.0, lam_bb)
+    write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
+    materials, objects = [], []
+    write_csv(os.path.join(out_dir, "ground.csv"), lam_ref, smooth_reflectance(rng, lam_ref))
+    materials.append({"name": "ground", "type": "lambertian",
+                      "albedo": {"val": os.path.join(out_dir, "ground.csv")}})
+    objects.append({"name": "ground", "type": "plane",
+                    "transform": {"scale": [40, 40, 1], "translate": [0, 0, -0.5]}, "material": "ground"})
+    for i in range(n_inst):
+        kind = rng.integers(0, 3)
+        name = f"m{i}"
+        if kind == 0:
+            p = os.path.join(out_dir, f"albedo_{i}.csv")
+            write_csv(p, lam_ref, smooth_reflectance(rng, lam_ref))
+            materials.append({"name": name, "type": "lambertian", "albedo": {"val": p}})
+        elif kind == 1:
+            p = os.path.join(out_dir, f"kd_{i}.csv")
+            write_csv(p, lam_ref, 0.7 * smooth_reflectance(rng, lam_ref))
+            alpha = float(math.exp(rng.uniform(math.log(10), math.log(1000))))
+            materials.append({"name": name, "type": "blinn", "Kd": {"val": p}, "Ks": {"val": 0.3},
+                              "ior": {"val": 1.5}, "glossiness": round(alpha, 3)})
+        else:
+            materials.append({"name": name, "type": "thinfilm",
+                              "film_ior": {"val": round(float(rng.uniform(1.33, 1.6)), 4)},
+                              "base_ior": {"val": 1.5},
+                              "thickness": round(float(rng.uniform(200, 800)), 2),
+                              "glossiness": 256})
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        objects.append({
+            "name": f"sphere{i}", "type": "trimesh", "material": name,
+            "source": os.path.join(out_dir, "icosphere.obj"),
+            "transform": {"scale": [float(s) for s in [rng.uniform(0.5, 2)] * 3],
+                          "rotate": {"axis": [float(x) for x in ax], "degrees": float(rng.uniform(0, 360))},
+                          "translate": [float(rng.uniform(-10, 10)), float(rng.uniform(-10, 10)),
+                                        float(rng.uniform(1.5, 21.5))]}})
+    lights = []
+    for j, (x, y) in enumerate([(-6.0, -4.0), (7.0, 5.0)]):
+        lights.append({"name": f"area{j}", "type": "quaddir", "specular": 1,
+                       "transform": {"scale": [2.5, 2.5, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
+                                     "translate": [x, y, 30.0]},
+                       "radiance": {"val": os.path.join(out_dir, "blackbody6500.csv"), "mult": 40.0}})
+    scene = {
+        "spectral_grid": GRID,
+        "hero_samples": hero,
+        "camera": {"pos": [0, -38, 14], "look_at": [0, 0, 10], "up": [0, 0, 1], "fov": 45,
+                   "ar": ar, "width": width},
+        "lights": lights, "objects": objects, "materials": materials,
+    }
+    path = os.path.join(out_dir, "icosphere_field.nls")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
