@@ -106,7 +106,7 @@ def box_room(out_dir, seed=1, width=256, hero=4):
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     lam_ref = np.arange(355.0, 836.0, 5.0)
-    lam_bb = np.arange(360.0, 831.0, 1.0)
+    lam_bb = np.arange(355.0, 841.0, 1.0)
     bb = blackbody(6500.0, lam_bb)
     write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
     # room x,y in [-1,1], z in [0,2]; normals point into the room
@@ -225,7 +225,7 @@ def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778,
     V, F, N = geodesic_sphere(freq, seed)
     write_obj(os.path.join(out_dir, "icosphere.obj"), V, F, N)
     lam_ref = np.arange(355.0, 836.0, 5.0)
-    lam_bb = np.arange(360.0, 831.0, 1.0)
+    lam_bb = np.arange(355.0, 841.0, 1.0)
     bb = blackbody(6500.0, lam_bb)
     write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
     materials, objects = [], []
@@ -244,9 +244,12 @@ def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778,
         elif kind == 1:
             p = os.path.join(out_dir, f"kd_{i}.csv")
             write_csv(p, lam_ref, 0.7 * smooth_reflectance(rng, lam_ref))
-            alpha = float(math.exp(rng.uniform(math.log(10), math.log(1000))))
+            # even integer exponent: the reference evaluates powf(h·n, α) with
+            # h·n < 0 at shading-normal silhouettes (cg.hpp:963), which is NaN
+            # (a trap in the reference) for non-integer α
+            alpha = 2 * round(math.exp(rng.uniform(math.log(10), math.log(1000))) / 2)
             materials.append({"name": name, "type": "blinn", "Kd": {"val": p}, "Ks": {"val": 0.3},
-                              "ior": {"val": 1.5}, "glossiness": round(alpha, 3)})
+                              "ior": {"val": 1.5}, "glossiness": alpha})
         else:
             materials.append({"name": name, "type": "thinfilm",
                               "film_ior": {"val": round(float(rng.uniform(1.33, 1.6)), 4)},
@@ -267,7 +270,7 @@ def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778,
         lights.append({"name": f"area{j}", "type": "quaddir", "specular": 1,
                        "transform": {"scale": [2.5, 2.5, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
                                      "translate": [x, y, 30.0]},
-                       "radiance": {"val": os.path.join(out_dir, "blackbody6500.csv"), "mult": 40.0}})
+                       "radiance": {"val": os.path.join(out_dir, "blackbody6500.csv"), "mult": 2.0}})
     scene = {
         "spectral_grid": GRID,
         "hero_samples": hero,
