@@ -28,7 +28,18 @@ struct lam_ctx {
   size_t scratch_size[6] = {0, 0, 0, 0, 0, 0};
   lam_timing timing{};
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // wavefront path state, queues and control block
+  lam::WaveState ws{};
+  int ws_hero = 0;
+  std::vector<void *> ws_bufs;
+  uint32_t *ctrl = nullptr;
+  lam::WaveLaunch wl{0, 0};
+  std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
 };
+
+// Path slots per wavefront pass.  The frame is processed in chunks of this
+// many paths; 4 Mi slots with 16 hero wavelengths is ~0.9 GB of path state.
+static constexpr size_t kMaxWaveCap = size_t(1) << 22;
 
 namespace {
 
@@ -108,6 +119,140 @@ int check_params(lam_ctx *ctx, const lam_render_params *p) {
   return LAM_OK;
 }
 
+void free_wave(lam_ctx *ctx) {
+  for (void *p : ctx->ws_bufs) (void)hipFree(p);
+  ctx->ws_bufs.clear();
+  ctx->ws = lam::WaveState{};
+  ctx->ws_hero = 0;
+}
+
+int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
+  if (!ctx->ctrl) {
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->ctrl), lam::kCtrlWords * 4));
+    int cus = 0;
+    HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    // extend/shadow: 16 KB of LDS stack per 64-lane block, 10 per CU;
+    // shade: 256-lane blocks, grid-stride
+    ctx->wl.trace_blocks = uint32_t(cus > 0 ? cus : 256) * 10u;
+    ctx->wl.shade_blocks = uint32_t(cus > 0 ? cus : 256) * 8u;
+  }
+  if (ctx->ws.cap >= cap && ctx->ws_hero >= hero) return LAM_OK;
+  free_wave(ctx);
+  cap = std::max(cap, size_t(64));
+  auto alloc = [&](size_t bytes, void **p) -> int {
+    HIPC(hipMalloc(p, bytes));
+    ctx->ws_bufs.push_back(*p);
+    return LAM_OK;
+  };
+  int rc;
+  void *p;
+  const size_t hf = size_t(hero) * cap * 4;
+  if ((rc = alloc(hf, &p))) return rc;
+  ctx->ws.beta = static_cast<float *>(p);
+  if ((rc = alloc(hf, &p))) return rc;
+  ctx->ws.L = static_cast<float *>(p);
+  if ((rc = alloc(hf, &p))) return rc;
+  ctx->ws.pend = static_cast<float *>(p);
+  if ((rc = alloc(cap * 32, &ctx->ws.ray[0]))) return rc;
+  if ((rc = alloc(cap * 32, &ctx->ws.ray[1]))) return rc;
+  if ((rc = alloc(cap * 16, &p))) return rc;
+  ctx->ws.hits = static_cast<lam::Hit *>(p);
+  if ((rc = alloc(cap * 32, &ctx->ws.sray))) return rc;
+  if ((rc = alloc(cap * 4, &p))) return rc;
+  ctx->ws.spath = static_cast<uint32_t *>(p);
+  ctx->ws.cap = cap;
+  ctx->ws_hero = hero;
+  return LAM_OK;
+}
+
+int check_ctrl_overflow(lam_ctx *ctx) {
+  if (!ctx->ctrl) return LAM_OK;
+  uint32_t flag = 0;
+  HIPC(hipMemcpy(&flag, ctx->ctrl + lam::kCtrlOverflow, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(ctx, LAM_E_BVH, "BVH traversal stack overflow");
+  return LAM_OK;
+}
+
+// Wavefront render of every pixel for passes spp_offset .. spp_offset+spp-1.
+// Each chunk's pass is captured once as a hipGraph (its first node advances
+// the pass index held in the control block) and replayed spp times.
+// LAM_FLAG_PROFILE launches without a graph and times every extend and
+// shadow launch with events; LAM_FLAG_NO_GRAPH launches without a graph.
+int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix) {
+  const size_t cap = std::min(npix, kMaxWaveCap);
+  int rc;
+  if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
+  HIPC(lam::launch_zero(ctx->ctrl, lam::kCtrlWords * 4, ctx->stream));
+  const lam::PathArgs pa = path_args(p);
+  const bool profile = p->flags & LAM_FLAG_PROFILE;
+  const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
+  const uint32_t iters = p->max_scatterings + 2;
+  lam::WaveProfile prof{};
+  if (profile) {
+    const size_t need = 4 + 4 * size_t(iters);
+    while (ctx->prof_ev.size() < need) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      ctx->prof_ev.push_back(e);
+    }
+    for (int i = 0; i < 4; i++) prof.ev[i] = ctx->prof_ev[i];
+    prof.ext0 = ctx->prof_ev.data() + 4;
+    prof.ext1 = prof.ext0 + iters;
+    prof.shd0 = prof.ext1 + iters;
+    prof.shd1 = prof.shd0 + iters;
+  }
+  for (size_t off = 0; off < npix; off += cap) {
+    const uint32_t n = uint32_t(std::min(cap, npix - off));
+    if (graph) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e1 = lam::enqueue_pass_next(ctx->ctrl, ctx->stream);
+      hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n,
+                                        ctx->ctrl, film, nullptr, ctx->wl, ctx->stream, nullptr);
+      HIPC(hipStreamEndCapture(ctx->stream, &g));
+      HIPC(e1);
+      HIPC(e2);
+      hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(e3);
+      HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset - 1u, ctx->stream));
+      hipError_t e4 = hipSuccess;
+      for (uint32_t s = 0; s < p->spp && e4 == hipSuccess; s++) e4 = hipGraphLaunch(ge, ctx->stream);
+      hipError_t e5 = hipStreamSynchronize(ctx->stream);
+      (void)hipGraphExecDestroy(ge);
+      HIPC(e4);
+      HIPC(e5);
+    } else {
+      for (uint32_t s = 0; s < p->spp; s++) {
+        HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + s, ctx->stream));
+        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
+                               film, nullptr, ctx->wl, ctx->stream, profile ? &prof : nullptr));
+        if (profile) {
+          HIPC(hipStreamSynchronize(ctx->stream));
+          float ms = 0.f;
+          for (uint32_t it = 0; it < iters; it++) {
+            HIPC(hipEventElapsedTime(&ms, prof.ext0[it], prof.ext1[it]));
+            ctx->timing.extend_ms += ms;
+            HIPC(hipEventElapsedTime(&ms, prof.shd0[it], prof.shd1[it]));
+            ctx->timing.shadow_ms += ms;
+            HIPC(hipEventElapsedTime(&ms, prof.ext1[it], prof.shd0[it]));
+            ctx->timing.shade_ms += ms;
+          }
+          ctx->timing.extend_launches += iters;
+          ctx->timing.shadow_launches += iters;
+          ctx->timing.shade_launches += iters;
+          HIPC(hipEventElapsedTime(&ms, prof.ev[0], prof.ev[1]));
+          ctx->timing.raygen_ms += ms;
+          HIPC(hipEventElapsedTime(&ms, prof.ev[2], prof.ev[3]));
+          ctx->timing.film_ms += ms;
+        }
+      }
+    }
+  }
+  return LAM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -143,6 +288,9 @@ void lam_destroy(lam_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_scene(ctx);
+  free_wave(ctx);
+  if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 6; i++)
     if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
@@ -345,11 +493,24 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pi
   HIPC(hipMemcpyAsync(ctx->scratch[0], pixel, n * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(ctx->scratch[1], sample, n * 4, hipMemcpyHostToDevice, ctx->stream));
   if ((rc = clear_err_flag(ctx))) return rc;
-  HIPC(lam::launch_trace_paths(ctx->sc, path_args(p), int(H), static_cast<uint32_t *>(ctx->scratch[0]),
-                               static_cast<uint32_t *>(ctx->scratch[1]), n, static_cast<float *>(ctx->scratch[2]),
-                               ctx->d_err, ctx->stream));
-  HIPC(hipMemcpyAsync(out, ctx->scratch[2], n * H * 4, hipMemcpyDeviceToHost, ctx->stream));
+  const uint32_t *dpix = static_cast<uint32_t *>(ctx->scratch[0]);
+  const uint32_t *dsmp = static_cast<uint32_t *>(ctx->scratch[1]);
+  float *dout = static_cast<float *>(ctx->scratch[2]);
+  if (p->flags & LAM_FLAG_MEGAKERNEL) {
+    HIPC(lam::launch_trace_paths(ctx->sc, path_args(p), int(H), dpix, dsmp, n, dout, ctx->d_err, ctx->stream));
+  } else {
+    const size_t cap = std::min(n, kMaxWaveCap);
+    if ((rc = ensure_wave(ctx, cap, int(H)))) return rc;
+    HIPC(lam::launch_zero(ctx->ctrl, lam::kCtrlWords * 4, ctx->stream));
+    for (size_t off = 0; off < n; off += cap) {
+      const uint32_t m = uint32_t(std::min(cap, n - off));
+      HIPC(lam::enqueue_pass(int(H), ctx->sc, path_args(p), dpix + off, dsmp + off, 0, ctx->ws, m, ctx->ctrl,
+                             nullptr, dout + off * H, ctx->wl, ctx->stream, nullptr));
+    }
+  }
+  HIPC(hipMemcpyAsync(out, dout, n * H * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
 
@@ -370,9 +531,13 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   if ((rc = clear_err_flag(ctx))) return rc;
   const lam::PathArgs pa = path_args(p);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
-  for (uint32_t s = 0; s < p->spp; s++)
-    HIPC(lam::launch_render_mega(ctx->sc, pa, int(p->hero), p->spp_offset + s, uint32_t(npix), film, ctx->d_err,
-                                 ctx->stream));
+  if (p->flags & LAM_FLAG_MEGAKERNEL) {
+    for (uint32_t s = 0; s < p->spp; s++)
+      HIPC(lam::launch_render_mega(ctx->sc, pa, int(p->hero), p->spp_offset + s, uint32_t(npix), film,
+                                   ctx->d_err, ctx->stream));
+  } else if ((rc = render_wavefront(ctx, p, film, npix))) {
+    return rc;
+  }
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
@@ -381,6 +546,7 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
 

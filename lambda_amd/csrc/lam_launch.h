@@ -26,4 +26,34 @@ hipError_t launch_resolve(const DevScene &sc, const float *film, uint32_t w, uin
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_math(int fn, const float *a, const float *b, size_t n, float *out, hipStream_t st);
 
+// ---- wavefront integrator (lam_wavefront.hip) ------------------------------
+// Path state and queues for `cap` path slots; β, L and the pending NEE term
+// are stored [k][slot] (H rows of cap floats).
+struct WaveState {
+  float *beta, *L, *pend;
+  void *ray[2];   // extend queues, 32 B per ray (o, tmax | d, slot)
+  Hit *hits;      // 16 B per queue entry
+  void *sray;     // shadow queue, 32 B per ray (o, tmax | d, skip)
+  uint32_t *spath;
+  size_t cap;
+};
+struct WaveLaunch {
+  uint32_t trace_blocks;  // persistent extend / shadow grid (64-lane blocks)
+  uint32_t shade_blocks;  // shade grid (256-lane blocks)
+};
+// HIP events around every extend and shadow launch of one pass
+struct WaveProfile {
+  hipEvent_t ev[4];
+  hipEvent_t *ext0, *ext1, *shd0, *shd1;
+};
+constexpr uint32_t kCtrlWords = 16;
+constexpr uint32_t kCtrlOverflow = 15;
+
+hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
+                        const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
+                        uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
+                        WaveProfile *prof);
+hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
+hipError_t enqueue_pass_next(uint32_t *ctrl, hipStream_t st);
+
 }  // namespace lam

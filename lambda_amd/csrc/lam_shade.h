@@ -566,4 +566,54 @@ __device__ __forceinline__ float hero_lambdas(const DevScene &sc, float xi, Lam<
   return 1.f / om;
 }
 
+// ---- film -------------------------------------------------------------------------
+// addHeroλ (cg.hpp:306-320) projected onto the CMFs: the splat weights of
+// wavelength l on the coefficient grid, with the reference's ordering of
+// (1-t) and t, folded into XYZ.
+__device__ __forceinline__ void splat_xyz(const DevScene &sc, float l, float s, float &X, float &Y, float &Z) {
+  const uint32_t n = sc.grid_n;
+  const float inf = sc.grid_inf, D = sc.grid_delta;
+  // i = number of grid points inf+iΔ <= l, capped at n
+  float g = (l - inf) / D;
+  int ci = g >= 0.f ? int(fminf(g, float(n))) + 1 : 0;
+  if (ci > int(n)) ci = int(n);
+  while (ci > 0 && !(inf + float(ci - 1) * D <= l)) ci--;
+  while (ci < int(n) && inf + float(ci) * D <= l) ci++;
+  const uint32_t i = uint32_t(ci);
+  const float l1 = inf + float(i) * D;
+  float t = (l1 - l) / D;
+  if (i == 0) t = 1.f;
+  if (i == n) t = 0.f;
+  if (i > 0) {
+    const float a = (1.f - t) * s;
+    X += a * sc.cmf[i - 1];
+    Y += a * sc.cmf[n + i - 1];
+    Z += a * sc.cmf[2 * n + i - 1];
+  }
+  if (i < n) {
+    const float b = t * s;
+    X += b * sc.cmf[i];
+    Y += b * sc.cmf[n + i];
+    Z += b * sc.cmf[2 * n + i];
+  }
+}
+
+// Kahan-compensated accumulation of one sample per pixel (6 floats/px)
+__device__ __forceinline__ void film_add(float *__restrict__ film, uint32_t pixel, float X, float Y, float Z) {
+  float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;  // 8 floats: s.xyz c.xyz pad
+  float4 s = f[0], c = f[1];
+  float v[3] = {X, Y, Z};
+  float *sp[3] = {&s.x, &s.y, &s.z};
+  float *cp[3] = {&c.x, &c.y, &c.z};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float y = v[q] - *cp[q];
+    const float t = *sp[q] + y;
+    *cp[q] = (t - *sp[q]) - y;
+    *sp[q] = t;
+  }
+  f[0] = s;
+  f[1] = c;
+}
+
 }  // namespace lam

@@ -1,0 +1,385 @@
+// Wavefront integrator for gfx950: the recursive Renderer::tracePath
+// (src/renderer.cpp:64-146) split into kernels linked by queues.
+//
+// One pass traces one path for each of N path slots.  Slot p's identity
+// (pixel, sample) comes either from lists (test hook) or from (p + offset,
+// pass index).  All paths of a pass start together, so every entry of the
+// extend queue at iteration s is at path vertex s.
+//
+// Per iteration s:
+//   k_iter_begin  reset the next-queue / shadow-queue counts and the work
+//                 fetch counters (a kernel, so the pass can be captured in a
+//                 hipGraph without memset nodes)
+//   k_extend      persistent closest-hit over the current queue: each wave
+//                 fetches 64 rays with one atomic
+//   k_shade       hit -> surface, emitter, NEE light sample, BSDF sample,
+//                 MIS weights; appends the continuation ray and the shadow ray
+//                 to compacted queues (wave ballot + prefix count + one atomic
+//                 per wave per queue)
+//   k_shadow      persistent any-hit; adds the pending NEE term when visible
+// then k_film splats each path's spectral radiance into the Kahan XYZ film.
+//
+// Path state is SoA ([k][slot]) so every access is coalesced; it holds only
+// the throughput β, the radiance L and the pending NEE term, H floats each.
+// Random numbers, hero wavelengths and the vertex index are recomputed from
+// (pixel, sample) and s, so replaying a path reproduces it exactly.
+#include "lam_launch.h"
+#include "lam_shade.h"
+
+namespace lam {
+
+// control block (uint32 words)
+enum : uint32_t {
+  CTRL_COUNT0 = 0,  // extend queue ping
+  CTRL_COUNT1 = 1,  // extend queue pong
+  CTRL_SHADOW = 2,  // shadow queue count
+  CTRL_FETCH_EXT = 3,
+  CTRL_FETCH_SHD = 4,
+  CTRL_PASS = 8,    // current pass (sample) index
+  CTRL_WORDS = 16
+};
+
+struct PathIds {
+  const uint32_t *pixel;   // nullable: then pixel = slot + pixel_offset
+  const uint32_t *sample;  // nullable: then sample = ctrl[CTRL_PASS]
+  uint32_t pixel_offset;
+};
+
+__device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
+                                        uint32_t &sample) {
+  pixel = ids.pixel ? ids.pixel[slot] : slot + ids.pixel_offset;
+  sample = ids.sample ? ids.sample[slot] : ctrl[CTRL_PASS];
+}
+
+template <int H>
+__device__ __forceinline__ float lam_for(const DevScene &sc, uint64_t key, Lam<H> &L) {
+  return hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), L);
+}
+
+// ---- ray generation: render (src/renderer.cpp:35-40) ------------------------------
+template <int H>
+__global__ __launch_bounds__(256) void k_raygen(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t n,
+                                                uint32_t *__restrict__ ctrl, uint32_t qsel) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    ctrl[CTRL_COUNT0 + qsel] = n;
+    ctrl[CTRL_FETCH_EXT] = 0;
+  }
+  if (p >= n) return;
+  uint32_t pixel, sample;
+  path_id(ids, ctrl, p, pixel, sample);
+  const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+  const uint32_t w = sc.cam.width;
+  const uint32_t row = pixel / w, colj = pixel - row * w;
+  const float u = float(colj) + lam_draw_f(key, LAM_DIM_U);
+  const float v = float(row) + lam_draw_f(key, LAM_DIM_V);
+  f3 o, d;
+  camera_ray(sc.cam, u, v, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), o, d);
+  float4 *q = reinterpret_cast<float4 *>(ws.ray[qsel]) + 2 * size_t(p);
+  q[0] = make_float4(o.x, o.y, o.z, kFltMax);
+  q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    ws.beta[size_t(k) * ws.cap + p] = 1.f;
+    ws.L[size_t(k) * ws.cap + p] = 0.f;
+  }
+}
+
+__global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctrl[CTRL_COUNT0 + next_q] = 0;
+    ctrl[CTRL_SHADOW] = 0;
+    ctrl[CTRL_FETCH_EXT] = 0;
+    ctrl[CTRL_FETCH_SHD] = 0;
+  }
+}
+
+__global__ void k_pass_set(uint32_t *__restrict__ ctrl, uint32_t pass) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = pass;
+}
+__global__ void k_pass_next(uint32_t *__restrict__ ctrl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = ctrl[CTRL_PASS] + 1;
+}
+
+// wave-wide chunk fetch: lane 0 takes 64 indices, broadcast to the wave
+__device__ __forceinline__ uint32_t fetch64(uint32_t *ctr) {
+  uint32_t base = 0;
+  if (threadIdx.x % 64 == 0) base = atomicAdd(ctr, 64u);
+  return __shfl(base, 0, 64);
+}
+
+// ---- extend: persistent closest hit ---------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
+                                               uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t stack[kStack * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  uint32_t *ctr = ctrl + CTRL_FETCH_EXT;
+  for (;;) {
+    const uint32_t base = fetch64(ctr);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
+      Ray r;
+      r.o = mk(a.x, a.y, a.z);
+      r.tmax = a.w;
+      r.d = mk(b.x, b.y, b.z);
+      r.skip = LAM_NONE;
+      TravCounts c{0, 0};
+      bool ovf = false;
+      Hit h = trace<false, COUNT>(sc, r, stack + lane, c, ovf);
+      if (ovf) {  // cannot happen under the host-checked stack bound
+        atomicOr(ctrl + 15, 1u);
+        h.obj = LAM_NONE;
+      }
+      hits[i] = h;
+      if (COUNT) {
+        atomicAdd(counts + 0, c.nodes);
+        atomicAdd(counts + 1, c.tris);
+        atomicAdd(counts + 2, 1u);
+      }
+    }
+  }
+}
+
+// ---- shade ------------------------------------------------------------------------------
+// wave-aggregated append: returns this lane's slot in a queue whose count
+// lives at *count (valid only where want is true)
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
+  const uint64_t mask = __ballot(want);
+  const uint32_t lane = threadIdx.x % 64;
+  const uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
+  const uint32_t total = __popcll(mask);
+  uint32_t base = 0;
+  const uint32_t leader = mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 0;
+  if (want && rank == 0) base = atomicAdd(count, total);
+  base = __shfl(base, leader, 64);
+  return base + rank;
+}
+
+template <int H>
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
+                                               uint32_t qsel, uint32_t *__restrict__ ctrl) {
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
+  const size_t cap = ws.cap;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // uniform trip count across the wave so the ballots see every lane
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wave_first = first - (threadIdx.x % 64);
+  for (uint32_t wbase = wave_first; wbase < n; wbase += stride) {
+    const uint32_t q = wbase + (threadIdx.x % 64);
+    bool push_next = false, push_shadow = false;
+    uint32_t p = 0;
+    Ray nxt, shd;
+    if (q < n) {
+      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
+      const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+      p = __float_as_uint(b.w);
+      const Hit hit = ws.hits[q];
+      if (hit.obj != LAM_NONE) {
+        uint32_t pixel, sample;
+        path_id(ids, ctrl, p, pixel, sample);
+        const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+        Lam<H> lam;
+        lam_for<H>(sc, key, lam);
+        float beta[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) beta[k] = ws.beta[size_t(k) * cap + p];
+        const Surf sf = surface(sc, hit, ro, rd);
+        const f3 o = neg(normalized(rd));
+        const f3 nn = sf.n();
+        const lam_material &mat = sc.materials[sf.mat];
+        if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
+          float le[H];
+          spec_eval<H>(sc, mat.spec[0], lam, le);
+          float f = 1.f;
+          if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+#pragma unroll
+          for (int k = 0; k < H; k++)
+            ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+        } else if (s <= pa.max_scat) {
+          const uint32_t nl = sc.n_lights;
+          const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+          const float pl = 1.f / float(nl);
+          const lam_light &lt = sc.lights[li];
+          LightSample<H> ls;
+          sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                          lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+          float coef[H];
+          bxdf<H>(sc, mat, lam, ls.dir, o, nn, sf.front, coef);
+          float lis[H];
+#pragma unroll
+          for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+          MatSample<H> ms;
+          sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
+                             lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)),
+                             lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)), pa.sample_p, ms);
+          float pend[H];
+          if (!ms.ok) {
+#pragma unroll
+            for (int k = 0; k < H; k++) pend[k] = beta[k] * lis[k];
+          } else {
+            const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+            const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], pa.sample_p);
+            const float p_L_i = ls.prob * pl;
+            const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+            const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+              pend[k] = beta[k] * (lis[k] * w_L);
+              ws.beta[size_t(k) * cap + p] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
+            }
+            push_next = true;
+            nxt.o = sf.p;
+            nxt.d = ms.dir;
+          }
+          if (ls.shadow) {
+#pragma unroll
+            for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + p] = pend[k];
+            push_shadow = true;
+            shd = ls.sray;
+          } else {
+#pragma unroll
+            for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += pend[k];
+          }
+        }
+      }
+    }
+    const uint32_t qn = wave_append(push_next, ctrl + CTRL_COUNT0 + (qsel ^ 1u));
+    if (push_next) {
+      rout[2 * size_t(qn)] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
+      rout[2 * size_t(qn) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
+    }
+    const uint32_t qs = wave_append(push_shadow, ctrl + CTRL_SHADOW);
+    if (push_shadow) {
+      float4 *sr = reinterpret_cast<float4 *>(ws.sray);
+      sr[2 * size_t(qs)] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
+      sr[2 * size_t(qs) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
+      ws.spath[qs] = p;
+    }
+  }
+}
+
+// ---- shadow: persistent any-hit, then the pending NEE term -----------------------------
+template <int H, bool COUNT>
+__global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32_t *__restrict__ ctrl,
+                                               uint32_t *__restrict__ counts) {
+  __shared__ uint32_t stack[kStack * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = ctrl[CTRL_SHADOW];
+  const float4 *__restrict__ sr = reinterpret_cast<const float4 *>(ws.sray);
+  const size_t cap = ws.cap;
+  for (;;) {
+    const uint32_t base = fetch64(ctrl + CTRL_FETCH_SHD);
+    if (base >= n) break;
+    const uint32_t i = base + lane;
+    if (i < n) {
+      const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
+      Ray r;
+      r.o = mk(a.x, a.y, a.z);
+      r.tmax = a.w;
+      r.d = mk(b.x, b.y, b.z);
+      r.skip = __float_as_uint(b.w);
+      TravCounts c{0, 0};
+      bool ovf = false;
+      const Hit h = trace<true, COUNT>(sc, r, stack + lane, c, ovf);
+      if (ovf) atomicOr(ctrl + 15, 1u);
+      if (h.obj == LAM_NONE && !ovf) {
+        const uint32_t p = ws.spath[i];
+#pragma unroll
+        for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += ws.pend[size_t(k) * cap + p];
+      }
+      if (COUNT) {
+        atomicAdd(counts + 3, c.nodes);
+        atomicAdd(counts + 4, c.tris);
+        atomicAdd(counts + 5, 1u);
+      }
+    }
+  }
+}
+
+// ---- film / output ----------------------------------------------------------------------
+template <int H>
+__global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t n,
+                                              const uint32_t *__restrict__ ctrl, float *__restrict__ film,
+                                              float *__restrict__ out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (out) {  // test hook: raw spectral radiance per path
+#pragma unroll
+    for (int k = 0; k < H; k++) out[size_t(p) * H + k] = ws.L[size_t(k) * ws.cap + p];
+    return;
+  }
+  uint32_t pixel, sample;
+  path_id(ids, ctrl, p, pixel, sample);
+  const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+  Lam<H> lam;
+  const float inv_om = lam_for<H>(sc, key, lam);
+  float X = 0.f, Y = 0.f, Z = 0.f;
+#pragma unroll
+  for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z);
+  film_add(film, pixel, X, Y, Z);
+}
+
+// ---------------------------------------------------------------------------
+// host side: one pass = raygen, (max_scat + 2) iterations, film
+// ---------------------------------------------------------------------------
+template <int H>
+static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const PathIds &ids, const WaveState &ws,
+                                 uint32_t n, uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl,
+                                 hipStream_t st, WaveProfile *prof) {
+  const uint32_t b256 = (n + 255) / 256;
+  auto ev = [&](int slot) {
+    if (prof) (void)hipEventRecord(prof->ev[slot], st);
+  };
+  ev(0);
+  hipLaunchKernelGGL((k_raygen<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, 0u);
+  ev(1);
+  uint32_t q = 0;
+  const uint32_t iters = pa.max_scat + 2;
+  for (uint32_t s = 0; s < iters; s++) {
+    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u);
+    if (prof) {
+      (void)hipEventRecord(prof->ext0[s], st);
+      hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
+                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, nullptr);
+      (void)hipEventRecord(prof->ext1[s], st);
+    } else {
+      hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
+                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, nullptr);
+    }
+    hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, s, q, ctrl);
+    if (prof) (void)hipEventRecord(prof->shd0[s], st);
+    hipLaunchKernelGGL((k_shadow<H, false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr);
+    if (prof) (void)hipEventRecord(prof->shd1[s], st);
+    q ^= 1u;
+  }
+  ev(2);
+  hipLaunchKernelGGL((k_film<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, out);
+  ev(3);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
+                        const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
+                        uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
+                        WaveProfile *prof) {
+  PathIds ids{pixel, sample, pixel_offset};
+  if (hero == 16) return enqueue_pass_t<16>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof);
+  return enqueue_pass_t<4>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof);
+}
+
+hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st) {
+  hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, st, ctrl, pass);
+  return hipGetLastError();
+}
+hipError_t enqueue_pass_next(uint32_t *ctrl, hipStream_t st) {
+  hipLaunchKernelGGL(k_pass_next, dim3(1), dim3(64), 0, st, ctrl);
+  return hipGetLastError();
+}
+
+}  // namespace lam

@@ -111,10 +111,11 @@ def _check_paths(a, b, rel=1e-4):
     assert bad.size == 0, f"{len(bad)} path values off by more than {rel}: worst {err.max():.3g} at {bad[:3]}"
 
 
+@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_MEGAKERNEL], ids=["wavefront", "megakernel"])
 @pytest.mark.parametrize("scene_name,hero,p,bounces", [
     ("box_scene", 4, 0.8, 8), ("field_scene", 4, 0.8, 12), ("mixed_scene", 16, 0.8, 8),
     ("mixed_scene", 4, 0.5, 4)])
-def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces):
+def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces, flags):
     scene = request.getfixturevalue(scene_name)
     ctx.upload(scene)
     orc = ob.Oracle(scene)
@@ -122,19 +123,21 @@ def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces):
     rng = np.random.default_rng(5)
     pix = rng.integers(0, npix, 8192).astype(np.uint32)
     smp = rng.integers(0, 64, 8192).astype(np.uint32)
-    pr = la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=hero, seed=9)
+    pr = la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=hero, seed=9, flags=flags)
     g, o = ctx.trace_paths(pr, pix, smp), orc.trace_paths(pr, pix, smp)
     assert np.isfinite(o).all()
     assert (o > 0).mean() > 0.05
     _check_paths(g, o)
 
 
+@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_GRAPH, la.LAM_FLAG_PROFILE, la.LAM_FLAG_MEGAKERNEL],
+                         ids=["graph", "nograph", "profile", "megakernel"])
 @pytest.mark.parametrize("scene_name,hero", [("box_scene", 4), ("mixed_scene", 16)])
-def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero):
+def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero, flags):
     scene = request.getfixturevalue(scene_name)
     ctx.upload(scene)
     orc = ob.Oracle(scene)
-    pr = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=hero, seed=1)
+    pr = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=hero, seed=1, flags=flags)
     g, gx = ctx.render(pr, want_xyz=True)
     o, ox = orc.render(pr, want_xyz=True)
     # XYZ projection of the tent-splat film equals the spectral film then
