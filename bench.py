@@ -71,9 +71,16 @@ def camera_rays(desc):
 def hip_lib():
     for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
         try:
-            return C.CDLL(name)
+            h = C.CDLL(name)
         except OSError:
             continue
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipMalloc.restype = C.c_int
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipFree.restype = C.c_int
+        return h
     raise RuntimeError("libamdhip64.so not found")
 
 
@@ -128,15 +135,17 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=15.0):
     h = scene.height
     w = scene.width
     pr = la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=hero, seed=7)
-    # calibrate on a thin stripe, then size the sample to the budget
+    # calibrate on a thin stripe, then size the sample (rows, then spp) to
+    # about budget_s seconds of CPU work
     step = 64
     t0 = time.perf_counter()
     orc.render(pr, threads=threads, row_step=step, row_phase=0)
     dt = time.perf_counter() - t0
     rows = len(range(0, h, step))
-    rate = rows * w / max(dt, 1e-6)
-    step = int(max(1, min(h, (h * w / max(rate * budget_s, 1.0)))))
-    spp_cpu = 1
+    rate = rows * w / max(dt, 1e-6)  # samples/s
+    want = rate * budget_s
+    step = int(max(1, min(h, h * w / max(want, 1.0))))
+    spp_cpu = int(max(1, min(spp, want / (h * w)))) if step == 1 else 1
     pr = la.params(spp=spp_cpu, max_scatterings=bounces, sample_p=p, hero=hero, seed=7)
     t0 = time.perf_counter()
     orc.render(pr, threads=threads, row_step=step, row_phase=1 % step)
@@ -156,6 +165,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--extend-only", action="store_true",
+                    help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
+                         "(the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -170,6 +182,19 @@ def main():
     log(f"scene: {scene.n_tris} tris, {st.n_nodes} BVH8 nodes, depth {st.depth}, stack bound "
         f"{st.stack_bound}, setup {time.perf_counter() - t0:.1f}s")
     w, h = scene.width, scene.height
+    if args.extend_only:
+        rl = extend_roofline(ctx, scene.desc, reps=3)
+        hip = hip_lib()
+        a, b = C.c_void_p(), C.c_void_p()
+        nbytes = 1 << 30
+        assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
+        for _ in range(3):
+            assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # device to device
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"extend": rl, "copy_bytes": nbytes}), flush=True)
+        ctx.close()
+        return
     if args.warmup:
         ctx.render(la.params(spp=args.warmup, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
                              spp_offset=1 << 20, flags=args.flags))
@@ -195,7 +220,7 @@ def main():
         out["roofline"] = extend_roofline(ctx, scene.desc)
         log(f"extend: {out['roofline']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, args.steps, bounces, p, hero)
+        out["cpu_baseline"] = cpu_baseline(scene, 1024, bounces, p, hero)
         log(f"cpu: {out['cpu_baseline']}")
     ctx.close()
     print(json.dumps(out), flush=True)

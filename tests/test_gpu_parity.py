@@ -104,11 +104,14 @@ def test_shadow_bit_exact(ctx, request, scene_name):
     assert np.array_equal(go, (gh["obj"] != la.LAM_NONE).astype(np.uint32))
 
 
-def _check_paths(a, b, rel=1e-4):
-    scale = max(float(np.abs(b).mean()), 1e-12)
-    err = np.abs(a.astype(np.float64) - b) / (np.abs(b) + 1e-6 * scale)
-    bad = np.argwhere(err > rel)
-    assert bad.size == 0, f"{len(bad)} path values off by more than {rel}: worst {err.max():.3g} at {bad[:3]}"
+def _check_paths(a, b, rel=1e-4, floor=1e-6):
+    """|gpu - oracle| <= rel |oracle| + floor * mean|oracle| for every path
+    and wavelength.  The absolute floor covers paths whose terms cancel to
+    ~0 (the forward sum and the recursive oracle associate differently)."""
+    scale = max(float(np.abs(b).mean()), 1e-30)
+    err = np.abs(a.astype(np.float64) - b) / (rel * np.abs(b) + floor * scale)
+    bad = np.argwhere(err > 1.0)
+    assert bad.size == 0, f"{len(bad)} path values out of tolerance: worst {err.max():.3g} at {bad[:3]}"
 
 
 @pytest.mark.parametrize("flags", [0, la.LAM_FLAG_MEGAKERNEL], ids=["wavefront", "megakernel"])
