@@ -132,9 +132,14 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
     int cus = 0;
     HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     // extend/shadow: 16 KB of LDS stack per 64-lane block, 10 per CU;
-    // shade: 256-lane blocks, grid-stride
+    // shade: 256-lane blocks, grid-stride, one occupancy round (its VGPR
+    // count allows 2 waves per SIMD = 2 blocks per CU)
     ctx->wl.trace_blocks = uint32_t(cus > 0 ? cus : 256) * 10u;
-    ctx->wl.shade_blocks = uint32_t(cus > 0 ? cus : 256) * 8u;
+    ctx->wl.shade_blocks = uint32_t(cus > 0 ? cus : 256) * 2u;
+    // below ~128 Ki paths a synchronised bounce is bounded by its slowest
+    // ray; the finish kernel then traces each remaining path to its end
+    ctx->wl.tail_threshold = 131072u;
+    ctx->wl.max_iters = 12u;
   }
   if (ctx->ws.cap >= cap && ctx->ws_hero >= hero) return LAM_OK;
   free_wave(ctx);
@@ -165,6 +170,15 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
   return LAM_OK;
 }
 
+lam::WaveLaunch launch_for(const lam_ctx *ctx, const lam_render_params *p) {
+  lam::WaveLaunch wl = ctx->wl;
+  if (p->flags & LAM_FLAG_NO_TAIL) {
+    wl.tail_threshold = 0;
+    wl.max_iters = 0;
+  }
+  return wl;
+}
+
 int check_ctrl_overflow(lam_ctx *ctx) {
   if (!ctx->ctrl) return LAM_OK;
   uint32_t flag = 0;
@@ -186,17 +200,19 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   const lam::PathArgs pa = path_args(p);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
-  const uint32_t iters = p->max_scatterings + 2;
+  const lam::WaveLaunch wl = launch_for(ctx, p);
+  const uint32_t iters = lam::wave_iterations(p->max_scatterings, wl);
   lam::WaveProfile prof{};
   if (profile) {
-    const size_t need = 4 + 4 * size_t(iters);
+    const size_t need = 4 + 5 * size_t(iters);
     while (ctx->prof_ev.size() < need) {
       hipEvent_t e;
       HIPC(hipEventCreate(&e));
       ctx->prof_ev.push_back(e);
     }
     for (int i = 0; i < 4; i++) prof.ev[i] = ctx->prof_ev[i];
-    prof.ext0 = ctx->prof_ev.data() + 4;
+    prof.fin0 = ctx->prof_ev.data() + 4;
+    prof.ext0 = prof.fin0 + iters;
     prof.ext1 = prof.ext0 + iters;
     prof.shd0 = prof.ext1 + iters;
     prof.shd1 = prof.shd0 + iters;
@@ -209,7 +225,7 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
       HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e1 = lam::enqueue_pass_next(ctx->ctrl, ctx->stream);
       hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n,
-                                        ctx->ctrl, film, nullptr, ctx->wl, ctx->stream, nullptr);
+                                        ctx->ctrl, film, nullptr, wl, ctx->stream, nullptr);
       HIPC(hipStreamEndCapture(ctx->stream, &g));
       HIPC(e1);
       HIPC(e2);
@@ -227,7 +243,7 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
       for (uint32_t s = 0; s < p->spp; s++) {
         HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + s, ctx->stream));
         HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
-                               film, nullptr, ctx->wl, ctx->stream, profile ? &prof : nullptr));
+                               film, nullptr, wl, ctx->stream, profile ? &prof : nullptr));
         if (profile) {
           HIPC(hipStreamSynchronize(ctx->stream));
           float ms = 0.f;
@@ -236,6 +252,8 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
             ctx->timing.extend_ms += ms;
             HIPC(hipEventElapsedTime(&ms, prof.shd0[it], prof.shd1[it]));
             ctx->timing.shadow_ms += ms;
+            HIPC(hipEventElapsedTime(&ms, prof.fin0[it], prof.ext0[it]));
+            ctx->timing.other_ms += ms;  // iteration reset + finish kernel
             HIPC(hipEventElapsedTime(&ms, prof.ext1[it], prof.shd0[it]));
             ctx->timing.shade_ms += ms;
           }
@@ -505,7 +523,7 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pi
     for (size_t off = 0; off < n; off += cap) {
       const uint32_t m = uint32_t(std::min(cap, n - off));
       HIPC(lam::enqueue_pass(int(H), ctx->sc, path_args(p), dpix + off, dsmp + off, 0, ctx->ws, m, ctx->ctrl,
-                             nullptr, dout + off * H, ctx->wl, ctx->stream, nullptr));
+                             nullptr, dout + off * H, launch_for(ctx, p), ctx->stream, nullptr));
     }
   }
   HIPC(hipMemcpyAsync(out, dout, n * H * 4, hipMemcpyDeviceToHost, ctx->stream));

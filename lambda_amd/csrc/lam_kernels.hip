@@ -66,76 +66,10 @@ __device__ void trace_path(const DevScene &sc, const PathArgs &pa, uint32_t pixe
   ray.tmax = kFltMax;
   ray.skip = LAM_NONE;
   inv_omega = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
-  float beta[H], L[H];
+  float beta[H];
 #pragma unroll
-  for (int k = 0; k < H; k++) { beta[k] = 1.f; L[k] = 0.f; }
-  TravCounts tc{0, 0};
-  for (uint32_t s = 0;; s++) {
-    const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
-    if (hit.obj == LAM_NONE || ovf) break;
-    const Surf sf = surface(sc, hit, ray.o, ray.d);
-    const f3 o = neg(normalized(ray.d));
-    const f3 n = sf.n();
-    const lam_material &mat = sc.materials[sf.mat];
-    if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
-      float le[H];
-      spec_eval<H>(sc, mat.spec[0], lam, le);
-      float f = 1.f;
-      if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
-#pragma unroll
-      for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
-      break;
-    }
-    if (s > pa.max_scat) break;
-    // light IS (src/renderer.cpp:91-106)
-    const uint32_t nl = sc.n_lights;
-    const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
-    const float pl = 1.f / float(nl);
-    const lam_light &lt = sc.lights[li];
-    LightSample<H> ls;
-    sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
-                    lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
-    float coef[H];
-    bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
-    bool vis = true;
-    if (ls.shadow) {
-      const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
-      vis = sh.obj == LAM_NONE;
-    }
-    float lis[H];
-#pragma unroll
-    for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
-    // material IS (src/renderer.cpp:116-120)
-    MatSample<H> ms;
-    sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
-                       lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
-                       pa.sample_p, ms);
-    if (!ms.ok) {
-      if (vis) {
-#pragma unroll
-        for (int k = 0; k < H; k++) L[k] += beta[k] * lis[k];
-      }
-      break;
-    }
-    // MIS (src/renderer.cpp:135-145), power heuristic β=2
-    const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
-    const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], pa.sample_p);
-    const float p_L_i = ls.prob * pl;
-    const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
-    const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
-    if (vis) {
-#pragma unroll
-      for (int k = 0; k < H; k++) L[k] += beta[k] * (lis[k] * w_L);
-    }
-#pragma unroll
-    for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
-    ray.o = sf.p;
-    ray.d = ms.dir;
-    ray.tmax = kFltMax;
-    ray.skip = LAM_NONE;
-  }
-#pragma unroll
-  for (int k = 0; k < H; k++) Lout[k] = L[k];
+  for (int k = 0; k < H; k++) { beta[k] = 1.f; Lout[k] = 0.f; }
+  path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, 0u, ray, beta, Lout, stk, ovf);
 }
 
 template <int H>

@@ -38,13 +38,20 @@ struct WaveState {
   size_t cap;
 };
 struct WaveLaunch {
-  uint32_t trace_blocks;  // persistent extend / shadow grid (64-lane blocks)
-  uint32_t shade_blocks;  // shade grid (256-lane blocks)
+  uint32_t trace_blocks;    // persistent extend / shadow / finish grid (64-lane blocks)
+  uint32_t shade_blocks;    // shade grid (256-lane blocks)
+  uint32_t tail_threshold;  // queue length at or below which k_finish takes over
+  uint32_t max_iters;       // 0: pure wavefront; else cap on bounce iterations per pass
 };
-// HIP events around every extend and shadow launch of one pass
+// bounce iterations enqueued per pass
+inline uint32_t wave_iterations(uint32_t max_scat, const WaveLaunch &wl) {
+  const uint32_t full = max_scat + 2;
+  return wl.max_iters && wl.max_iters < full ? wl.max_iters : full;
+}
+// HIP events around every finish, extend and shadow launch of one pass
 struct WaveProfile {
   hipEvent_t ev[4];
-  hipEvent_t *ext0, *ext1, *shd0, *shd1;
+  hipEvent_t *fin0, *ext0, *ext1, *shd0, *shd1;
 };
 constexpr uint32_t kCtrlWords = 16;
 constexpr uint32_t kCtrlOverflow = 15;

@@ -35,6 +35,8 @@ enum : uint32_t {
   CTRL_SHADOW = 2,  // shadow queue count
   CTRL_FETCH_EXT = 3,
   CTRL_FETCH_SHD = 4,
+  CTRL_TAIL = 5,    // 1: the current queue is finished by k_finish
+  CTRL_FETCH_FIN = 6,
   CTRL_PASS = 8,    // current pass (sample) index
   CTRL_WORDS = 16
 };
@@ -64,6 +66,7 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, PathArgs pa, PathId
   if (p == 0) {
     ctrl[CTRL_COUNT0 + qsel] = n;
     ctrl[CTRL_FETCH_EXT] = 0;
+    ctrl[CTRL_TAIL] = 0;
   }
   if (p >= n) return;
   uint32_t pixel, sample;
@@ -85,12 +88,20 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, PathArgs pa, PathId
   }
 }
 
-__global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q) {
+// Resets the counters of iteration s and decides whether the paths left in
+// the current queue are finished by one thread each (k_finish): when the
+// queue is short enough that a synchronised bounce costs about as much as
+// its slowest ray, or at the last scheduled iteration (force).
+__global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q, uint32_t tail_threshold,
+                             uint32_t force) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctrl[CTRL_COUNT0 + next_q] = 0;
     ctrl[CTRL_SHADOW] = 0;
     ctrl[CTRL_FETCH_EXT] = 0;
     ctrl[CTRL_FETCH_SHD] = 0;
+    ctrl[CTRL_FETCH_FIN] = 0;
+    const uint32_t c = ctrl[CTRL_COUNT0 + (next_q ^ 1u)];
+    if (c > 0 && (c <= tail_threshold || force)) ctrl[CTRL_TAIL] = 1;
   }
 }
 
@@ -101,11 +112,14 @@ __global__ void k_pass_next(uint32_t *__restrict__ ctrl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = ctrl[CTRL_PASS] + 1;
 }
 
-// wave-wide chunk fetch: lane 0 takes 64 indices, broadcast to the wave
-__device__ __forceinline__ uint32_t fetch64(uint32_t *ctr) {
-  uint32_t base = 0;
-  if (threadIdx.x % 64 == 0) base = atomicAdd(ctr, 64u);
-  return __shfl(base, 0, 64);
+// Work distribution of the persistent trace grids (one 64-lane wave per
+// block): block b first takes chunk b (no atomic), then further chunks
+// gridDim.x + k from one atomic per wave.  A short queue therefore costs no
+// atomics at all; a long one costs one per 64 rays.
+__device__ __forceinline__ uint32_t next_chunk(uint32_t *ctr) {
+  uint32_t k = 0;
+  if (threadIdx.x % 64 == 0) k = atomicAdd(ctr, 1u);
+  return gridDim.x + __shfl(k, 0, 64);
 }
 
 // ---- extend: persistent closest hit ---------------------------------------------------
@@ -114,10 +128,11 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
                                                uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ counts) {
   __shared__ uint32_t stack[kStack * 64];
   const uint32_t lane = threadIdx.x;
+  if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   uint32_t *ctr = ctrl + CTRL_FETCH_EXT;
-  for (;;) {
-    const uint32_t base = fetch64(ctr);
+  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctr)) {
+    const uint32_t base = chunk * 64u;
     if (base >= n) break;
     const uint32_t i = base + lane;
     if (i < n) {
@@ -162,6 +177,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
 template <int H>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
+  if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
@@ -270,11 +286,12 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
                                                uint32_t *__restrict__ counts) {
   __shared__ uint32_t stack[kStack * 64];
   const uint32_t lane = threadIdx.x;
+  if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_SHADOW];
   const float4 *__restrict__ sr = reinterpret_cast<const float4 *>(ws.sray);
   const size_t cap = ws.cap;
-  for (;;) {
-    const uint32_t base = fetch64(ctrl + CTRL_FETCH_SHD);
+  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctrl + CTRL_FETCH_SHD)) {
+    const uint32_t base = chunk * 64u;
     if (base >= n) break;
     const uint32_t i = base + lane;
     if (i < n) {
@@ -298,6 +315,48 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
         atomicAdd(counts + 4, c.tris);
         atomicAdd(counts + 5, 1u);
       }
+    }
+  }
+}
+
+// ---- finish: the tail of the queue, one path per lane to its end ------------------------
+template <int H>
+__global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
+                                               uint32_t qsel, uint32_t *__restrict__ ctrl) {
+  __shared__ uint32_t stack[kStack * 64];
+  if (!ctrl[CTRL_TAIL]) return;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  const size_t cap = ws.cap;
+  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctrl + CTRL_FETCH_FIN)) {
+    const uint32_t base = chunk * 64u;
+    if (base >= n) break;
+    const uint32_t q = base + lane;
+    if (q < n) {
+      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
+      Ray r;
+      r.o = mk(a.x, a.y, a.z);
+      r.tmax = a.w;
+      r.d = mk(b.x, b.y, b.z);
+      r.skip = LAM_NONE;
+      const uint32_t p = __float_as_uint(b.w);
+      uint32_t pixel, sample;
+      path_id(ids, ctrl, p, pixel, sample);
+      const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+      Lam<H> lam;
+      lam_for<H>(sc, key, lam);
+      float beta[H], L[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        beta[k] = ws.beta[size_t(k) * cap + p];
+        L[k] = ws.L[size_t(k) * cap + p];
+      }
+      bool ovf = false;
+      path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stack + lane, ovf);
+      if (ovf) atomicOr(ctrl + 15, 1u);
+#pragma unroll
+      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] = L[k];
     }
   }
 }
@@ -340,9 +399,13 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
   hipLaunchKernelGGL((k_raygen<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, 0u);
   ev(1);
   uint32_t q = 0;
-  const uint32_t iters = pa.max_scat + 2;
+  const uint32_t iters = wave_iterations(pa.max_scat, wl);
   for (uint32_t s = 0; s < iters; s++) {
-    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u);
+    const bool last = wl.max_iters && s + 1 == iters;
+    if (prof) (void)hipEventRecord(prof->fin0[s], st);
+    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u, wl.tail_threshold, last ? 1u : 0u);
+    if (wl.max_iters)
+      hipLaunchKernelGGL((k_finish<H>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, pa, ids, ws, s, q, ctrl);
     if (prof) {
       (void)hipEventRecord(prof->ext0[s], st);
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,

@@ -114,7 +114,8 @@ def _check_paths(a, b, rel=1e-4, floor=1e-6):
     assert bad.size == 0, f"{len(bad)} path values out of tolerance: worst {err.max():.3g} at {bad[:3]}"
 
 
-@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_MEGAKERNEL], ids=["wavefront", "megakernel"])
+@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_TAIL, la.LAM_FLAG_MEGAKERNEL],
+                         ids=["default", "wavefront", "megakernel"])
 @pytest.mark.parametrize("scene_name,hero,p,bounces", [
     ("box_scene", 4, 0.8, 8), ("field_scene", 4, 0.8, 12), ("mixed_scene", 16, 0.8, 8),
     ("mixed_scene", 4, 0.5, 4)])
@@ -133,8 +134,9 @@ def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces, flags):
     _check_paths(g, o)
 
 
-@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_GRAPH, la.LAM_FLAG_PROFILE, la.LAM_FLAG_MEGAKERNEL],
-                         ids=["graph", "nograph", "profile", "megakernel"])
+@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_GRAPH, la.LAM_FLAG_PROFILE, la.LAM_FLAG_MEGAKERNEL,
+                                   la.LAM_FLAG_NO_TAIL],
+                         ids=["graph", "nograph", "profile", "megakernel", "notail"])
 @pytest.mark.parametrize("scene_name,hero", [("box_scene", 4), ("mixed_scene", 16)])
 def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero, flags):
     scene = request.getfixturevalue(scene_name)
@@ -148,6 +150,19 @@ def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero, flag
     np.testing.assert_allclose(gx, ox, rtol=2e-4, atol=2e-5 * float(np.abs(ox).max()))
     relmse = np.mean((g - o) ** 2) / np.mean(o ** 2)
     assert relmse < 1e-8, relmse
+
+
+def test_tail_handoff_is_exact(ctx, tmp_path):
+    """262,144 paths: the wavefront runs the first bounces and hands the
+    short queue to the finish kernel; the image must equal the pure
+    wavefront's bit for bit (same per-path arithmetic, same film order)."""
+    from lambda_amd import scenes
+    s = la.load_nls(scenes.box_room(str(tmp_path), seed=1, width=512))
+    ctx.upload(s)
+    a = ctx.render(la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=5))
+    b = ctx.render(la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=5, flags=la.LAM_FLAG_NO_TAIL))
+    c = ctx.render(la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=5, flags=la.LAM_FLAG_MEGAKERNEL))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
 def test_render_against_fp64_truth(ctx, box_scene):
