@@ -111,17 +111,19 @@ def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     nodes, tris = cnt[0::2].astype(np.float64), cnt[1::2].astype(np.float64)
     bytes_per_launch = float(np.sum(32 + 16 + 128 * nodes + 48 * tris))
     ctx.extend_dev(d_rays, n, d_hits)  # warm
-    ctx.reset_timing()
-    for _ in range(reps):
-        ctx.extend_dev(d_rays, n, d_hits)
-    t = ctx.timing()
-    avg_ms = t.extend_ms / t.extend_launches
+    avg_ms = ctx.extend_dev_timed(d_rays, n, d_hits, reps)
     for p in (d_rays, d_hits, d_cnt):
         hip.hipFree(p)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    rec_path = os.path.join(ROOT, "profiles", "extend_icosphere_field_4Mrays.traffic.json")
+    if os.path.exists(rec_path):  # HBM bytes per launch from the committed rocprofv3 --pmc runs
+        rec = json.load(open(rec_path))
+        if rec.get("rays_per_launch") == n:
+            traffic = round(rec["traffic_bytes"])
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
         "kernel": "k_trace_rays<closest>", "rays_per_launch": n, "avg_ms": round(avg_ms, 4),
         "rays_per_s": round(n / (avg_ms * 1e-3), 1), "nodes_per_ray": round(nodes.mean(), 3),
         "tris_per_ray": round(tris.mean(), 3), "bytes_per_launch": bytes_per_launch,

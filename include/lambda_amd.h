@@ -236,6 +236,11 @@ int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n,
  * triangles tested per ray. */
 int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
                    lam_hit *d_hits, uint32_t *d_counts);
+/* Measurement hook: `reps` back-to-back closest-hit launches over device
+ * rays between one pair of HIP events on the context's stream; *avg_ms
+ * receives the mean duration of one launch. */
+int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
+                         lam_hit *d_hits, uint32_t reps, double *avg_ms);
 
 /* Test hook: the device's elementary functions on n inputs (fn 0 sinf,
  * 1 cosf, 2 powf(a,b), 3 atan2f(a,b), 4 sqrtf(a), 5 a/b), for bitwise

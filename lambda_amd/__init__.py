@@ -154,6 +154,7 @@ def lib():
         "lam_extend": (C.c_int, [vp, vp, sz, vp]),
         "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
         "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
+        "lam_extend_dev_timed": (C.c_int, [vp, vp, sz, vp, C.c_uint32, C.POINTER(C.c_double)]),
         "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
@@ -368,6 +369,11 @@ class Context:
 
     def extend_dev(self, rays_ptr, n, hits_ptr, counts_ptr=None):
         self._check(self._lib.lam_extend_dev(self._h, rays_ptr, n, hits_ptr, counts_ptr))
+
+    def extend_dev_timed(self, rays_ptr, n, hits_ptr, reps):
+        ms = C.c_double()
+        self._check(self._lib.lam_extend_dev_timed(self._h, rays_ptr, n, hits_ptr, int(reps), C.byref(ms)))
+        return ms.value
 
     def trace_paths(self, p, pixel, sample):
         pixel = np.ascontiguousarray(pixel, dtype=np.uint32)

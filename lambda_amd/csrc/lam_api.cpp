@@ -492,6 +492,28 @@ int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hit
   return check_err_flag(ctx);
 }
 
+int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hits, uint32_t reps,
+                         double *avg_ms) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!d_rays || !d_hits || !avg_ms || reps == 0 || n == 0) return fail(ctx, LAM_E_INVALID, "bad argument");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t r = 0; r < reps; r++)
+    HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, nullptr, ctx->d_err, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  const double ms = event_ms(ctx);
+  *avg_ms = ms / reps;
+  ctx->timing.extend_ms += ms;
+  ctx->timing.extend_launches += reps;
+  ctx->timing.extend_rays += n * reps;
+  return check_err_flag(ctx);
+}
+
 int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pixel, const uint32_t *sample,
                     size_t n, float *out) {
   if (!ctx) return LAM_E_INVALID;
