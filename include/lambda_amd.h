@@ -225,6 +225,32 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p,
                     const uint32_t *pixel, const uint32_t *sample, size_t n,
                     float *radiance_out);
 
+/* ---- differentiation (path-replay adjoint) ---------------------------- */
+/* Which material parameters are differentiated, and where their gradients
+ * go in the output vector.  Per material (n_materials entries each; -1 =
+ * not differentiated):
+ *   mat_d       thin-film thickness d (thinfilm::d, cg.hpp:998)      1 slot
+ *   mat_ior     thin-film film IOR coefficients (film_ior, cg.hpp:997) grid.n slots
+ *   mat_albedo  Lambertian albedo coefficients (albedo, cg.hpp:923)    grid.n slots
+ * Gradients with respect to a material's coefficient spectrum are per grid
+ * coefficient; a flat spectrum's scalar gradient is their sum. */
+typedef struct {
+  uint32_t n_params;
+  const int32_t *mat_d;
+  const int32_t *mat_ior;
+  const int32_t *mat_albedo;
+} lam_diff_params;
+
+/* Gradient of an image loss with respect to the parameters above:
+ * grad[j] = Σ_pixels Σ_c dloss_drgb[px][c] · ∂rgb[px][c]/∂θ_j for the image
+ * lam_render(p) produces (same seed, spp, spp_offset).  dloss_drgb: w*h*3,
+ * top row first (as lam_render's output).  Each path is replayed from its
+ * counter-RNG key, so device memory does not grow with path length.
+ * Sampling decisions are held fixed (none of these parameters changes a
+ * sampled direction, a pdf or a Russian-roulette decision). */
+int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params *dp,
+                const float *dloss_drgb, float *grad);
+
 /* Closest-hit query, intersect::scene (cg.hpp:1469-1489). */
 int lam_extend(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit *hits);
 /* Shadow query: occluded[i] = 1 iff a hit exists in [BIAS, tmax]

@@ -102,6 +102,11 @@ class SceneDesc(C.Structure):
                 ("tri_prim", C.POINTER(C.c_uint32)), ("camera", Camera)]
 
 
+class DiffParamsC(C.Structure):
+    _fields_ = [("n_params", C.c_uint32), ("mat_d", C.POINTER(C.c_int32)),
+                ("mat_ior", C.POINTER(C.c_int32)), ("mat_albedo", C.POINTER(C.c_int32))]
+
+
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_scatterings", C.c_uint32), ("sample_p", C.c_float),
                 ("hero", C.c_uint32), ("seed", C.c_uint64), ("spp_offset", C.c_uint32),
@@ -151,6 +156,7 @@ def lib():
         "lam_get_scene_stats": (C.c_int, [vp, C.POINTER(SceneStats)]),
         "lam_render": (C.c_int, [vp, C.POINTER(RenderParams), fp, fp]),
         "lam_trace_paths": (C.c_int, [vp, C.POINTER(RenderParams), u32p, u32p, sz, fp]),
+        "lam_adjoint": (C.c_int, [vp, C.POINTER(RenderParams), C.POINTER(DiffParamsC), fp, fp]),
         "lam_extend": (C.c_int, [vp, vp, sz, vp]),
         "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
         "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
@@ -307,6 +313,143 @@ class Scene:
         return None if v is None else v.decode()
 
 
+class SceneVariant:
+    """A scene description whose materials and spectrum table replace those of
+    a base scene (geometry, lights and camera are the base scene's, which
+    must outlive the variant).  Accepted wherever a Scene is uploaded."""
+
+    def __init__(self, base, materials, spectra):
+        self.base = base
+        self._mats = (Material * len(materials))(*materials)
+        self._spec = np.ascontiguousarray(spectra, dtype=np.float32)
+        d = SceneDesc()
+        C.memmove(C.byref(d), base.desc_ptr, C.sizeof(SceneDesc))
+        d.n_spectra = self._spec.shape[0]
+        d.spectra = _fptr(self._spec)
+        d.n_materials = len(materials)
+        d.materials = C.cast(self._mats, C.POINTER(Material))
+        self._desc = d
+
+    @property
+    def desc_ptr(self):
+        return C.pointer(self._desc)
+
+    @property
+    def desc(self):
+        return self._desc
+
+    @property
+    def hero(self):
+        return self.base.hero
+
+    @property
+    def width(self):
+        return int(self._desc.camera.width)
+
+    @property
+    def height(self):
+        return int(self._desc.camera.height)
+
+    def materials(self):
+        return [self._mats[i] for i in range(len(self._mats))]
+
+    def spectra(self):
+        return self._spec
+
+
+def _copy_material(m):
+    c = Material()
+    C.memmove(C.byref(c), C.byref(m), C.sizeof(Material))
+    return c
+
+
+class DiffParams:
+    """The differentiated parameters (lam_diff_params) of one scene: a slot
+    for the thin-film thickness d of each material in `d`, grid.n slots for
+    the film-IOR coefficients of each material in `ior` (thin film) and for
+    the albedo coefficients of each material in `albedo` (Lambertian), in
+    that order.  Materials are given by index or by name."""
+
+    def __init__(self, scene, d=(), ior=(), albedo=()):
+        desc = scene.desc
+        nm, gn = desc.n_materials, desc.grid.n
+        mats = [desc.materials[i] for i in range(nm)]
+        names = {}
+        base = scene.base if isinstance(scene, SceneVariant) else scene
+        for i in range(nm):
+            n = base.material_name(i)
+            if n is not None:
+                names[n] = i
+
+        def idx(m):
+            if isinstance(m, str):
+                if m not in names:
+                    raise KeyError(f"no material named {m!r}")
+                return names[m]
+            m = int(m)
+            if not 0 <= m < nm:
+                raise IndexError(f"material {m} out of range")
+            return m
+
+        self.n_materials, self.grid_n = nm, gn
+        self.mat_d = np.full(nm, -1, dtype=np.int32)
+        self.mat_ior = np.full(nm, -1, dtype=np.int32)
+        self.mat_albedo = np.full(nm, -1, dtype=np.int32)
+        off = 0
+        for lst, arr, kind, width in ((d, self.mat_d, MAT_THINFILM, 1), (ior, self.mat_ior, MAT_THINFILM, gn),
+                                      (albedo, self.mat_albedo, MAT_LAMBERTIAN, gn)):
+            for m in lst:
+                i = idx(m)
+                if mats[i].type != kind:
+                    raise ValueError(f"material {i} has type {mats[i].type}, not {kind}")
+                if arr[i] >= 0:
+                    raise ValueError(f"material {i} listed twice")
+                arr[i] = off
+                off += width
+        self.n_params = off
+
+    def c_struct(self):
+        i32 = C.POINTER(C.c_int32)
+        return DiffParamsC(self.n_params, self.mat_d.ctypes.data_as(i32), self.mat_ior.ctypes.data_as(i32),
+                           self.mat_albedo.ctypes.data_as(i32))
+
+    def values(self, scene):
+        """The parameter vector θ of a scene (or variant)."""
+        mats = scene.materials()
+        spec = scene.spectra()
+        th = np.zeros(self.n_params, dtype=np.float32)
+        gn = self.grid_n
+        for m in range(self.n_materials):
+            if self.mat_d[m] >= 0:
+                th[self.mat_d[m]] = mats[m].d
+            if self.mat_ior[m] >= 0:
+                th[self.mat_ior[m]:self.mat_ior[m] + gn] = spec[mats[m].spec[0]]
+            if self.mat_albedo[m] >= 0:
+                th[self.mat_albedo[m]:self.mat_albedo[m] + gn] = spec[mats[m].spec[0]]
+        return th
+
+    def apply(self, scene, theta):
+        """A SceneVariant of `scene` with the parameters set to θ; each
+        differentiated spectrum gets a row of its own."""
+        base = scene.base if isinstance(scene, SceneVariant) else scene
+        theta = np.asarray(theta, dtype=np.float32)
+        if theta.shape != (self.n_params,):
+            raise ValueError(f"theta has shape {theta.shape}, expected ({self.n_params},)")
+        mats = [_copy_material(m) for m in scene.materials()]
+        spec = [np.asarray(scene.spectra(), dtype=np.float32)]
+        rows = spec[0].shape[0]
+        gn = self.grid_n
+        for m in range(self.n_materials):
+            if self.mat_d[m] >= 0:
+                mats[m].d = float(theta[self.mat_d[m]])
+            for arr in (self.mat_ior, self.mat_albedo):
+                if arr[m] >= 0:
+                    spec.append(theta[arr[m]:arr[m] + gn][None, :])
+                    mats[m].spec[0] = rows
+                    rows += 1
+        return SceneVariant(base, mats, np.concatenate(spec, axis=0))
+
+
 def load_nls(path, data_dir=None):
     """nl::cg::load::loadNLS (lib/numli/cg.hpp:2455-2470)."""
     return Scene.load(path, data_dir)
@@ -391,6 +534,19 @@ class Context:
         self._check(self._lib.lam_render(self._h, C.byref(p), _fptr(rgb),
                                          _fptr(xyz) if want_xyz else None))
         return (rgb, xyz) if want_xyz else rgb
+
+    def adjoint(self, p, dp, dloss_drgb):
+        """lam_adjoint: gradient of an image loss with respect to the
+        parameters of `dp` (a DiffParams) at the uploaded scene, given
+        ∂loss/∂rgb (h x w x 3, as render's output)."""
+        d = self.scene.desc
+        g = np.ascontiguousarray(dloss_drgb, dtype=np.float32)
+        if g.shape != (d.camera.height, d.camera.width, 3):
+            raise ValueError(f"dloss_drgb has shape {g.shape}")
+        grad = np.zeros(dp.n_params, dtype=np.float32)
+        cs = dp.c_struct()
+        self._check(self._lib.lam_adjoint(self._h, C.byref(p), C.byref(cs), _fptr(g), _fptr(grad)))
+        return grad
 
     def timing(self):
         t = Timing()

@@ -590,6 +590,61 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   return check_err_flag(ctx);
 }
 
+int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params *dp, const float *dloss_drgb,
+                float *grad) {
+  if (!ctx) return LAM_E_INVALID;
+  int rc;
+  if ((rc = check_params(ctx, p))) return rc;
+  if (!dp || !dloss_drgb || !grad) return fail(ctx, LAM_E_INVALID, "NULL argument");
+  if (!dp->mat_d || !dp->mat_ior || !dp->mat_albedo) return fail(ctx, LAM_E_INVALID, "parameter maps missing");
+  if (dp->n_params == 0) return LAM_OK;
+  if (dp->n_params > lam::kMaxDiffParams)
+    return fail(ctx, LAM_E_LIMIT, "at most " + std::to_string(lam::kMaxDiffParams) + " parameters");
+  const uint32_t nm = ctx->sc.n_materials, gn = ctx->sc.grid_n;
+  for (uint32_t m = 0; m < nm; m++) {
+    const int32_t d = dp->mat_d[m], io = dp->mat_ior[m], al = dp->mat_albedo[m];
+    if (d >= 0 && uint32_t(d) >= dp->n_params) return fail(ctx, LAM_E_INVALID, "mat_d index out of range");
+    if (io >= 0 && uint64_t(io) + gn > dp->n_params) return fail(ctx, LAM_E_INVALID, "mat_ior range out of range");
+    if (al >= 0 && uint64_t(al) + gn > dp->n_params)
+      return fail(ctx, LAM_E_INVALID, "mat_albedo range out of range");
+  }
+  HIPC(hipSetDevice(ctx->device));
+  const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
+  const size_t npix = size_t(w) * h;
+  if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
+  // scratch: [0] parameter maps, [1] dloss, [2] gradient + fetch counter
+  const size_t maps = size_t(nm) * 3 * 4;
+  if ((rc = ensure_scratch(ctx, 0, maps))) return rc;
+  if ((rc = ensure_scratch(ctx, 1, npix * 12))) return rc;
+  if ((rc = ensure_scratch(ctx, 2, size_t(dp->n_params) * 4 + 64))) return rc;
+  int32_t *dmaps = static_cast<int32_t *>(ctx->scratch[0]);
+  HIPC(hipMemcpyAsync(dmaps, dp->mat_d, nm * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dmaps + nm, dp->mat_ior, nm * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dmaps + 2 * nm, dp->mat_albedo, nm * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->scratch[1], dloss_drgb, npix * 12, hipMemcpyHostToDevice, ctx->stream));
+  float *dgrad = static_cast<float *>(ctx->scratch[2]);
+  const size_t gbytes = (size_t(dp->n_params) * 4 + 15) / 16 * 16;
+  uint32_t *ctr = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ctx->scratch[2]) + gbytes);
+  HIPC(lam::launch_zero(dgrad, gbytes, ctx->stream));
+  if ((rc = clear_err_flag(ctx))) return rc;
+  int cus = 0;
+  HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const uint32_t blocks = uint32_t(cus > 0 ? cus : 256) * 8u;
+  const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t s = 0; s < p->spp; s++) {
+    HIPC(lam::launch_zero(ctr, 16, ctx->stream));
+    HIPC(lam::launch_adjoint(int(p->hero), ctx->sc, path_args(p), dmaps, dmaps + nm, dmaps + 2 * nm, dp->n_params,
+                             p->spp_offset + s, uint32_t(npix), inv_n, static_cast<float *>(ctx->scratch[1]), dgrad,
+                             ctr, ctx->d_err, blocks, ctx->stream));
+  }
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPC(hipMemcpyAsync(grad, dgrad, size_t(dp->n_params) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->timing.render_ms = event_ms(ctx);
+  return check_err_flag(ctx);
+}
+
 int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out) {
   if (!ctx) return LAM_E_INVALID;
   if (n == 0) return LAM_OK;

@@ -61,6 +61,13 @@ hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const 
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
                         WaveProfile *prof);
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
+
+// ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------
+constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block
+hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
+                          const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
+                          uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
+                          uint32_t blocks, hipStream_t st);
 hipError_t enqueue_pass_next(uint32_t *ctrl, hipStream_t st);
 
 }  // namespace lam

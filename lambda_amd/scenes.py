@@ -15,6 +15,13 @@ OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
                      nm, film IOR U[1.33,1.6] over base 1.5); two area
                      lights; 1920x1080.
 
+  box_room_films(dir) the box room with each of its 10 quads and 3 face groups
+                     of each block given a Lambertian material of its own (16
+                     albedos), plus 64 small thin-film spheres (d U[200,800]
+                     nm, film IOR U[1.33,1.6] over base 1.5) on a jittered
+                     8 x 8 grid in front of the back wall; 512x512.  The
+                     workload of the path-replay adjoint.
+
 Each function returns the path of the .nls file.  Same seed, same bytes.
 """
 from __future__ import annotations
@@ -163,6 +170,86 @@ def box_room(out_dir, seed=1, width=256, hero=4):
         "materials": materials,
     }
     path = os.path.join(out_dir, "box_room.nls")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
+
+
+def box_room_films(out_dir, seed=3, width=512, hero=4, n_films=64, film_freq=3, film_radius=0.06,
+                   glossiness=32):
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    lam_ref = np.arange(355.0, 836.0, 5.0)
+    lam_bb = np.arange(355.0, 841.0, 1.0)
+    bb = blackbody(6500.0, lam_bb)
+    write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
+    P = lambda x, y, z: (x, y, z)
+    h = 0.25
+    quads = {  # the 10 quads of the room, normals into the room
+        "floor": (P(-1, -1, 0), P(1, -1, 0), P(1, 1, 0), P(-1, 1, 0)),
+        "ceiling0": (P(-1, -1, 2), P(-h, -1, 2), P(-h, 1, 2), P(-1, 1, 2)),
+        "ceiling1": (P(h, -1, 2), P(1, -1, 2), P(1, 1, 2), P(h, 1, 2)),
+        "ceiling2": (P(-h, -1, 2), P(h, -1, 2), P(h, -h, 2), P(-h, -h, 2)),
+        "ceiling3": (P(-h, h, 2), P(h, h, 2), P(h, 1, 2), P(-h, 1, 2)),
+        "back0": (P(-1, 1, 0), P(-1, 1, 2), P(0, 1, 2), P(0, 1, 0)),
+        "back1": (P(0, 1, 0), P(0, 1, 2), P(1, 1, 2), P(1, 1, 0)),
+        "left": (P(-1, -1, 0), P(-1, -1, 2), P(-1, 1, 2), P(-1, 1, 0)),
+        "right": (P(1, 1, 0), P(1, 1, 2), P(1, -1, 2), P(1, -1, 0)),
+        "front": (P(1, -1, 0), P(1, -1, 2), P(-1, -1, 2), P(-1, -1, 0)),
+    }
+
+    def inward(q):
+        a, b, c = (np.array(x, dtype=float) for x in q[:3])
+        n = np.cross(b - a, c - a)
+        center = np.mean(np.array(q, dtype=float), axis=0)
+        return q if np.dot(n, np.array([0, 0, 1.0]) - center) > 0 else tuple(reversed(q))
+
+    groups = {nm: [inward(q)] for nm, q in quads.items()}
+    for nm, (x0, y0, x1, y1, hh) in {"tall": (-0.7, 0.0, -0.1, 0.6, 1.2),
+                                     "short": (0.1, -0.5, 0.7, 0.1, 0.6)}.items():
+        top, my, py, mx, px, bot = _block(x0, y0, x1, y1, hh)
+        groups[f"{nm}_top"] = [top]
+        groups[f"{nm}_x"] = [mx, px, bot]
+        groups[f"{nm}_y"] = [my, py]
+    objects, materials = [], []
+    for nm, qs in groups.items():
+        write_csv(os.path.join(out_dir, f"albedo_{nm}.csv"), lam_ref, smooth_reflectance(rng, lam_ref))
+        materials.append({"name": f"mat_{nm}", "type": "lambertian",
+                          "albedo": {"val": os.path.join(out_dir, f"albedo_{nm}.csv")}})
+        V, F, N = _mesh_from_quads(qs)
+        write_obj(os.path.join(out_dir, f"{nm}.obj"), V, F, N)
+        objects.append({"name": nm, "type": "trimesh", "transform": {}, "material": f"mat_{nm}",
+                        "source": os.path.join(out_dir, f"{nm}.obj")})
+    V, F, N = geodesic_sphere(film_freq, seed, amp=0.0)
+    write_obj(os.path.join(out_dir, "film_sphere.obj"), V, F, N)
+    side = int(round(math.sqrt(n_films)))
+    for i in range(n_films):
+        r, c = divmod(i, side)
+        x = -0.8 + 1.6 * (c + 0.5 + rng.uniform(-0.25, 0.25)) / side
+        z = 0.25 + 1.5 * (r + 0.5 + rng.uniform(-0.25, 0.25)) / max(side, (n_films + side - 1) // side)
+        y = float(rng.uniform(0.7, 0.9))
+        name = f"film{i}"
+        materials.append({"name": name, "type": "thinfilm",
+                          "film_ior": {"val": round(float(rng.uniform(1.33, 1.6)), 4)},
+                          "base_ior": {"val": 1.5},
+                          "thickness": round(float(rng.uniform(200, 800)), 2),
+                          "glossiness": glossiness})
+        objects.append({"name": f"sphere{i}", "type": "trimesh", "material": name,
+                        "source": os.path.join(out_dir, "film_sphere.obj"),
+                        "transform": {"scale": [film_radius] * 3, "translate": [float(x), y, float(z)]}})
+    scene = {
+        "spectral_grid": GRID,
+        "hero_samples": hero,
+        "camera": {"pos": [0, -0.95, 1.0], "look_at": [0, 1, 1.0], "up": [0, 0, 1],
+                   "fov": 70, "ar": 1.0, "width": width},
+        "lights": [{"name": "ceiling-light", "type": "quaddir", "specular": 1,
+                    "transform": {"scale": [h, h, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
+                                  "translate": [0, 0, 2.0]},
+                    "radiance": {"val": os.path.join(out_dir, "blackbody6500.csv"), "mult": 6.0}}],
+        "objects": objects,
+        "materials": materials,
+    }
+    path = os.path.join(out_dir, "box_room_films.nls")
     with open(path, "w") as f:
         json.dump(scene, f, indent=1)
     return path

@@ -35,6 +35,9 @@ def _load(f64):
     L.orc_trace_paths.argtypes = [vp, vp, u32p, u32p, sz, fp]
     L.orc_render.restype = C.c_int
     L.orc_render.argtypes = [vp, vp, fp, fp, C.c_int, C.c_int, C.c_int]
+    i32p, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    L.orc_jvp.restype = C.c_int
+    L.orc_jvp.argtypes = [vp, vp, i32p, i32p, i32p, C.c_uint32, C.c_int, dp, fp, C.c_int, dp]
     L.orc_math.restype = None
     L.orc_math.argtypes = [C.c_int, fp, fp, sz, fp]
     L.orc_rng.restype = None
@@ -108,6 +111,26 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle render failed")
         return (rgb, xyz) if want_xyz else rgb
+
+
+    def jvp(self, p, dp, v, dloss, threads=0):
+        """Directional derivatives of the image loss Σ dloss·rgb along the rows
+        of v (K x n_params, K <= 16) by forward-mode tangents; dp is a
+        lambda_amd.DiffParams."""
+        v = np.ascontiguousarray(np.atleast_2d(v), dtype=np.float64)
+        K = v.shape[0]
+        if v.shape[1] != dp.n_params or not 1 <= K <= 16:
+            raise ValueError(f"tangents have shape {v.shape}")
+        g = np.ascontiguousarray(dloss, dtype=np.float32)
+        out = np.zeros(K, dtype=np.float64)
+        i32 = C.POINTER(C.c_int32)
+        rc = self.L.orc_jvp(self._h, C.cast(C.pointer(p), C.c_void_p), dp.mat_d.ctypes.data_as(i32),
+                            dp.mat_ior.ctypes.data_as(i32), dp.mat_albedo.ctypes.data_as(i32), dp.n_params, K,
+                            v.ctypes.data_as(C.POINTER(C.c_double)), _f(g), int(threads),
+                            out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError("oracle jvp failed")
+        return out
 
 
 def math(fn, a, b=None):

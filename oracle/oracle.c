@@ -925,7 +925,88 @@ typedef struct {
   real p;
 } opath;
 
-static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro, v3 rd, uint32_t sct, real *Li) {
+/* ---- forward-mode tangents: the checker of the path-replay adjoint ------------------
+ * Directional derivatives of the path radiance along K tangent directions of
+ * the differentiated parameters (thin-film d, film IOR coefficients,
+ * Lambertian albedo coefficients; the layout of lam_diff_params).  Sampling
+ * decisions do not depend on these parameters, so the tangent of the
+ * recursive estimator is the recursion of the tangents of its factors. */
+#define KMAX 16
+#define DPI 3.14159265358979323846
+typedef struct {
+  const int32_t *mat_d, *mat_ior, *mat_albedo; /* per material; -1: not a parameter */
+  const double *v;                             /* K directions of n_params entries */
+  uint32_t n_params;
+  int K;
+} otan;
+
+/* spec1's interpolation applied to tangent coefficients */
+static double spec1_tan(const orc_scene *s, const double *row, real l) {
+  const lam_scene_desc *d = s->d;
+  real x, t;
+  uint32_t i;
+  if (l <= (real)d->grid.inf) return row[0];
+  if (l >= (real)d->grid.sup) return row[d->grid.n - 1];
+  x = (l - (real)d->grid.inf) / s->delta;
+  i = (uint32_t)x;
+  if (i > d->grid.n - 2) i = d->grid.n - 2;
+  t = x - (real)i;
+  return (1 - (double)t) * row[i] + (double)t * row[i + 1];
+}
+
+/* thinfilm::BRDFcosθ (cg.hpp:1031-1051) in double with libm as a function of
+ * (d, n1) at fixed geometry (ci = i·h, hn = h·n) */
+static double tf_val(double alpha, double d, double n1, double n2, double l, double ci, double hn) {
+  double n01 = 1 / n1, cos_l = sqrt(1 - n01 * n01) * (1 - ci * ci);
+  double cos_d = cos(4 * DPI * n1 * d * cos_l / l);
+  double r001 = (1 - n1) * (1 - n1) / ((1 + n1) * (1 + n1));
+  double r012 = (n1 - n2) * (n1 - n2) / ((n1 + n2) * (n1 + n2));
+  double pw5 = pow(1 - ci, 5);
+  double R01 = r001 + (1 - r001) * pw5, R12 = r012 + (1 - r012) * pw5, ravg = 2 * sqrt(R01 * R12);
+  return (R01 + R12 + ravg * cos_d) / (R01 * R12 + 1 + ravg * cos_d) * (alpha + 2) * 0.125 / DPI * pow(hn, alpha);
+}
+
+/* tangents of BxDFcosθ of material mi along the K directions: exact for the
+ * albedo (linear), fp64 central differences for the thin film */
+static void bxdf_tan(const orc_scene *s, uint32_t mi, const lam_material *m, const olam *L, v3 i, v3 o, v3 n,
+                     int front, const otan *tn, double *out) {
+  int j, k, H = L->H;
+  for (j = 0; j < tn->K; j++)
+    for (k = 0; k < H; k++) out[j * HMAX + k] = 0;
+  if (m->type == LAM_MAT_LAMBERTIAN && tn->mat_albedo[mi] >= 0) {
+    double md = (double)rmax(0, vdot(i, n)) / DPI;
+    for (j = 0; j < tn->K; j++) {
+      const double *row = tn->v + (size_t)j * tn->n_params + tn->mat_albedo[mi];
+      for (k = 0; k < H; k++) out[j * HMAX + k] = spec1_tan(s, row, L->l[k]) * md;
+    }
+  } else if (m->type == LAM_MAT_THINFILM && (tn->mat_d[mi] >= 0 || tn->mat_ior[mi] >= 0)) {
+    v3 h;
+    double ci, hn, d = m->d;
+    if (!front || vdot(i, n) < 0 || vdot(o, n) < 0) return;
+    h = half_vec(i, o, n);
+    ci = (double)vdot(i, h);
+    hn = (double)vdot(h, n);
+    for (k = 0; k < H; k++) {
+      double n1 = (double)spec1(s, m->spec[0], L->l[k]), n2 = (double)spec1(s, m->spec[1], L->l[k]);
+      double l = (double)L->l[k];
+      for (j = 0; j < tn->K; j++) {
+        const double *vj = tn->v + (size_t)j * tn->n_params;
+        double dd = tn->mat_d[mi] >= 0 ? vj[tn->mat_d[mi]] : 0;
+        double dn = tn->mat_ior[mi] >= 0 ? spec1_tan(s, vj + tn->mat_ior[mi], L->l[k]) : 0;
+        double sc = fabs(dd) / (fabs(d) > 1 ? fabs(d) : 1), e;
+        if (fabs(dn) / n1 > sc) sc = fabs(dn) / n1;
+        if (sc == 0) continue;
+        e = 1e-6 / sc;
+        out[j * HMAX + k] = (tf_val(m->alpha, d + e * dd, n1 + e * dn, n2, l, ci, hn) -
+                             tf_val(m->alpha, d - e * dd, n1 - e * dn, n2, l, ci, hn)) / (2 * e);
+      }
+    }
+  }
+}
+
+/* tn, dLi: optional tangents (dLi[j*HMAX+k] for direction j) */
+static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro, v3 rd, uint32_t sct, real *Li,
+                       const otan *tn, double *dLi) {
   const lam_scene_desc *d = s->d;
   oray r;
   ohit h;
@@ -937,7 +1018,10 @@ static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro
   omsample ms;
   real coef[HMAX], lis[HMAX], nxt[HMAX], pl, p_L_mati, p_mat_Li, p_L_i, w_L, w_mat;
   uint32_t li;
-  int k, H = L->H;
+  int j, k, H = L->H, K = tn ? tn->K : 0;
+  double dcoef[KMAX * HMAX], dnxt[KMAX * HMAX], dmult[KMAX * HMAX];
+  for (j = 0; j < K; j++)
+    for (k = 0; k < H; k++) dLi[j * HMAX + k] = 0;
   r.o = ro; r.d = rd; r.tmax = UBF; r.skip = NONE;
   trace(s, &r, 0, 1, &h);
   if (h.obj == NONE) { for (k = 0; k < H; k++) Li[k] = 0; return; }
@@ -959,16 +1043,29 @@ static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro
   sample_light(s, lt, L, &sf, (real)om_flt(pa->key, om_vdim(sct, 1)), (real)om_flt(pa->key, om_vdim(sct, 2)), &ls);
   bxdf(s, mat, L, ls.dir, o, n, sf.front, coef);
   for (k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+  if (K) bxdf_tan(s, sf.mat, mat, L, ls.dir, o, n, sf.front, tn, dcoef);
   sample_material(s, mat, L, &sf, o, (real)om_flt(pa->key, om_vdim(sct, 3)), (real)om_flt(pa->key, om_vdim(sct, 4)),
                   (real)om_flt(pa->key, om_vdim(sct, 5)), pa->p, &ms);
-  if (!ms.ok) { for (k = 0; k < H; k++) Li[k] = lis[k]; return; }
-  trace_path(s, pa, L, sf.p, ms.dir, sct + 1, nxt);
+  if (!ms.ok) {
+    for (k = 0; k < H; k++) Li[k] = lis[k];
+    for (j = 0; j < K; j++)
+      for (k = 0; k < H; k++) dLi[j * HMAX + k] = (double)ls.mult[k] * dcoef[j * HMAX + k] / ((double)ls.prob * pl);
+    return;
+  }
+  trace_path(s, pa, L, sf.p, ms.dir, sct + 1, nxt, tn, dnxt);
+  if (K) bxdf_tan(s, sf.mat, mat, L, ms.dir, o, n, sf.front, tn, dmult);
   p_L_mati = prob_light(s, lt, sf.p, ms.dir) * pl;
   p_mat_Li = prob_material(s, mat, &sf, ls.dir, o, L->l[0], pa->p);
   p_L_i = ls.prob * pl;
   w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
   w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
   for (k = 0; k < H; k++) Li[k] = nxt[k] * ms.mult[k] / ms.prob * w_mat + lis[k] * w_L;
+  for (j = 0; j < K; j++)
+    for (k = 0; k < H; k++) {
+      const int q = j * HMAX + k;
+      dLi[q] = (dnxt[q] * (double)ms.mult[k] + (double)nxt[k] * dmult[q]) / (double)ms.prob * (double)w_mat +
+               (double)ls.mult[k] * dcoef[q] / ((double)ls.prob * pl) * (double)w_L;
+    }
 }
 
 /* camera ray + hero wavelengths for (pixel, sample) (src/renderer.cpp:35-40,
@@ -1014,26 +1111,33 @@ int orc_trace_paths(const orc_scene *s, const lam_render_params *p, const uint32
     pa.max_scat = p->max_scatterings;
     pa.p = (real)p->sample_p;
     start_path(s, pa.key, pixel[i], H, &L, &ro, &rd);
-    trace_path(s, &pa, &L, ro, rd, 0, Li);
+    trace_path(s, &pa, &L, ro, rd, 0, Li, NULL, NULL);
     for (k = 0; k < H; k++) out[i * H + k] = (float)Li[k];
   }
   return 0;
 }
 
-/* coefficientλ::addHeroλ (cg.hpp:306-320), weights exactly as the reference */
-static void add_hero(const orc_scene *s, real *v, const olam *L, const real *sv) {
+/* coefficientλ::addHeroλ (cg.hpp:306-320), weights exactly as the reference:
+ * wavelength l adds (1-t)·s to bin i-1 (if i > 0) and t·s to bin i (if i < n) */
+static void hero_weights(const orc_scene *s, real l, uint32_t *bin, real *tw) {
   const lam_scene_desc *d = s->d;
-  uint32_t n = d->grid.n;
+  uint32_t n = d->grid.n, i = 0;
+  real l1, t;
+  for (; (real)((float)d->grid.inf + (float)i * (float)s->delta) <= l && i < n; i++) {
+  }
+  l1 = (real)((float)d->grid.inf + (float)i * (float)s->delta);
+  t = (l1 - l) / s->delta;
+  if (i == 0) t = 1;
+  if (i == n) t = 0;
+  *bin = i;
+  *tw = t;
+}
+static void add_hero(const orc_scene *s, real *v, const olam *L, const real *sv) {
+  uint32_t n = s->d->grid.n, i;
+  real t;
   int k;
   for (k = 0; k < L->H; k++) {
-    uint32_t i = 0;
-    real l1, t;
-    for (; (real)((float)d->grid.inf + (float)i * (float)s->delta) <= L->l[k] && i < n; i++) {
-    }
-    l1 = (real)((float)d->grid.inf + (float)i * (float)s->delta);
-    t = (l1 - L->l[k]) / s->delta;
-    if (i == 0) t = 1;
-    if (i == n) t = 0;
+    hero_weights(s, L->l[k], &i, &t);
     if (i > 0) v[i - 1] += (1 - t) * sv[k];
     if (i < n) v[i] += t * sv[k];
   }
@@ -1076,7 +1180,7 @@ int orc_render(const orc_scene *s, const lam_render_params *p, float *linrgb, fl
         pa.max_scat = p->max_scatterings;
         pa.p = (real)p->sample_p;
         prob = start_path(s, pa.key, (uint32_t)(i * w + j), H, &L, &ro, &rd);
-        trace_path(s, &pa, &L, ro, rd, 0, Li);
+        trace_path(s, &pa, &L, ro, rd, 0, Li, NULL, NULL);
         for (m = 0; m < H; m++) sv[m] = Li[m] / prob;
         add_hero(s, acc, &L, sv);
       }
@@ -1101,6 +1205,88 @@ int orc_render(const orc_scene *s, const lam_render_params *p, float *linrgb, fl
       b = (real)0.0556434f * X - (real)0.2040259f * Y + (real)1.0572252f * Z;
       linrgb[o] = (float)r; linrgb[o + 1] = (float)g; linrgb[o + 2] = (float)b;
     }
+  return 0;
+}
+
+/* Directional derivatives of the image loss: out[j] = Σ_px Σ_c dloss[px][c] ·
+ * d rgb[px][c] / dε along direction j (v + j*n_params) for the image orc_render
+ * produces, 1 <= K <= 16.  The film is linear in the path radiance, so each
+ * path's radiance tangent is splatted, projected and converted like its
+ * radiance (src/renderer.cpp:14-61, cg.hpp:144-148, 306-341).  Per-pixel sums
+ * are added in pixel order, so the result does not depend on threading. */
+int orc_jvp(const orc_scene *s, const lam_render_params *p, const int32_t *mat_d, const int32_t *mat_ior,
+            const int32_t *mat_albedo, uint32_t n_params, int K, const double *v, const float *dloss, int threads,
+            double *out) {
+  const lam_scene_desc *d = s->d;
+  long w = d->camera.width, h = d->camera.height, px;
+  int H = (int)p->hero, j;
+  uint32_t nb = d->grid.n, m;
+  double *part;
+  otan tn;
+  if (H != 4 && H != 16) return -1;
+  if (nb > 512 || K < 1 || K > KMAX) return -1;
+  for (m = 0; m < d->n_materials; m++) {
+    if (mat_d[m] >= 0 && (uint32_t)mat_d[m] >= n_params) return -1;
+    if (mat_ior[m] >= 0 && (uint64_t)mat_ior[m] + nb > n_params) return -1;
+    if (mat_albedo[m] >= 0 && (uint64_t)mat_albedo[m] + nb > n_params) return -1;
+  }
+  part = (double *)malloc(sizeof(double) * (size_t)(w * h) * (size_t)K);
+  if (!part) return -1;
+  tn.mat_d = mat_d; tn.mat_ior = mat_ior; tn.mat_albedo = mat_albedo; tn.v = v; tn.n_params = n_params; tn.K = K;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+  for (px = 0; px < w * h; px++) {
+    long i = px / w, jj = px - i * w;
+    double dacc[KMAX][512], N = (double)(float)((uint64_t)p->spp * (uint64_t)H);
+    size_t o = (size_t)((h - i - 1) * w + jj) * 3;
+    uint32_t k, q, bin;
+    int a;
+    for (a = 0; a < K; a++)
+      for (q = 0; q < nb; q++) dacc[a][q] = 0;
+    for (k = 0; k < p->spp; k++) {
+      opath pa;
+      olam L;
+      v3 ro, rd;
+      real Li[HMAX], prob, t;
+      double dLi[KMAX * HMAX];
+      int mm;
+      pa.key = om_path_key(p->seed, (uint32_t)px, p->spp_offset + k);
+      pa.max_scat = p->max_scatterings;
+      pa.p = (real)p->sample_p;
+      prob = start_path(s, pa.key, (uint32_t)px, H, &L, &ro, &rd);
+      trace_path(s, &pa, &L, ro, rd, 0, Li, &tn, dLi);
+      for (mm = 0; mm < H; mm++) {
+        hero_weights(s, L.l[mm], &bin, &t);
+        for (a = 0; a < K; a++) {
+          double sv = dLi[a * HMAX + mm] / (double)prob;
+          if (bin > 0) dacc[a][bin - 1] += (1 - (double)t) * sv;
+          if (bin < nb) dacc[a][bin] += (double)t * sv;
+        }
+      }
+    }
+    for (a = 0; a < K; a++) {
+      double X = 0, Y = 0, Z = 0, r, g, b;
+      for (q = 0; q < nb; q++) {
+        double vn = dacc[a][q] / N;
+        X += vn * (double)d->cmf[q];
+        Y += vn * (double)d->cmf[nb + q];
+        Z += vn * (double)d->cmf[2 * nb + q];
+      }
+      X *= (double)s->delta; Y *= (double)s->delta; Z *= (double)s->delta;
+      r = (double)3.2404542f * X - (double)1.5371385f * Y - (double)0.4985314f * Z;
+      g = (double)-0.9692660f * X + (double)1.8760108f * Y + (double)0.0415560f * Z;
+      b = (double)0.0556434f * X - (double)0.2040259f * Y + (double)1.0572252f * Z;
+      part[(size_t)px * K + a] = (double)dloss[o] * r + (double)dloss[o + 1] * g + (double)dloss[o + 2] * b;
+    }
+  }
+  for (j = 0; j < K; j++) out[j] = 0;
+  for (px = 0; px < w * h; px++)
+    for (j = 0; j < K; j++) out[j] += part[(size_t)px * K + j];
+  free(part);
   return 0;
 }
 
