@@ -18,6 +18,12 @@ Also reported:
   cpu_baseline  the CPU oracle (C restatement of the reference render loop,
                 OpenMP over pixels) on a bounded row sample of the same
                 frame and settings, on this host's cores.
+  adjoint       the differentiation workload: box room with 16 Lambertian
+                albedos and 64 thin-film spheres at 512x512; parameters 64
+                thicknesses + 16 albedos x 95 grid coefficients; target at
+                parameters x (1 +- U[0,0.15]); L2 image loss; forward 64 spp
+                + path-replay adjoint 64 spp, 16 bounces, p=0.8.  grads/s =
+                complete gradients per second at that sample count.
 """
 import argparse
 import ctypes as C
@@ -159,6 +165,41 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=15.0):
                       f"p={p}, hero {hero}; {dt:.1f} s"}
 
 
+def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
+    d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c3_seed{seed}")
+    path = os.path.join(d, "box_room_films.nls")
+    if not os.path.exists(path):
+        scenes.box_room_films(d, seed=seed, width=512)
+    scene = la.load_nls(path)
+    dp = la.DiffParams(scene, d=[f"film{i}" for i in range(64)], albedo=list(range(16)))
+    th = dp.values(scene)
+    rng = np.random.default_rng(seed)
+    th_t = (th * (1 + rng.uniform(-0.15, 0.15, th.shape))).astype(np.float32)
+    ctx = la.Context(0)
+    pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=seed)
+    ctx.upload(dp.apply(scene, th_t))
+    target = ctx.render(pr)
+    ctx.upload(scene)
+    ctx.adjoint(la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=seed + 1),
+                dp, np.zeros_like(target))  # warm-up
+    t0 = time.perf_counter()
+    rgb = ctx.render(pr)
+    fwd = time.perf_counter() - t0
+    dloss = 2 * (rgb - target)
+    t0 = time.perf_counter()
+    grad = ctx.adjoint(pr, dp, dloss)
+    adj = time.perf_counter() - t0
+    adj_stream = ctx.timing().render_ms
+    ctx.close()
+    paths = scene.width * scene.height * spp
+    return {"workload": "box room, 16 Lambertian albedos + 64 thin-film spheres (11,564 tris), 512x512, "
+                        "L2 loss to a +-15% perturbed target, 16 bounces, p=0.8, hero 4",
+            "n_params": int(dp.n_params), "spp": spp, "forward_s": round(fwd, 4), "adjoint_s": round(adj, 4),
+            "adjoint_stream_ms": round(adj_stream, 2), "adjoint_paths_per_s": round(paths / adj, 1),
+            "grads_per_s": round(1.0 / adj, 3), "loss": float(((rgb - target) ** 2).sum()),
+            "grad_finite": bool(np.isfinite(grad).all())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +207,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -221,10 +263,13 @@ def main():
     if not args.no_roofline:
         out["roofline"] = extend_roofline(ctx, scene.desc)
         log(f"extend: {out['roofline']}")
+    ctx.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
+        log(f"adjoint: {out['adjoint']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, 1024, bounces, p, hero)
         log(f"cpu: {out['cpu_baseline']}")
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

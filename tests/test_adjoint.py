@@ -240,4 +240,4 @@ def test_autograd_binding(ctx, film_scene):
         g = theta.grad
         step = 1e-3 * torch.abs(theta) * torch.sign(g)
         l2 = ((lag.render(ctx, film_scene, dp, p, theta - step) - target) ** 2).sum()
-    assert float(l2) < float(loss)
+    assert float(l2) < loss.item()
