@@ -251,6 +251,48 @@ typedef struct {
 int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params *dp,
                 const float *dloss_drgb, float *grad);
 
+/* ---- learned path guiding -------------------------------------------- */
+/* No reference counterpart: a proposal distribution for the continuation
+ * direction of tracePath's BSDF sample (src/renderer.cpp:116-145), learned
+ * online.  A small MLP (6 inputs: position in the scene box, outgoing
+ * direction as two angles, hero wavelength, each in [0,1]; 12-frequency
+ * encoding; 4 hidden layers of 64; bf16 weights on MFMA with fp32
+ * accumulation) outputs 8 von Mises-Fisher lobes (40 raw values: per lobe
+ * an unnormalised mean direction, log κ and a weight logit).  At a vertex
+ * whose material's sampler it can stand in for (Lambertian, Blinn without
+ * transmission, thin film), the continuation is drawn from the mixture with
+ * probability alpha and from the reference's sampler otherwise, weighted by
+ * one-sample MIS over both, so the image converges to lam_render's. */
+typedef struct {
+  float alpha;             /* guide-branch probability, [0, 1); 0 = lam_render */
+  uint32_t train_passes;   /* the first passes train the network online */
+  uint32_t steps_per_pass; /* Adam steps after each training pass */
+  uint32_t batch;          /* training records per step, multiple of 256, <= 65536 */
+  float lr;                /* Adam step size */
+  uint64_t seed;           /* network initialisation (if none yet) and minibatches */
+} lam_guide_params;
+
+/* lam_render with the learned proposal.  The network persists in the
+ * context (initialised from g->seed on first use, or by lam_guide_reset /
+ * lam_guide_set_params) and keeps training across calls. */
+int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_params *g,
+                      float *linrgb, float *xyz);
+/* number of fp32 network parameters (W1[64][150] b1 W2[64][64] b2 W3 b3 W4
+ * b4 W5[40][64] b5; feature f < 144 is sin/cos(π 2^j x_d) with f = 24d+2j+c,
+ * 144+d is x_d) */
+size_t lam_guide_param_count(void);
+int lam_guide_reset(lam_ctx *ctx, uint64_t seed);
+int lam_guide_set_params(lam_ctx *ctx, const float *theta);
+int lam_guide_get_params(lam_ctx *ctx, float *theta);
+/* Network outputs (n x 40) for inputs (n x 6), through the MFMA kernel. */
+int lam_guide_infer(lam_ctx *ctx, const float *in, size_t n, float *raw);
+/* Training loss -Σ (w_i/Σw) log p(ω_i | in_i) and its gradient (param_count
+ * floats) for rows (n x 10: in[6], ω[3], w), 1 <= n <= 65536. */
+int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad);
+/* `steps` Adam steps (β1 0.9, β2 0.99) on one batch of rows; *loss = the
+ * last step's loss before its update. */
+int lam_guide_train(lam_ctx *ctx, const float *rows, size_t n, uint32_t steps, float lr, float *loss);
+
 /* Closest-hit query, intersect::scene (cg.hpp:1469-1489). */
 int lam_extend(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit *hits);
 /* Shadow query: occluded[i] = 1 iff a hit exists in [BIAS, tmax]

@@ -107,6 +107,19 @@ class DiffParamsC(C.Structure):
                 ("mat_ior", C.POINTER(C.c_int32)), ("mat_albedo", C.POINTER(C.c_int32))]
 
 
+class GuideParams(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("train_passes", C.c_uint32), ("steps_per_pass", C.c_uint32),
+                ("batch", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint64)]
+
+
+def guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1):
+    """lam_guide_params: guide-branch probability and online-training schedule."""
+    return GuideParams(float(alpha), int(train_passes), int(steps_per_pass), int(batch), float(lr), int(seed))
+
+
+GUIDE_IN, GUIDE_OUT, GUIDE_LOBES = 6, 40, 8
+
+
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_scatterings", C.c_uint32), ("sample_p", C.c_float),
                 ("hero", C.c_uint32), ("seed", C.c_uint64), ("spp_offset", C.c_uint32),
@@ -157,6 +170,14 @@ def lib():
         "lam_render": (C.c_int, [vp, C.POINTER(RenderParams), fp, fp]),
         "lam_trace_paths": (C.c_int, [vp, C.POINTER(RenderParams), u32p, u32p, sz, fp]),
         "lam_adjoint": (C.c_int, [vp, C.POINTER(RenderParams), C.POINTER(DiffParamsC), fp, fp]),
+        "lam_render_guided": (C.c_int, [vp, C.POINTER(RenderParams), C.POINTER(GuideParams), fp, fp]),
+        "lam_guide_param_count": (C.c_size_t, []),
+        "lam_guide_reset": (C.c_int, [vp, C.c_uint64]),
+        "lam_guide_set_params": (C.c_int, [vp, fp]),
+        "lam_guide_get_params": (C.c_int, [vp, fp]),
+        "lam_guide_infer": (C.c_int, [vp, fp, sz, fp]),
+        "lam_guide_loss_grad": (C.c_int, [vp, fp, sz, fp, fp]),
+        "lam_guide_train": (C.c_int, [vp, fp, sz, C.c_uint32, C.c_float, fp]),
         "lam_extend": (C.c_int, [vp, vp, sz, vp]),
         "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
         "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
@@ -547,6 +568,53 @@ class Context:
         cs = dp.c_struct()
         self._check(self._lib.lam_adjoint(self._h, C.byref(p), C.byref(cs), _fptr(g), _fptr(grad)))
         return grad
+
+    # ---- learned path guiding ----
+    def render_guided(self, p, g, want_xyz=False):
+        """lam_render_guided: render with the learned proposal (g: GuideParams)."""
+        d = self.scene.desc
+        w, h = d.camera.width, d.camera.height
+        rgb = np.zeros((h, w, 3), dtype=np.float32)
+        xyz = np.zeros((h, w, 3), dtype=np.float32) if want_xyz else None
+        self._check(self._lib.lam_render_guided(self._h, C.byref(p), C.byref(g), _fptr(rgb),
+                                                _fptr(xyz) if want_xyz else None))
+        return (rgb, xyz) if want_xyz else rgb
+
+    def guide_reset(self, seed=1):
+        self._check(self._lib.lam_guide_reset(self._h, int(seed)))
+
+    def guide_params_count(self):
+        return int(self._lib.lam_guide_param_count())
+
+    def guide_get_params(self):
+        th = np.zeros(self.guide_params_count(), dtype=np.float32)
+        self._check(self._lib.lam_guide_get_params(self._h, _fptr(th)))
+        return th
+
+    def guide_set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        if th.shape != (self.guide_params_count(),):
+            raise ValueError(f"theta has shape {th.shape}")
+        self._check(self._lib.lam_guide_set_params(self._h, _fptr(th)))
+
+    def guide_infer(self, inputs):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, GUIDE_IN)
+        raw = np.zeros((len(x), GUIDE_OUT), dtype=np.float32)
+        self._check(self._lib.lam_guide_infer(self._h, _fptr(x), len(x), _fptr(raw)))
+        return raw
+
+    def guide_loss_grad(self, rows):
+        r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 10)
+        loss = C.c_float()
+        grad = np.zeros(self.guide_params_count(), dtype=np.float32)
+        self._check(self._lib.lam_guide_loss_grad(self._h, _fptr(r), len(r), C.byref(loss), _fptr(grad)))
+        return loss.value, grad
+
+    def guide_train(self, rows, steps, lr):
+        r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 10)
+        loss = C.c_float()
+        self._check(self._lib.lam_guide_train(self._h, _fptr(r), len(r), int(steps), float(lr), C.byref(loss)))
+        return loss.value
 
     def timing(self):
         t = Timing()

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <chrono>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "host/bvh8.hpp"
+#include "lam_guide.h"
 #include "lam_launch.h"
 #include "lambda_amd.h"
 
@@ -33,8 +35,23 @@ struct lam_ctx {
   int ws_hero = 0;
   std::vector<void *> ws_bufs;
   uint32_t *ctrl = nullptr;
-  lam::WaveLaunch wl{0, 0};
+  lam::WaveLaunch wl{0, 0, 0, 0, 0};
   std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
+  float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // scene box (guiding inputs)
+  // guiding network: fp32 master parameters, Adam moments, gradient,
+  // bf16-rounded copy (one allocation), packed fragments, scalars
+  bool guide_ready = false;
+  float *g_th = nullptr, *g_m = nullptr, *g_v = nullptr, *g_grad = nullptr, *g_thq = nullptr, *g_scal = nullptr;
+  void *g_pack = nullptr;
+  uint32_t g_step = 0;
+  float *g_ws = nullptr;  // training batch workspace
+  size_t g_ws_bytes = 0;
+  // guided-render buffers: network outputs per queue entry, records
+  void *g_mix = nullptr;
+  size_t g_mix_cap = 0;
+  float4 *g_rec = nullptr;
+  uint32_t g_rec_cap = 0;
+  uint32_t *g_rec_count = nullptr, *g_rec_of_slot = nullptr;
 };
 
 // Path slots per wavefront pass.  The frame is processed in chunks of this
@@ -140,6 +157,8 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
     // ray; the finish kernel then traces each remaining path to its end
     ctx->wl.tail_threshold = 131072u;
     ctx->wl.max_iters = 12u;
+    // guiding network: 51 KB of LDS per 256-lane block, 3 blocks per CU
+    ctx->wl.guide_blocks = uint32_t(cus > 0 ? cus : 256) * 3u;
   }
   if (ctx->ws.cap >= cap && ctx->ws_hero >= hero) return LAM_OK;
   free_wave(ctx);
@@ -187,12 +206,117 @@ int check_ctrl_overflow(lam_ctx *ctx) {
   return LAM_OK;
 }
 
+void free_guide(lam_ctx *ctx) {
+  void *bufs[] = {ctx->g_th, ctx->g_pack, ctx->g_ws, ctx->g_mix, ctx->g_rec, ctx->g_rec_count, ctx->g_rec_of_slot};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  ctx->g_th = ctx->g_m = ctx->g_v = ctx->g_grad = ctx->g_thq = ctx->g_scal = nullptr;
+  ctx->g_pack = nullptr;
+  ctx->g_ws = nullptr;
+  ctx->g_ws_bytes = 0;
+  ctx->g_mix = nullptr;
+  ctx->g_mix_cap = 0;
+  ctx->g_rec = nullptr;
+  ctx->g_rec_cap = 0;
+  ctx->g_rec_count = ctx->g_rec_of_slot = nullptr;
+  ctx->guide_ready = false;
+}
+
+int ensure_guide(lam_ctx *ctx) {
+  if (ctx->g_th) return LAM_OK;
+  const size_t np = (size_t(lam::kGParams) + 63) / 64 * 64;
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, np * 4 * 5 + 64));
+  float *f = static_cast<float *>(p);
+  ctx->g_th = f;
+  ctx->g_m = f + np;
+  ctx->g_v = f + 2 * np;
+  ctx->g_grad = f + 3 * np;
+  ctx->g_thq = f + 4 * np;
+  ctx->g_scal = f + 5 * np;
+  HIPC(hipMalloc(&ctx->g_pack, lam::kGPackBytes));
+  return LAM_OK;
+}
+
+int ensure_guide_ws(lam_ctx *ctx, uint32_t batch) {
+  const size_t need = lam::guide_train_bytes(batch);
+  if (ctx->g_ws_bytes >= need) return LAM_OK;
+  if (ctx->g_ws) HIPC(hipFree(ctx->g_ws));
+  ctx->g_ws = nullptr;
+  ctx->g_ws_bytes = 0;
+  HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_ws), need));
+  ctx->g_ws_bytes = need;
+  return LAM_OK;
+}
+
+// He-uniform hidden layers, a 10x smaller output layer, zero biases (seeded)
+void guide_init_params(uint64_t seed, std::vector<float> &th) {
+  th.assign(lam::kGParams, 0.f);
+  uint64_t st = seed ^ 0x6a09e667f3bcc909ull;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * 0x1.0p-53;
+  };
+  const int W[5] = {lam::kGW1, lam::kGW2, lam::kGW3, lam::kGW4, lam::kGW5};
+  const int fan_in[5] = {lam::kGFeat, lam::kGHid, lam::kGHid, lam::kGHid, lam::kGHid};
+  const int count[5] = {lam::kGHid * lam::kGFeat, lam::kGHid * lam::kGHid, lam::kGHid * lam::kGHid,
+                        lam::kGHid * lam::kGHid, lam::kGOut * lam::kGHid};
+  for (int l = 0; l < 5; l++) {
+    const double a = std::sqrt(6.0 / fan_in[l]) * (l == 4 ? 0.1 : 1.0);
+    for (int i = 0; i < count[l]; i++) th[size_t(W[l]) + i] = float((2.0 * next() - 1.0) * a);
+  }
+}
+
+int guide_upload_params(lam_ctx *ctx, const float *th) {
+  int rc;
+  if ((rc = ensure_guide(ctx))) return rc;
+  const size_t np = (size_t(lam::kGParams) + 63) / 64 * 64;
+  HIPC(hipMemcpyAsync(ctx->g_th, th, size_t(lam::kGParams) * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(lam::launch_zero(ctx->g_m, np * 8, ctx->stream));  // m and v
+  HIPC(lam::guide_pack(ctx->g_th, ctx->g_pack, ctx->g_thq, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->g_step = 0;
+  ctx->guide_ready = true;
+  return LAM_OK;
+}
+
+lam::GuideTrain guide_train_state(lam_ctx *ctx, uint32_t batch, float lr, uint64_t seed) {
+  lam::GuideTrain gt{};
+  gt.th = ctx->g_th;
+  gt.m = ctx->g_m;
+  gt.v = ctx->g_v;
+  gt.grad = ctx->g_grad;
+  gt.thq = ctx->g_thq;
+  gt.scal = ctx->g_scal;
+  gt.ws = ctx->g_ws;
+  gt.pack = ctx->g_pack;
+  gt.batch = batch;
+  gt.step = ctx->g_step;
+  gt.lr = lr;
+  gt.b1 = 0.9f;
+  gt.b2 = 0.99f;
+  gt.eps = 1e-8f;
+  gt.seed = seed;
+  return gt;
+}
+
+// Guided-render settings for render_wavefront
+struct GuideRun {
+  const lam_guide_params *gp;
+  lam::GuideDev gd_render, gd_train;
+};
+
 // Wavefront render of every pixel for passes spp_offset .. spp_offset+spp-1.
 // Each chunk's pass is captured once as a hipGraph (its first node advances
 // the pass index held in the control block) and replayed spp times.
 // LAM_FLAG_PROFILE launches without a graph and times every extend and
 // shadow launch with events; LAM_FLAG_NO_GRAPH launches without a graph.
-int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix) {
+int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix,
+                     const GuideRun *gr = nullptr) {
   const size_t cap = std::min(npix, kMaxWaveCap);
   int rc;
   if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
@@ -219,31 +343,51 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   }
   for (size_t off = 0; off < npix; off += cap) {
     const uint32_t n = uint32_t(std::min(cap, npix - off));
+    uint32_t first = 0;  // passes of this chunk already traced
+    if (gr && off == 0) {
+      // training passes: trace with records, weight them, then Adam steps
+      const lam_guide_params *gp = gr->gp;
+      const uint32_t tp = std::min(gp->train_passes, p->spp);
+      lam::GuideTrain gt = guide_train_state(ctx, gp->batch, gp->lr, gp->seed);
+      for (; first < tp; first++) {
+        HIPC(lam::launch_zero(gr->gd_train.rec_count, 4, ctx->stream));
+        HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + first, ctx->stream));
+        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
+                               film, nullptr, wl, ctx->stream, nullptr, &gr->gd_train, true));
+        HIPC(lam::guide_weights(int(p->hero), ctx->ws, gr->gd_train, ctx->stream));
+        HIPC(lam::guide_train_records(gt, gr->gd_train.rec, gr->gd_train.rec_count, gr->gd_train.rec_cap,
+                                      gp->steps_per_pass, ctx->stream));
+      }
+      ctx->g_step = gt.step;
+      HIPC(hipStreamSynchronize(ctx->stream));
+    }
+    const lam::GuideDev *gd = gr ? &gr->gd_render : nullptr;
+    if (first >= p->spp) continue;
     if (graph) {
       hipGraph_t g = nullptr;
       hipGraphExec_t ge = nullptr;
       HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e1 = lam::enqueue_pass_next(ctx->ctrl, ctx->stream);
       hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n,
-                                        ctx->ctrl, film, nullptr, wl, ctx->stream, nullptr);
+                                        ctx->ctrl, film, nullptr, wl, ctx->stream, nullptr, gd, false);
       HIPC(hipStreamEndCapture(ctx->stream, &g));
       HIPC(e1);
       HIPC(e2);
       hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPC(e3);
-      HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset - 1u, ctx->stream));
+      HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + first - 1u, ctx->stream));
       hipError_t e4 = hipSuccess;
-      for (uint32_t s = 0; s < p->spp && e4 == hipSuccess; s++) e4 = hipGraphLaunch(ge, ctx->stream);
+      for (uint32_t s = first; s < p->spp && e4 == hipSuccess; s++) e4 = hipGraphLaunch(ge, ctx->stream);
       hipError_t e5 = hipStreamSynchronize(ctx->stream);
       (void)hipGraphExecDestroy(ge);
       HIPC(e4);
       HIPC(e5);
     } else {
-      for (uint32_t s = 0; s < p->spp; s++) {
+      for (uint32_t s = first; s < p->spp; s++) {
         HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + s, ctx->stream));
         HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
-                               film, nullptr, wl, ctx->stream, profile ? &prof : nullptr));
+                               film, nullptr, wl, ctx->stream, profile ? &prof : nullptr, gd, false));
         if (profile) {
           HIPC(hipStreamSynchronize(ctx->stream));
           float ms = 0.f;
@@ -307,6 +451,7 @@ void lam_destroy(lam_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_scene(ctx);
   free_wave(ctx);
+  free_guide(ctx);
   if (ctx->ctrl) (void)hipFree(ctx->ctrl);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 6; i++)
@@ -388,6 +533,33 @@ int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
   std::vector<uint32_t> analytic;
   for (uint32_t i = 0; i < d->n_objects; i++)
     if (d->objects[i].type != LAM_OBJ_TRIMESH) analytic.push_back(i);
+  // scene box: triangle vertices and the transformed corners of the unit
+  // sphere box / plane quad (the guiding network's position inputs)
+  {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    auto grow = [&](const float *v) {
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], v[k]);
+        hi[k] = std::max(hi[k], v[k]);
+      }
+    };
+    for (size_t i = 0; i < size_t(nt) * 3; i++) grow(d->tri_v + 3 * i);
+    for (uint32_t i : analytic) {
+      const float *M = d->objects[i].T.M;
+      const bool sphere = d->objects[i].type == LAM_OBJ_SPHERE;
+      for (int c = 0; c < (sphere ? 8 : 4); c++) {
+        const float x = (c & 1) ? 1.f : -1.f, y = (c & 2) ? 1.f : -1.f, z = sphere ? ((c & 4) ? 1.f : -1.f) : 0.f;
+        float v[3];
+        for (int r = 0; r < 3; r++) v[r] = M[4 * r] * x + M[4 * r + 1] * y + M[4 * r + 2] * z + M[4 * r + 3];
+        grow(v);
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      if (!(lo[k] <= hi[k])) lo[k] = hi[k] = 0.f;
+      ctx->box_lo[k] = lo[k];
+      ctx->box_hi[k] = hi[k] > lo[k] ? hi[k] : lo[k] + 1.f;
+    }
+  }
 
   lam::DevScene &sc = ctx->sc;
   int rc;
@@ -642,6 +814,173 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   HIPC(hipMemcpyAsync(grad, dgrad, size_t(dp->n_params) * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  return check_err_flag(ctx);
+}
+
+// ---- guiding --------------------------------------------------------------------------
+size_t lam_guide_param_count(void) { return size_t(lam::kGParams); }
+
+int lam_guide_reset(lam_ctx *ctx, uint64_t seed) {
+  if (!ctx) return LAM_E_INVALID;
+  HIPC(hipSetDevice(ctx->device));
+  std::vector<float> th;
+  guide_init_params(seed, th);
+  return guide_upload_params(ctx, th.data());
+}
+
+int lam_guide_set_params(lam_ctx *ctx, const float *theta) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!theta) return fail(ctx, LAM_E_INVALID, "NULL parameters");
+  for (int i = 0; i < lam::kGParams; i++)
+    if (!std::isfinite(theta[i])) return fail(ctx, LAM_E_INVALID, "parameters must be finite");
+  HIPC(hipSetDevice(ctx->device));
+  return guide_upload_params(ctx, theta);
+}
+
+int lam_guide_get_params(lam_ctx *ctx, float *theta) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!theta) return fail(ctx, LAM_E_INVALID, "NULL output");
+  if (!ctx->guide_ready) return fail(ctx, LAM_E_INVALID, "guiding network not initialised");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(theta, ctx->g_th, size_t(lam::kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return LAM_OK;
+}
+
+int lam_guide_infer(lam_ctx *ctx, const float *in, size_t n, float *raw) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n == 0) return LAM_OK;
+  if (!in || !raw) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (!ctx->guide_ready) return fail(ctx, LAM_E_INVALID, "guiding network not initialised");
+  if (n > (size_t(1) << 24)) return fail(ctx, LAM_E_LIMIT, "batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_scratch(ctx, 0, n * lam::kGIn * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 1, n * lam::kGOut * 4))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[0], in, n * lam::kGIn * 4, hipMemcpyHostToDevice, ctx->stream));
+  int cus = 0;
+  HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const uint32_t blocks = uint32_t(std::min<size_t>((n + 255) / 256, size_t(cus > 0 ? cus : 256) * 3));
+  HIPC(lam::guide_infer(ctx->g_pack, static_cast<const float *>(ctx->scratch[0]), uint32_t(n),
+                        static_cast<float *>(ctx->scratch[1]), blocks, ctx->stream));
+  HIPC(hipMemcpyAsync(raw, ctx->scratch[1], n * lam::kGOut * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return LAM_OK;
+}
+
+static int guide_rows_in(lam_ctx *ctx, const float *rows, size_t n) {
+  if (!rows) return fail(ctx, LAM_E_INVALID, "NULL rows");
+  if (n == 0 || n > 65536) return fail(ctx, LAM_E_LIMIT, "batch must hold 1 to 65536 rows");
+  if (!ctx->guide_ready) return fail(ctx, LAM_E_INVALID, "guiding network not initialised");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_guide_ws(ctx, uint32_t(n)))) return rc;
+  if ((rc = ensure_scratch(ctx, 0, n * 40))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[0], rows, n * 40, hipMemcpyHostToDevice, ctx->stream));
+  return LAM_OK;
+}
+
+int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad) {
+  if (!ctx) return LAM_E_INVALID;
+  int rc;
+  if ((rc = guide_rows_in(ctx, rows, n))) return rc;
+  HIPC(lam::guide_grad_rows(ctx->g_thq, static_cast<const float *>(ctx->scratch[0]), uint32_t(n), ctx->g_ws,
+                            ctx->g_grad, ctx->g_scal, ctx->stream));
+  float sc[2] = {0.f, 0.f};
+  HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (grad) HIPC(hipMemcpyAsync(grad, ctx->g_grad, size_t(lam::kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (loss) *loss = sc[1];
+  return LAM_OK;
+}
+
+int lam_guide_train(lam_ctx *ctx, const float *rows, size_t n, uint32_t steps, float lr, float *loss) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!(lr > 0.f)) return fail(ctx, LAM_E_INVALID, "lr must be positive");
+  int rc;
+  if ((rc = guide_rows_in(ctx, rows, n))) return rc;
+  lam::GuideTrain gt = guide_train_state(ctx, uint32_t(n), lr, 0);
+  HIPC(lam::guide_train_rows(gt, static_cast<const float *>(ctx->scratch[0]), uint32_t(n), steps, ctx->stream));
+  ctx->g_step = gt.step;
+  float sc[2] = {0.f, 0.f};
+  HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (loss) *loss = sc[1];
+  return LAM_OK;
+}
+
+int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_params *g, float *linrgb,
+                      float *xyz) {
+  if (!ctx) return LAM_E_INVALID;
+  int rc;
+  if ((rc = check_params(ctx, p))) return rc;
+  if (!g) return fail(ctx, LAM_E_INVALID, "guide params are NULL");
+  if (!linrgb) return fail(ctx, LAM_E_INVALID, "NULL output buffer");
+  if (!(g->alpha >= 0.f && g->alpha < 1.f)) return fail(ctx, LAM_E_INVALID, "alpha must be in [0, 1)");
+  if (g->train_passes && (g->batch < 256 || g->batch > 65536 || g->batch % 256))
+    return fail(ctx, LAM_E_INVALID, "batch must be a multiple of 256 in [256, 65536]");
+  if (g->train_passes && !(g->lr > 0.f)) return fail(ctx, LAM_E_INVALID, "lr must be positive");
+  if (p->flags & LAM_FLAG_MEGAKERNEL) return fail(ctx, LAM_E_INVALID, "guiding needs the wavefront path");
+  HIPC(hipSetDevice(ctx->device));
+  if (!ctx->guide_ready && (rc = lam_guide_reset(ctx, g->seed))) return rc;
+  const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
+  const size_t npix = size_t(w) * h;
+  if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
+  const size_t cap = std::min(npix, kMaxWaveCap);
+  if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
+  if (ctx->g_mix_cap < ctx->ws.cap) {
+    if (ctx->g_mix) HIPC(hipFree(ctx->g_mix));
+    if (ctx->g_rec_of_slot) HIPC(hipFree(ctx->g_rec_of_slot));
+    ctx->g_mix = nullptr;
+    ctx->g_rec_of_slot = nullptr;
+    ctx->g_mix_cap = 0;
+    HIPC(hipMalloc(&ctx->g_mix, ctx->ws.cap * 80));
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_rec_of_slot), ctx->ws.cap * 4));
+    ctx->g_mix_cap = ctx->ws.cap;
+  }
+  if (g->train_passes) {
+    const uint32_t rcap = uint32_t(std::min<size_t>(ctx->ws.cap * 8, size_t(1) << 25));
+    if (ctx->g_rec_cap < rcap) {
+      if (ctx->g_rec) HIPC(hipFree(ctx->g_rec));
+      ctx->g_rec = nullptr;
+      ctx->g_rec_cap = 0;
+      HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_rec), size_t(rcap) * 64));
+      ctx->g_rec_cap = rcap;
+    }
+    if (!ctx->g_rec_count) HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_rec_count), 64));
+    if ((rc = ensure_guide_ws(ctx, g->batch))) return rc;
+  }
+  GuideRun gr{};
+  gr.gp = g;
+  lam::GuideDev &gd = gr.gd_render;
+  gd.frag = static_cast<const uint4 *>(ctx->g_pack);
+  gd.mix = static_cast<uint4 *>(ctx->g_mix);
+  gd.alpha = g->alpha;
+  gd.lo = lam::f3{ctx->box_lo[0], ctx->box_lo[1], ctx->box_lo[2]};
+  gd.inv_ext = lam::f3{1.f / (ctx->box_hi[0] - ctx->box_lo[0]), 1.f / (ctx->box_hi[1] - ctx->box_lo[1]),
+                       1.f / (ctx->box_hi[2] - ctx->box_lo[2])};
+  gr.gd_train = gd;
+  gr.gd_train.rec = ctx->g_rec;
+  gr.gd_train.rec_count = ctx->g_rec_count;
+  gr.gd_train.rec_cap = ctx->g_rec_cap;
+  gr.gd_train.rec_of_slot = ctx->g_rec_of_slot;
+  if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;
+  if ((rc = ensure_scratch(ctx, 4, npix * 12))) return rc;
+  if ((rc = ensure_scratch(ctx, 5, npix * 12))) return rc;
+  float *film = static_cast<float *>(ctx->scratch[3]);
+  HIPC(lam::launch_zero(film, npix * 32, ctx->stream));
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  if ((rc = render_wavefront(ctx, p, film, npix, &gr))) return rc;
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
+  HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
+                           xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
+  HIPC(hipMemcpyAsync(linrgb, ctx->scratch[4], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
+  if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->timing.render_ms = event_ms(ctx);
+  if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
 

@@ -1,0 +1,404 @@
+// Learned path guiding: a small MLP maps a path vertex (position, outgoing
+// direction, hero wavelength) to a mixture of 8 von Mises-Fisher lobes over
+// the continuation direction.  It is evaluated with bf16 MFMA
+// (v_mfma_f32_16x16x32_bf16, fp32 accumulate) one wave at a time, and mixed
+// with the reference's BSDF sampler under one-sample MIS so the estimator
+// keeps the expectation of the plain render loop.
+//
+// Network: 6 inputs in [0,1] -> frequency encoding (sin/cos(π 2^j x) for
+// j = 0..11 and the raw inputs: 150 features, padded to 160) -> 64 -> 64 ->
+// 64 -> 64 (ReLU) -> 40 = 8 lobes x (3-component mean direction, log κ,
+// weight logit).  The mean direction is a 3-vector normalised after the
+// network (no angle wrap).
+//
+// MFMA operand layouts (16x16x32 bf16; lane l, n = l % 16, g = l / 16):
+//   A[m][k]: lane holds A[n][8g + j], j = 0..7
+//   B[k][n]: lane holds B[8g + j][n]
+//   D[m][n]: lane holds D[4g + i][n], i = 0..3
+// Layers are computed transposed, D = W · Xᵀ (rows = neurons, columns = 16
+// queries), so a layer's output registers already hold, per lane, the 16
+// neurons {16t + 4g + i} of one query.  The next layer takes them as its B
+// operand unchanged by storing its weight columns in the matching order:
+// K index 32c + 8g + j  <->  neuron 32c + 16(j / 4) + 4g + (j % 4).
+#pragma once
+#include "lam_launch.h"
+#include "lam_shade.h"
+
+namespace lam {
+
+constexpr int kGIn = 6, kGFreq = 12, kGFeat = 150, kGFeatPad = 160, kGHid = 64, kGLobes = 8, kGOut = 40,
+              kGOutPad = 48;
+// fp32 parameter vector: W1[64][150] b1 W2[64][64] b2 W3 b3 W4 b4 W5[40][64] b5
+constexpr int kGW1 = 0, kGB1 = kGW1 + kGHid * kGFeat, kGW2 = kGB1 + kGHid, kGB2 = kGW2 + kGHid * kGHid,
+              kGW3 = kGB2 + kGHid, kGB3 = kGW3 + kGHid * kGHid, kGW4 = kGB3 + kGHid, kGB4 = kGW4 + kGHid * kGHid,
+              kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGOut * kGHid, kGParams = kGB5 + kGOut;
+// packed weight fragments: layer 1 (4 row tiles x 5 K chunks), layers 2-4
+// (4 x 2 each), layer 5 (3 x 2); 64 lanes x 16 bytes each; then the biases
+// as fp32 (64 x 4 + 48)
+constexpr int kGFragL1 = 0, kGFragL2 = 20, kGFragL5 = 44, kGFrags = 50;
+constexpr int kGBiasWords = 4 * kGHid + kGOutPad;
+constexpr size_t kGPackBytes = size_t(kGFrags) * 64 * 16 + size_t(kGBiasWords) * 4;
+constexpr float kGLogKMin = -4.f, kGLogKMax = 9.f;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 -> bf16, round to nearest even (finite inputs)
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+  const uint32_t u = uint32_t(b) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+__host__ __device__ inline float bfround(float f) { return bf2f(f2bf(f)); }
+
+// K position (within a 64-wide hidden layer) of neuron k's column
+__host__ __device__ inline int g_perm(int k) {
+  const int c = k >> 5, g = (k >> 3) & 3, j = k & 7;
+  return 32 * c + 16 * (j >> 2) + 4 * g + (j & 3);
+}
+
+// Feature f of the encoding: f = 24 d + 2 j + c (f < 144) is sin (c = 0) or
+// cos (c = 1) of π 2^j x_d; f = 144 + d is x_d itself; 150..159 are zero.
+// Any 8-aligned block of features holds one input and four frequencies.
+__device__ inline float g_feature(int f, const float *in) {
+  if (f >= kGFeat) return 0.f;
+  if (f >= 144) return in[f - 144];
+  const int d = f / 24, j = (f % 24) >> 1;
+  // π 2^j x = 2π (2^(j-1) x): v_sin/v_cos take revolutions
+  const float x = in[d] * float(1 << j) * .5f;
+  const float fr = x - floorf(x);
+  return (f & 1) ? __builtin_amdgcn_cosf(fr) : __builtin_amdgcn_sinf(fr);
+}
+
+__device__ inline uint32_t pack_bf2(float a, float b) { return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16); }
+__device__ inline bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ inline f32x4 mfma16(uint4 a, uint4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
+}
+// ReLU of a layer output (4 row tiles) as the next layer's two bf16 B chunks
+__device__ inline void relu_pack(const f32x4 *acc, uint4 *xb) {
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const f32x4 a = acc[2 * c], b = acc[2 * c + 1];
+    xb[c].x = pack_bf2(fmaxf(a[0], 0.f), fmaxf(a[1], 0.f));
+    xb[c].y = pack_bf2(fmaxf(a[2], 0.f), fmaxf(a[3], 0.f));
+    xb[c].z = pack_bf2(fmaxf(b[0], 0.f), fmaxf(b[1], 0.f));
+    xb[c].w = pack_bf2(fmaxf(b[2], 0.f), fmaxf(b[3], 0.f));
+  }
+}
+
+// the six inputs of one query as named scalars (an array indexed by a
+// lane-dependent input number would be placed in scratch memory)
+struct GIn {
+  float x0, x1, x2, x3, x4, x5;
+};
+__device__ inline float gin_at(const GIn &q, int d) {
+  return d == 0 ? q.x0 : d == 1 ? q.x1 : d == 2 ? q.x2 : d == 3 ? q.x3 : d == 4 ? q.x4 : q.x5;
+}
+
+// the 8 bf16 features 32c + 8g .. +7 of one query (one input, four frequencies)
+__device__ inline uint4 feature_chunk(int c, uint32_t g, const GIn &q) {
+  const int f0 = 32 * c + 8 * int(g);
+  float v[8];
+  if (f0 < 144) {
+    const int d = f0 / 24, j0 = (f0 % 24) >> 1;
+    float x = gin_at(q, d) * float(1 << j0) * .5f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float fr = x - floorf(x);
+      v[2 * k] = __builtin_amdgcn_sinf(fr);
+      v[2 * k + 1] = __builtin_amdgcn_cosf(fr);
+      x *= 2.f;
+    }
+  } else if (f0 == 144) {
+    v[0] = q.x0; v[1] = q.x1; v[2] = q.x2; v[3] = q.x3; v[4] = q.x4; v[5] = q.x5;
+    v[6] = v[7] = 0.f;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = 0.f;
+  }
+  return make_uint4(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7]));
+}
+
+// The network over the 64 queries of a wave (one per lane) as 4 groups of
+// 16 MFMA columns; each weight fragment is read once and applied to all four
+// groups.  Every lane of the wave must call it (lanes without a query pass
+// any input).  frag/bias: packed parameters (LDS or global).  The outputs
+// stay in the MFMA result layout: o[G][t][i] of lane (n, g) is output row
+// 16t + 4g + i of the query in lane 16G + n (rows >= 40 are padding).
+__device__ inline void mlp_wave(const uint4 *__restrict__ frag, const float *__restrict__ bias, const float *in,
+                                f32x4 (&o)[4][3]) {
+  const uint32_t lane = threadIdx.x & 63u, n = lane & 15u, g = lane >> 4;
+  GIn q[4];
+#pragma unroll
+  for (int G = 0; G < 4; G++) {
+    const int src = int(16 * G + n);
+    q[G] = GIn{__shfl(in[0], src, 64), __shfl(in[1], src, 64), __shfl(in[2], src, 64),
+               __shfl(in[3], src, 64), __shfl(in[4], src, 64), __shfl(in[5], src, 64)};
+  }
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    f32x4 b;
+#pragma unroll
+    for (int i = 0; i < 4; i++) b[i] = bias[16 * t + 4 * g + i];
+#pragma unroll
+    for (int G = 0; G < 4; G++) acc[G][t] = b;
+  }
+#pragma unroll 1
+  for (int c = 0; c < 5; c++) {
+    uint4 xb[4];
+#pragma unroll
+    for (int G = 0; G < 4; G++) xb[G] = feature_chunk(c, g, q[G]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const uint4 a = frag[(kGFragL1 + t * 5 + c) * 64 + lane];
+#pragma unroll
+      for (int G = 0; G < 4; G++) acc[G][t] = mfma16(a, xb[G], acc[G][t]);
+    }
+  }
+#pragma unroll 1
+  for (int l = 0; l < 3; l++) {
+    uint4 xb[4][2];
+#pragma unroll
+    for (int G = 0; G < 4; G++) relu_pack(acc[G], xb[G]);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      f32x4 b;
+#pragma unroll
+      for (int i = 0; i < 4; i++) b[i] = bias[kGHid * (l + 1) + 16 * t + 4 * g + i];
+#pragma unroll
+      for (int G = 0; G < 4; G++) acc[G][t] = b;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const uint4 a = frag[(kGFragL2 + 8 * l + t * 2 + c) * 64 + lane];
+#pragma unroll
+        for (int G = 0; G < 4; G++) acc[G][t] = mfma16(a, xb[G][c], acc[G][t]);
+      }
+    }
+  }
+  uint4 xb[4][2];
+#pragma unroll
+  for (int G = 0; G < 4; G++) relu_pack(acc[G], xb[G]);
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    f32x4 b;
+#pragma unroll
+    for (int i = 0; i < 4; i++) b[i] = bias[4 * kGHid + 16 * t + 4 * g + i];
+#pragma unroll
+    for (int G = 0; G < 4; G++) o[G][t] = b;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const uint4 a = frag[(kGFragL5 + t * 2 + c) * 64 + lane];
+#pragma unroll
+      for (int G = 0; G < 4; G++) o[G][t] = mfma16(a, xb[G][c], o[G][t]);
+    }
+  }
+}
+
+// ---- the mixture ----------------------------------------------------------------------
+struct GuideMix {
+  f3 mu[kGLobes];
+  float kappa[kGLobes], w[kGLobes];
+};
+
+__device__ inline void mix_from_raw(const float *raw, GuideMix &m) {
+  float mx = -3.4e38f;
+#pragma unroll
+  for (int L = 0; L < kGLobes; L++) mx = fmaxf(mx, raw[5 * L + 4]);
+  float s = 0.f;
+#pragma unroll
+  for (int L = 0; L < kGLobes; L++) {
+    const f3 v = mk(raw[5 * L], raw[5 * L + 1], raw[5 * L + 2]);
+    const float l2 = len2(v);
+    m.mu[L] = l2 > 1e-24f ? mul(v, 1.f / sqrtf(l2)) : mk(0.f, 0.f, 1.f);
+    m.kappa[L] = expf(fminf(fmaxf(raw[5 * L + 3], kGLogKMin), kGLogKMax));
+    m.w[L] = expf(raw[5 * L + 4] - mx);
+    s += m.w[L];
+  }
+  const float inv = 1.f / s;
+#pragma unroll
+  for (int L = 0; L < kGLobes; L++) m.w[L] *= inv;
+}
+
+// vMF density κ / (2π (1 - e^{-2κ})) · e^{κ(μ·ω - 1)}
+__device__ inline float vmf_pdf(f3 mu, float k, f3 w) {
+  return k / (2.f * kPi * -expm1f(-2.f * k)) * expf(k * (dot(mu, w) - 1.f));
+}
+__device__ inline float mix_pdf(const GuideMix &m, f3 w) {
+  float p = 0.f;
+#pragma unroll
+  for (int L = 0; L < kGLobes; L++) p += m.w[L] * vmf_pdf(m.mu[L], m.kappa[L], w);
+  return p;
+}
+// orthonormal basis around a unit vector (Duff et al. 2017, branchless)
+__device__ inline void basis_of(f3 n, f3 &t, f3 &b) {
+  const float sg = copysignf(1.f, n.z);
+  const float a = -1.f / (sg + n.z);
+  const float c = n.x * n.y * a;
+  t = mk(1.f + sg * n.x * n.x * a, sg * c, -sg * n.x);
+  b = mk(c, sg + n.y * n.y * a, -n.y);
+}
+// vMF sampling, w = 1 + log(ξ + (1-ξ) e^{-2κ}) / κ (Jakob 2012)
+__device__ inline f3 vmf_sample(f3 mu, float k, float u1, float u2) {
+  const float w = fminf(fmaxf(1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k, -1.f), 1.f);
+  const float r = sqrtf(fmaxf(0.f, 1.f - w * w));
+  const float ph = 2.f * kPi * u2;
+  f3 t, b;
+  basis_of(mu, t, b);
+  return normalized(add(add(mul(mu, w), mul(t, r * cosf(ph))), mul(b, r * sinf(ph))));
+}
+__device__ inline f3 mix_sample(const GuideMix &m, float ul, float u1, float u2) {
+  // lobe L with cumulative weight above ul (a select chain: the arrays stay
+  // in registers)
+  f3 mu = m.mu[kGLobes - 1];
+  float k = m.kappa[kGLobes - 1], c = 0.f;
+  bool found = false;
+#pragma unroll
+  for (int L = 0; L < kGLobes - 1; L++) {
+    c += m.w[L];
+    const bool take = !found && ul < c;
+    mu = take ? m.mu[L] : mu;
+    k = take ? m.kappa[L] : k;
+    found = found || take;
+  }
+  return vmf_sample(mu, k, u1, u2);
+}
+
+// ---- the plain continuation as a density over directions ---------------------------------
+// Where guiding applies: the reference's Lambertian sampler, the Blinn
+// sampler when it never transmits (w_t = 0, so fresnelSplit gives p_r = 1)
+// and the thin-film sampler where it continues at all (front side, o·n >= 0).
+__device__ inline bool guide_supported(const lam_material &m, const Surf &s, f3 o) {
+  if (m.type == LAM_MAT_LAMBERTIAN) return true;
+  if (m.type == LAM_MAT_BLINN) return m.w_t == 0.f;
+  if (m.type == LAM_MAT_THINFILM) return s.front && !(dot(o, s.F.z) < 0.f);
+  return false;
+}
+
+// q(ω): density with which sample_material proposes ω (roulette included).
+// G(ω) = Σ_lobes mult_l(ω) w_l(ω) q_l(ω) / prob_l(ω): the plain continuation
+// (mult_l / prob_l · w_l for lobe l's sample) has expectation ∫ G(ω) L_i(ω) dω,
+// where prob_l is the pdf value the reference computes for lobe l (not always
+// the true density: the Blinn and thin-film half-vector pdfs use
+// (n·h)^(α+1)/4 for (n·h)^α / (4 o·h)) and w_l its MIS weight against the
+// light, pL = that light's pdf for ω.  G is written only when want_G.
+template <int H>
+__device__ void guide_eval(const DevScene &sc, const lam_material &m, const Lam<H> &L, const Surf &s, f3 o, f3 wi,
+                           float p, float pL, bool want_G, float *G, float &q) {
+  q = 0.f;
+  if (want_G) {
+#pragma unroll
+    for (int k = 0; k < H; k++) G[k] = 0.f;
+  }
+  const f3 ns = s.n();
+  const float pL2 = pL * pL;
+  if (m.type == LAM_MAT_LAMBERTIAN) {
+    const float c = dot(wi, ns);
+    if (!(c > 0.f)) return;
+    const float pi = p * c * kInvPi;
+    q = pi;
+    if (want_G) {
+      float rho[H];
+      spec_eval<H>(sc, m.spec[0], L, rho);
+      const float w = pi * pi / (pi * pi + pL2);
+#pragma unroll
+      for (int k = 0; k < H; k++) G[k] = rho[k] * kInvPi * c * w;
+    }
+    return;
+  }
+  // reflection half vector on the sampled side (h·ns >= 0)
+  const f3 hv = add(o, wi);
+  const float l2 = len2(hv);
+  f3 h = mk(0.f, 0.f, 1.f);
+  const bool has_h = l2 > 1e-12f;
+  if (has_h) {
+    h = mul(hv, 1.f / sqrtf(l2));
+    if (dot(h, ns) < 0.f) h = neg(h);
+  }
+  if (m.type == LAM_MAT_BLINN) {
+    // lobe choice by xl in [0, p): glossy below p p_spec, diffuse below
+    // p (p_spec + p_d); sel_* are the true selection probabilities
+    const float pp_spec = p * m.p_spec, pp_d = p * m.p_d;
+    const float sel_s = fminf(p, pp_spec), sel_d = fmaxf(0.f, fminf(p, pp_spec + pp_d) - sel_s);
+    const float c = dot(wi, ns);
+    if (c > 0.f && pp_d > 0.f) {  // diffuse lobe
+      const float pi = pp_d * (c * kInvPi), qd = sel_d * (c * kInvPi);
+      q += qd;
+      if (want_G) {
+        float kd[H];
+        spec_eval<H>(sc, m.spec[0], L, kd);
+        const float w = pi * pi / (pi * pi + pL2) * (qd / pi);
+#pragma unroll
+        for (int k = 0; k < H; k++) G[k] += kd[k] * kInvPi * c * w;
+      }
+    }
+    if (has_h && sel_s > 0.f) {  // glossy reflection lobe
+      const float wz = dot(h, s.F.z);  // the sampler's local z (sign flipped on the back)
+      const float aw = fabsf(wz), ci = dot(o, h);
+      if (aw > 0.f && ci != 0.f) {
+        const float qr = sel_s * (m.alpha + 1) * .5f * kInvPi * lam_powf(aw, m.alpha) * .25f / fabsf(ci);
+        q += qr;
+        if (want_G) {
+          const float pr = pp_spec * blinnh_prob(m.alpha, wz) * .25f;
+          float ior[H], ks[H], kt[H];
+          spec_eval<H>(sc, m.spec[4], L, ior);
+          spec_eval<H>(sc, m.spec[1], L, ks);
+          spec_eval<H>(sc, m.spec[2], L, kt);
+          const float pw5 = lam_powf(1.f - ci, 5.f);
+          const float pw = lam_powf(wz, m.alpha);
+          const float w = pr * pr / (pr * pr + pL2);
+          const float ratio = pr > 0.f ? w * qr / pr : 0.f;
+#pragma unroll
+          for (int k = 0; k < H; k++) {
+            const float eta = s.front ? 1.f / ior[k] : ior[k];
+            const float ct2 = 1.f - eta * eta * (1.f - ci * ci);
+            const float f0 = F0s(eta);
+            const float F = ct2 < 0.f ? 1.f : f0 + (1.f - f0) * pw5;
+            G[k] += (ks[k] + F * kt[k]) * (m.alpha + 2) * .125f * kInvPi * pw * ratio;
+          }
+        }
+      }
+    }
+    return;
+  }
+  if (m.type == LAM_MAT_THINFILM) {
+    if (!has_h) return;
+    const f3 n = s.F.z;
+    const float nh = dot(n, h), oh = dot(o, h);
+    if (!(nh > 0.f) || oh == 0.f) return;
+    const float qf = p * (m.alpha + 1) * .5f * kInvPi * lam_powf(nh, m.alpha) * .25f / fabsf(oh);
+    q = qf;
+    if (want_G) {
+      const float pf = p * lam_powf(nh, m.alpha + 1) * (m.alpha + 1) * .5f * kInvPi * .25f;
+      float f[H];
+      thinfilm_f<H>(sc, m, L, wi, o, n, s.front, f);
+      const float w = pf * pf / (pf * pf + pL2);
+      const float ratio = pf > 0.f ? w * qf / pf : 0.f;
+#pragma unroll
+      for (int k = 0; k < H; k++) G[k] = f[k] * ratio;
+    }
+  }
+}
+
+// network inputs of a vertex, each in [0,1]: position in the scene box,
+// outgoing direction as (θ/π, φ/2π), hero wavelength on the grid
+template <int H>
+__device__ inline void guide_inputs(const GuideDev &gd, const DevScene &sc, f3 x, f3 o, const Lam<H> &L,
+                                    float *in) {
+  in[0] = fminf(fmaxf((x.x - gd.lo.x) * gd.inv_ext.x, 0.f), 1.f);
+  in[1] = fminf(fmaxf((x.y - gd.lo.y) * gd.inv_ext.y, 0.f), 1.f);
+  in[2] = fminf(fmaxf((x.z - gd.lo.z) * gd.inv_ext.z, 0.f), 1.f);
+  in[3] = acosf(fminf(fmaxf(o.z, -1.f), 1.f)) * kInvPi;
+  in[4] = (atan2f(o.y, o.x) + kPi) * (.5f * kInvPi);
+  in[5] = (L.l[0] - sc.grid_inf) / (sc.grid_sup - sc.grid_inf);
+}
+
+}  // namespace lam

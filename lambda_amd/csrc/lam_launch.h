@@ -42,6 +42,7 @@ struct WaveLaunch {
   uint32_t shade_blocks;    // shade grid (256-lane blocks)
   uint32_t tail_threshold;  // queue length at or below which k_finish takes over
   uint32_t max_iters;       // 0: pure wavefront; else cap on bounce iterations per pass
+  uint32_t guide_blocks;    // guiding-network grid (256-lane blocks, 51 KB LDS each)
 };
 // bounce iterations enqueued per pass
 inline uint32_t wave_iterations(uint32_t max_scat, const WaveLaunch &wl) {
@@ -56,10 +57,44 @@ struct WaveProfile {
 constexpr uint32_t kCtrlWords = 16;
 constexpr uint32_t kCtrlOverflow = 15;
 
+// ---- path guiding (lam_guide.hip; the guided shade kernel in lam_wavefront.hip) ----------
+// Device view of the guiding network for the shade kernel: inputs of a vertex
+// are its position in the scene box, its outgoing direction and its hero
+// wavelength, each mapped to [0,1].
+struct GuideDev {
+  const uint4 *frag;      // packed bf16 fragments + fp32 biases
+  uint4 *mix;             // network outputs per extend-queue entry: 40 fp16 (5 uint4)
+  float alpha;            // probability of the guide branch at supported vertices
+  f3 lo, inv_ext;         // scene box
+  float4 *rec;            // training records (4 float4 each); null when not training
+  uint32_t *rec_count;    // records appended (may exceed rec_cap; the excess is dropped)
+  uint32_t rec_cap;
+  uint32_t *rec_of_slot;  // record written by each path slot at the current vertex
+};
+// Training state (fp32 master parameters and Adam moments on the device)
+struct GuideTrain {
+  float *th, *m, *v, *grad, *thq, *scal, *ws;
+  void *pack;
+  uint32_t batch, step;
+  float lr, b1, b2, eps;
+  uint64_t seed;
+};
+size_t guide_train_bytes(uint32_t batch);
+hipError_t guide_pack(const float *th, void *pack, float *thq, hipStream_t st);
+hipError_t guide_infer(const void *pack, const float *in, uint32_t n, float *raw, uint32_t blocks, hipStream_t st);
+hipError_t guide_grad_rows(const float *thq, const float *rows, uint32_t batch, float *ws, float *grad, float *scal,
+                           hipStream_t st);
+hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t batch, uint32_t steps, hipStream_t st);
+hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t *rec_count, uint32_t rec_cap,
+                               uint32_t steps, hipStream_t st);
+// training weights of the records of the pass just traced (needs its final L)
+hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipStream_t st);
+
+// gd: null for the plain render loop; train: write training records
 hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
                         const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
-                        WaveProfile *prof);
+                        WaveProfile *prof, const GuideDev *gd = nullptr, bool train = false);
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
 
 // ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------

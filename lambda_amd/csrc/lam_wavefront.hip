@@ -23,6 +23,7 @@
 // the throughput β, the radiance L and the pending NEE term, H floats each.
 // Random numbers, hero wavelengths and the vertex index are recomputed from
 // (pixel, sample) and s, so replaying a path reproduces it exactly.
+#include "lam_guide.h"
 #include "lam_launch.h"
 #include "lam_shade.h"
 
@@ -280,10 +281,329 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds
   }
 }
 
+// ---- guided shade ----------------------------------------------------------------------
+// k_shade with the learned proposal.  At a vertex whose material supports it
+// the continuation is drawn from the vMF mixture with probability α
+// (dimension GUIDE) and from the reference's sampler otherwise; both branches
+// weight it by one-sample MIS over the mixture
+//   M(ω) = (1-α) q(ω) + α P_s p_g(ω),
+// q = the sampler's true density, P_s = its survival probability:
+//   guide branch:   β ← β ⊙ G(ω) / M(ω)        (survives with probability P_s)
+//   sampler branch: β ← β ⊙ mult/prob w_mat · q(ω) / M(ω)
+// with G the plain continuation's integrand (guide_eval), so the expectation
+// of every vertex is that of k_shade.  The NEE term, its MIS weight and its
+// dependence on survival are unchanged.  α = 0 reproduces k_shade.  The
+// mixture of each vertex comes from k_guide_queue.
+// TRAIN: each guided continuation appends a training record (network inputs,
+// ω, Σβ, ΣL before the vertex, ΣNEE term, slot; k_shadow marks a visible
+// NEE term).
+// The network for every entry of the extend queue (after k_extend): inputs
+// from the hit (position ≈ o + t d), the ray and the path's hero wavelength;
+// 40 outputs stored as fp16.  One wave evaluates 64 entries with MFMA, the
+// packed parameters staged in LDS once per block.
+template <int H>
+__global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                     GuideDev gd, uint32_t qsel, const uint32_t *__restrict__ ctrl) {
+  __shared__ uint4 sfrag[kGFrags * 64];
+  __shared__ float sbias[kGBiasWords];
+  if (ctrl[CTRL_TAIL]) return;
+  {
+    const float *gb = reinterpret_cast<const float *>(gd.frag + kGFrags * 64);
+    for (uint32_t i = threadIdx.x; i < uint32_t(kGFrags) * 64; i += blockDim.x) sfrag[i] = gd.frag[i];
+    for (uint32_t i = threadIdx.x; i < uint32_t(kGBiasWords); i += blockDim.x) sbias[i] = gb[i];
+  }
+  __syncthreads();
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t wbase = first - (threadIdx.x % 64); wbase < n; wbase += stride) {
+    const uint32_t q = wbase + (threadIdx.x % 64);
+    float gin[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool valid = false;
+    if (q < n) {
+      const Hit hit = ws.hits[q];
+      if (hit.obj != LAM_NONE) {
+        const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
+        const f3 d = mk(b.x, b.y, b.z);
+        const f3 x = add(mk(a.x, a.y, a.z), mul(d, hit.t));
+        uint32_t pixel, sample;
+        path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
+        Lam<H> lam;
+        lam_for<H>(sc, lam_path_key(pa.seed, pixel, sample), lam);
+        guide_inputs<H>(gd, sc, x, neg(normalized(d)), lam, gin);
+        valid = true;
+      }
+    }
+    const uint64_t vmask = __ballot(valid);
+    if (!vmask) continue;
+    f32x4 o[4][3];
+    mlp_wave(sfrag, sbias, gin, o);
+    // lane (n, g) holds rows 16t + 4g .. +3 of the query in lane 16G + n
+    const uint32_t lane = threadIdx.x % 64, ln = lane % 16, lg = lane / 16;
+    uint2 *mix = reinterpret_cast<uint2 *>(gd.mix);
+#pragma unroll
+    for (int G = 0; G < 4; G++) {
+      const uint32_t src = 16 * G + ln;
+      if (!((vmask >> src) & 1u)) continue;
+#pragma unroll
+      for (int t = 0; t < 3; t++) {
+        const uint32_t row = 16 * t + 4 * lg;
+        if (row < uint32_t(kGOut)) {
+          const f32x4 v = o[G][t];
+          const uint32_t w0 = uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[0]))) |
+                              (uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[1]))) << 16);
+          const uint32_t w1 = uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[2]))) |
+                              (uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[3]))) << 16);
+          mix[size_t(wbase + src) * 10 + row / 4] = make_uint2(w0, w1);
+        }
+      }
+    }
+  }
+}
+
+__device__ inline void load_mix(const uint4 *src, float *raw) {
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+    const uint4 v = src[c];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      raw[8 * c + 2 * j] = float(__builtin_bit_cast(_Float16, uint16_t(w[j] & 0xFFFFu)));
+      raw[8 * c + 2 * j + 1] = float(__builtin_bit_cast(_Float16, uint16_t(w[j] >> 16)));
+    }
+  }
+}
+
+template <int H, bool TRAIN>
+__global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                      GuideDev gd, uint32_t s, uint32_t qsel,
+                                                      uint32_t *__restrict__ ctrl) {
+  if (ctrl[CTRL_TAIL]) return;
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
+  const size_t cap = ws.cap;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wave_first = first - (threadIdx.x % 64);
+  for (uint32_t wbase = wave_first; wbase < n; wbase += stride) {
+    const uint32_t q = wbase + (threadIdx.x % 64);
+    bool push_next = false, push_shadow = false, live = false, guided = false, want_rec = false;
+    uint32_t p = 0, mat_id = 0;
+    Ray nxt, shd;
+    uint64_t key = 0;
+    Lam<H> lam;
+    float beta[H];
+    Surf sf;
+    f3 o = mk(0.f, 0.f, 1.f);
+    float gin[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (q < n) {
+      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
+      const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+      p = __float_as_uint(b.w);
+      const Hit hit = ws.hits[q];
+      if (hit.obj != LAM_NONE) {
+        uint32_t pixel, sample;
+        path_id(ids, ctrl, p, pixel, sample);
+        key = lam_path_key(pa.seed, pixel, sample);
+        lam_for<H>(sc, key, lam);
+#pragma unroll
+        for (int k = 0; k < H; k++) beta[k] = ws.beta[size_t(k) * cap + p];
+        sf = surface(sc, hit, ro, rd);
+        o = neg(normalized(rd));
+        mat_id = sf.mat;
+        const lam_material &mat = sc.materials[mat_id];
+        if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
+          float le[H];
+          spec_eval<H>(sc, mat.spec[0], lam, le);
+          float f = 1.f;
+          if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+#pragma unroll
+          for (int k = 0; k < H; k++)
+            ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+        } else if (s <= pa.max_scat) {
+          live = true;
+          guided = gd.alpha > 0.f && guide_supported(mat, sf, o);
+          if (TRAIN && guided) guide_inputs<H>(gd, sc, add(ro, mul(rd, hit.t)), o, lam, gin);
+        }
+      }
+    }
+    // phase 2: NEE and the continuation
+    float rec_b = 0.f, rec_l = 0.f, rec_p = 0.f, rec_v = 0.f;  // training record sums
+    if (live) {
+      const lam_material &mat = sc.materials[mat_id];
+      const f3 nn = sf.n();
+      const uint32_t nl = sc.n_lights;
+      const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+      const float pl = 1.f / float(nl);
+      const lam_light &lt = sc.lights[li];
+      LightSample<H> ls;
+      sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                      lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+      float coef[H];
+      bxdf<H>(sc, mat, lam, ls.dir, o, nn, sf.front, coef);
+      float lis[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+      const float alpha = guided ? gd.alpha : 0.f;
+      const float P = pa.sample_p;
+      // survival probability of the reference's sampler at this material
+      const float Ps = mat.type == LAM_MAT_BLINN ? fminf(P, P * (mat.p_spec + mat.p_d)) : P;
+      const uint4 *mixq = gd.mix + 5 * size_t(q);
+      bool ok = false;
+      f3 wi = mk(0.f, 0.f, 0.f);
+      float fac[H];
+      const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
+      if (lam_draw_f(key, lam_vdim(s, LAM_VDIM_GUIDE)) < alpha) {
+        if (xl < Ps) {
+          ok = true;
+          float pg;
+          {
+            GuideMix gm;
+            float raw[kGOut];
+            load_mix(mixq, raw);
+            mix_from_raw(raw, gm);
+            wi = mix_sample(gm, lam_draw_f(key, lam_vdim(s, LAM_VDIM_SPARE)),
+                            lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)),
+                            lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)));
+            pg = mix_pdf(gm, wi);
+          }
+          const float pLw = prob_light(sc, lt, sf.p, wi) * pl;
+          float G[H], qv;
+          guide_eval<H>(sc, mat, lam, sf, o, wi, P, pLw, true, G, qv);
+          const float M = (1.f - alpha) * qv + alpha * Ps * pg;
+          const float inv = M > 0.f ? 1.f / M : 0.f;
+#pragma unroll
+          for (int k = 0; k < H; k++) fac[k] = G[k] * inv;
+        }
+      } else {
+        MatSample<H> ms;
+        sample_material<H>(sc, mat, lam, sf, o, xl, lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)),
+                           lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)), P, ms);
+        if (ms.ok) {
+          ok = true;
+          wi = ms.dir;
+          const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+          const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+          float scale = 1.f;
+          if (alpha > 0.f) {
+            float qv, pg;
+            guide_eval<H>(sc, mat, lam, sf, o, wi, P, 0.f, false, nullptr, qv);
+            {
+              GuideMix gm;
+              float raw[kGOut];
+              load_mix(mixq, raw);
+              mix_from_raw(raw, gm);
+              pg = mix_pdf(gm, wi);
+            }
+            const float M = (1.f - alpha) * qv + alpha * Ps * pg;
+            scale = M > 0.f ? qv / M : 1.f / (1.f - alpha);
+          }
+#pragma unroll
+          for (int k = 0; k < H; k++) fac[k] = ms.mult[k] / ms.prob * w_mat * scale;
+        }
+      }
+      float pend[H];
+      if (!ok) {
+#pragma unroll
+        for (int k = 0; k < H; k++) pend[k] = beta[k] * lis[k];
+      } else {
+        const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], P);
+        const float p_L_i = ls.prob * pl;
+        const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+          pend[k] = beta[k] * (lis[k] * w_L);
+          const float nb = beta[k] * fac[k];
+          any = any || nb != 0.f;
+          ws.beta[size_t(k) * cap + p] = nb;
+        }
+        // a zero-weight continuation carries nothing: drop it (the plain
+        // kernel traces it with β = 0)
+        push_next = any || alpha == 0.f;
+        nxt.o = sf.p;
+        nxt.d = wi;
+      }
+      if (TRAIN) want_rec = guided && push_next;
+      if (TRAIN && want_rec) {
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+          rec_b += beta[k];
+          rec_l += ws.L[size_t(k) * cap + p];
+          rec_p += pend[k];
+        }
+        rec_v = ls.shadow ? 0.f : 1.f;
+      }
+      if (ls.shadow) {
+#pragma unroll
+        for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + p] = pend[k];
+        push_shadow = true;
+        shd = ls.sray;
+      } else {
+#pragma unroll
+        for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += pend[k];
+      }
+    }
+    if (TRAIN) {
+      const uint32_t r = wave_append(want_rec, gd.rec_count);
+      if (live) gd.rec_of_slot[p] = (want_rec && r < gd.rec_cap) ? r : LAM_NONE;
+      if (want_rec && r < gd.rec_cap) {
+        float4 *R = gd.rec + 4 * size_t(r);
+        R[0] = make_float4(gin[0], gin[1], gin[2], gin[3]);
+        R[1] = make_float4(gin[4], gin[5], rec_b, rec_l);
+        R[2] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, rec_p);
+        R[3] = make_float4(__uint_as_float(p), rec_v, 0.f, 0.f);
+      }
+    }
+    const uint32_t qn = wave_append(push_next, ctrl + CTRL_COUNT0 + (qsel ^ 1u));
+    if (push_next) {
+      rout[2 * size_t(qn)] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
+      rout[2 * size_t(qn) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
+    }
+    const uint32_t qs = wave_append(push_shadow, ctrl + CTRL_SHADOW);
+    if (push_shadow) {
+      float4 *sr = reinterpret_cast<float4 *>(ws.sray);
+      sr[2 * size_t(qs)] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
+      sr[2 * size_t(qs) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
+      ws.spath[qs] = p;
+    }
+  }
+}
+
+// training weight of each record of the pass just traced:
+//   w = max(0, ΣL_final - ΣL_before - [NEE visible] ΣNEE) / Σβ
+template <int H>
+__global__ void k_guide_weights(WaveState ws, float4 *__restrict__ rec, const uint32_t *__restrict__ rec_count,
+                                uint32_t rec_cap) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nr = min(*rec_count, rec_cap);
+  if (r >= nr) return;
+  const float4 b = rec[4 * size_t(r) + 1], c = rec[4 * size_t(r) + 2], d = rec[4 * size_t(r) + 3];
+  const uint32_t p = __float_as_uint(d.x);
+  float Lf = 0.f;
+#pragma unroll
+  for (int k = 0; k < H; k++) Lf += ws.L[size_t(k) * ws.cap + p];
+  const float dl = Lf - b.w - d.y * c.w;
+  rec[4 * size_t(r) + 3].z = b.z > 0.f ? fmaxf(dl, 0.f) / b.z : 0.f;
+}
+
+hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipStream_t st) {
+  const uint32_t blocks = (gd.rec_cap + 255) / 256;
+  if (hero == 16)
+    hipLaunchKernelGGL((k_guide_weights<16>), dim3(blocks), dim3(256), 0, st, ws, gd.rec, gd.rec_count, gd.rec_cap);
+  else
+    hipLaunchKernelGGL((k_guide_weights<4>), dim3(blocks), dim3(256), 0, st, ws, gd.rec, gd.rec_count, gd.rec_cap);
+  return hipGetLastError();
+}
+
 // ---- shadow: persistent any-hit, then the pending NEE term -----------------------------
-template <int H, bool COUNT>
+template <int H, bool COUNT, bool REC = false>
 __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32_t *__restrict__ ctrl,
-                                               uint32_t *__restrict__ counts) {
+                                               uint32_t *__restrict__ counts, float4 *__restrict__ rec = nullptr,
+                                               const uint32_t *__restrict__ rec_of_slot = nullptr,
+                                               uint32_t rec_cap = 0) {
   __shared__ uint32_t stack[kStack * 64];
   const uint32_t lane = threadIdx.x;
   if (ctrl[CTRL_TAIL]) return;
@@ -309,6 +629,10 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
         const uint32_t p = ws.spath[i];
 #pragma unroll
         for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += ws.pend[size_t(k) * cap + p];
+        if (REC) {  // the vertex's training record: its NEE term reached L
+          const uint32_t r = rec_of_slot[p];
+          if (r < rec_cap) rec[4 * size_t(r) + 3].y = 1.f;
+        }
       }
       if (COUNT) {
         atomicAdd(counts + 3, c.nodes);
@@ -390,7 +714,7 @@ __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds 
 template <int H>
 static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const PathIds &ids, const WaveState &ws,
                                  uint32_t n, uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl,
-                                 hipStream_t st, WaveProfile *prof) {
+                                 hipStream_t st, WaveProfile *prof, const GuideDev *gd, bool train) {
   const uint32_t b256 = (n + 255) / 256;
   auto ev = [&](int slot) {
     if (prof) (void)hipEventRecord(prof->ev[slot], st);
@@ -415,9 +739,23 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
                          reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, nullptr);
     }
-    hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, s, q, ctrl);
+    if (gd)
+      hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, q, ctrl);
+    if (!gd)
+      hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, s, q, ctrl);
+    else if (train)
+      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd,
+                         s, q, ctrl);
+    else
+      hipLaunchKernelGGL((k_shade_guided<H, false>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws,
+                         *gd, s, q, ctrl);
     if (prof) (void)hipEventRecord(prof->shd0[s], st);
-    hipLaunchKernelGGL((k_shadow<H, false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr);
+    if (gd && train)
+      hipLaunchKernelGGL((k_shadow<H, false, true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
+                         gd->rec, gd->rec_of_slot, gd->rec_cap);
+    else
+      hipLaunchKernelGGL((k_shadow<H, false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
+                         nullptr, nullptr, 0u);
     if (prof) (void)hipEventRecord(prof->shd1[s], st);
     q ^= 1u;
   }
@@ -430,10 +768,10 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
 hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
                         const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
-                        WaveProfile *prof) {
+                        WaveProfile *prof, const GuideDev *gd, bool train) {
   PathIds ids{pixel, sample, pixel_offset};
-  if (hero == 16) return enqueue_pass_t<16>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof);
-  return enqueue_pass_t<4>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof);
+  if (hero == 16) return enqueue_pass_t<16>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof, gd, train);
+  return enqueue_pass_t<4>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof, gd, train);
 }
 
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st) {

@@ -22,6 +22,15 @@ OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
                      8 x 8 grid in front of the back wall; 512x512.  The
                      workload of the path-replay adjoint.
 
+  glossy_interior(dir)
+                     indirectly lit glossy interior: a 10 m x 10 m x 4 m room
+                     shell (Lambertian), 200 instances of a rounded box
+                     (subdivision 14: 2,352 tris; ~470 k in total) with Blinn
+                     lobes (α uniform 50-500, no transmission), and one small
+                     light under the ceiling hidden by a plate below it, so
+                     the room is lit by the plate's and the ceiling's bounce;
+                     1024x1024.  The workload of the learned proposal.
+
 Each function returns the path of the .nls file.  Same seed, same bytes.
 """
 from __future__ import annotations
@@ -366,6 +375,134 @@ def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778,
         "lights": lights, "objects": objects, "materials": materials,
     }
     path = os.path.join(out_dir, "icosphere_field.nls")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
+
+
+def rounded_box(k, p=8.0):
+    """cube faces subdivided k x k, pushed onto the superellipsoid
+    |x|^p + |y|^p + |z|^p = 1 (rounded edges); smooth vertex normals"""
+    verts, index, faces = [], {}, []
+
+    def vid(v):
+        key = tuple(np.round(v, 9))
+        if key not in index:
+            index[key] = len(verts)
+            verts.append(v)
+        return index[key]
+
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            u_ax, v_ax = [a for a in range(3) if a != axis]
+            grid = {}
+            for i in range(k + 1):
+                for j in range(k + 1):
+                    v = np.zeros(3)
+                    v[axis] = sgn
+                    v[u_ax] = -1 + 2 * i / k
+                    v[v_ax] = -1 + 2 * j / k
+                    grid[i, j] = vid(v)
+            for i in range(k):
+                for j in range(k):
+                    a, b, c, d = grid[i, j], grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]
+                    faces += [(a, b, c), (a, c, d)]
+    V = np.array(verts)
+    V = V / (np.sum(np.abs(V) ** p, axis=1) ** (1.0 / p))[:, None]
+    F = np.array(faces)
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    fn = np.cross(b - a, c - a)
+    flip = np.einsum("ij,ij->i", fn, a + b + c) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    N = np.zeros_like(V)
+    for t in range(3):
+        np.add.at(N, F[:, t], fn)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    return V, F, N
+
+
+def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14):
+    os.makedirs(out_dir, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    lam_ref = np.arange(355.0, 836.0, 5.0)
+    lam_bb = np.arange(355.0, 841.0, 1.0)
+    bb = blackbody(4500.0, lam_bb)
+    write_csv(os.path.join(out_dir, "blackbody4500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
+    P = lambda x, y, z: (x, y, z)
+    X, Y, H = 5.0, 5.0, 4.0
+    shell = {
+        "floor": (P(-X, -Y, 0), P(X, -Y, 0), P(X, Y, 0), P(-X, Y, 0)),
+        "ceiling": (P(-X, -Y, H), P(-X, Y, H), P(X, Y, H), P(X, -Y, H)),
+        "wall_n": (P(-X, Y, 0), P(X, Y, 0), P(X, Y, H), P(-X, Y, H)),
+        "wall_s": (P(X, -Y, 0), P(-X, -Y, 0), P(-X, -Y, H), P(X, -Y, H)),
+        "wall_e": (P(X, Y, 0), P(X, -Y, 0), P(X, -Y, H), P(X, Y, H)),
+        "wall_w": (P(-X, -Y, 0), P(-X, Y, 0), P(-X, Y, H), P(-X, -Y, H)),
+    }
+
+    def inward(q):
+        a, b, c = (np.array(x, dtype=float) for x in q[:3])
+        n = np.cross(b - a, c - a)
+        center = np.mean(np.array(q, dtype=float), axis=0)
+        return q if np.dot(n, np.array([0, 0, H / 2]) - center) > 0 else tuple(reversed(q))
+
+    objects, materials = [], []
+    for nm, q in shell.items():
+        write_csv(os.path.join(out_dir, f"albedo_{nm}.csv"), lam_ref,
+                  0.5 + 0.4 * smooth_reflectance(rng, lam_ref))
+        materials.append({"name": f"mat_{nm}", "type": "lambertian",
+                          "albedo": {"val": os.path.join(out_dir, f"albedo_{nm}.csv")}})
+        V, F, N = _mesh_from_quads([inward(q)])
+        write_obj(os.path.join(out_dir, f"{nm}.obj"), V, F, N)
+        objects.append({"name": nm, "type": "trimesh", "transform": {}, "material": f"mat_{nm}",
+                        "source": os.path.join(out_dir, f"{nm}.obj")})
+    # the light hangs under the ceiling at (lx, ly), facing down; a 2 m plate
+    # 0.3 m below hides it from the room, which is lit by the plate's and
+    # then the ceiling's bounce.  (The exponent is even: with an odd one the
+    # reference's quad "dir" light gives negative radiance to points behind
+    # it, cg.hpp:1625-1668, and the ceiling is.)
+    lx, ly, ph = 2.5, 2.5, 1.0
+    V, F, N = _mesh_from_quads(_block(lx - ph, ly - ph, lx + ph, ly + ph, 0.05))
+    write_obj(os.path.join(out_dir, "plate.obj"), V, F, N)
+    write_csv(os.path.join(out_dir, "albedo_plate.csv"), lam_ref, 0.8 * np.ones_like(lam_ref))
+    materials.append({"name": "mat_plate", "type": "lambertian",
+                      "albedo": {"val": os.path.join(out_dir, "albedo_plate.csv")}})
+    objects.append({"name": "plate", "type": "trimesh", "material": "mat_plate",
+                    "source": os.path.join(out_dir, "plate.obj"), "transform": {"translate": [0, 0, H - 0.35]}})
+    # glossy boxes
+    V, F, N = rounded_box(box_subdiv)
+    write_obj(os.path.join(out_dir, "rbox.obj"), V, F, N)
+    placed = 0
+    while placed < n_boxes:
+        x, y = rng.uniform(-4.4, 4.4, 2)
+        if (x + 4.5) ** 2 + (y + 4.5) ** 2 < 2.0 ** 2:  # keep the camera's corner clear
+            continue
+        sc = float(rng.uniform(0.15, 0.45))
+        z = float(rng.uniform(sc, 3.0))
+        kd = os.path.join(out_dir, f"kd_{placed}.csv")
+        write_csv(kd, lam_ref, 0.6 * smooth_reflectance(rng, lam_ref))
+        alpha = 2 * int(round(rng.uniform(50, 500) / 2))
+        materials.append({"name": f"box{placed}", "type": "blinn", "Kd": {"val": kd}, "Ks": {"val": 0.35},
+                          "ior": {"val": 1.5}, "glossiness": alpha})
+        objects.append({"name": f"box{placed}", "type": "trimesh", "material": f"box{placed}",
+                        "source": os.path.join(out_dir, "rbox.obj"),
+                        "transform": {"scale": [sc, sc, sc],
+                                      "rotate": {"axis": [0, 0, 1], "degrees": float(rng.uniform(0, 90))},
+                                      "translate": [float(x), float(y), z]}})
+        placed += 1
+    scene = {
+        "spectral_grid": GRID,
+        "hero_samples": hero,
+        "camera": {"pos": [-4.5, -4.5, 1.7], "look_at": [1.0, 1.0, 1.2], "up": [0, 0, 1], "fov": 70,
+                   "ar": 1.0, "width": width},
+        "lights": [{"name": "hidden", "type": "quaddir", "specular": 2,
+                    "transform": {"scale": [0.15, 0.15, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
+                                  "translate": [lx, ly, H - 0.01]},
+                    "radiance": {"val": os.path.join(out_dir, "blackbody4500.csv"), "mult": 400.0}}],
+        "objects": objects,
+        "materials": materials,
+    }
+    path = os.path.join(out_dir, "glossy_interior.nls")
     with open(path, "w") as f:
         json.dump(scene, f, indent=1)
     return path

@@ -1,0 +1,241 @@
+"""Learned path guiding: the network (MFMA inference, training gradient,
+Adam), and the guided render (unbiased against the plain render loop).
+
+CPU:  the numpy oracle's hand-written backward against torch float64
+      autograd of the same forward; the vMF mixture integrates to 1.
+GPU:  lam_guide_infer vs the oracle forward (same bf16 roundings);
+      lam_guide_loss_grad vs the oracle gradient; Adam's first step;
+      training fits a known distribution; lam_render_guided with α = 0 is
+      lam_render; the guided image has the plain image's expectation.
+"""
+import numpy as np
+import pytest
+
+import lambda_amd as la
+from oracle import guide as og
+
+
+def _rows(rng, n, theta=None):
+    inp = rng.uniform(0, 1, (n, 6))
+    om = rng.normal(size=(n, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    w = rng.exponential(1.0, n)
+    return np.concatenate([inp, om, w[:, None]], axis=1).astype(np.float32)
+
+
+def _theta(seed):
+    """parameters drawn like lam_guide_reset's (scale only; values differ)"""
+    rng = np.random.default_rng(seed)
+    P = {}
+    for name, shape in og._SIZES:
+        fan = shape[1] if len(shape) == 2 else 1
+        P[name] = rng.uniform(-1, 1, shape) * (np.sqrt(6.0 / fan) if len(shape) == 2 else 0.1)
+    P["W5"] *= 0.3
+    return og.pack(P).astype(np.float32)
+
+
+# ---- CPU ---------------------------------------------------------------------------
+
+def test_oracle_backward_matches_autograd():
+    torch = pytest.importorskip("torch")
+    th = _theta(1).astype(np.float64)
+    rows = _rows(np.random.default_rng(2), 300).astype(np.float64)
+    loss, g = og.loss_grad(th, rows)
+
+    T = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    parts, off = {}, 0
+    for name, shape in og._SIZES:
+        n = int(np.prod(shape))
+        parts[name] = T[off:off + n].reshape(shape)
+        off += n
+
+    def st(x, rounded):  # straight-through rounding
+        return x + (torch.as_tensor(rounded, dtype=torch.float64) - x).detach()
+
+    h = torch.as_tensor(og.features(rows[:, :6]), dtype=torch.float64)
+    for l in range(1, 6):
+        W = parts[f"W{l}"]
+        Wq = st(W, og.bf16(W.detach().numpy()))
+        b = parts[f"b{l}"]
+        bq = st(b, b.detach().numpy().astype(np.float32).astype(np.float64))
+        z = h @ Wq.T + bq
+        if l < 5:
+            a = torch.relu(z)
+            h = st(a, og.bf16(a.detach().numpy()))
+    r = z.reshape(-1, 8, 5)
+    mu = r[:, :, :3] / r[:, :, :3].norm(dim=2, keepdim=True)
+    lk = r[:, :, 3]
+    kap = torch.exp(torch.clamp(lk, og.LOGK_MIN, og.LOGK_MAX))
+    logpi = torch.log_softmax(r[:, :, 4], dim=1)
+    om = torch.as_tensor(rows[:, 6:9])
+    md = (mu * om[:, None, :]).sum(dim=2)
+    lv = torch.log(kap) - np.log(2 * np.pi) - torch.log(-torch.expm1(-2 * kap)) + kap * (md - 1)
+    logp = torch.logsumexp(logpi + lv, dim=1)
+    w = torch.as_tensor(rows[:, 9])
+    L = -(w / w.sum() * logp).sum()
+    L.backward()
+    assert abs(L.item() - loss) <= 1e-12 * abs(loss)
+    gt = T.grad.numpy()
+    assert np.allclose(g, gt, rtol=1e-9, atol=1e-12 * np.abs(gt).max())
+
+
+def test_mixture_normalised():
+    """∫ p dω = 1 on a Fibonacci sphere (κ up to ~60)"""
+    rng = np.random.default_rng(4)
+    N = 400000
+    k = np.arange(N) + 0.5
+    z = 1 - 2 * k / N
+    phi = np.pi * (1 + 5 ** 0.5) * k
+    s = np.sqrt(1 - z * z)
+    dirs = np.stack([s * np.cos(phi), s * np.sin(phi), z], axis=1)
+    for _ in range(3):
+        raw = rng.normal(size=40)
+        raw[3::5] = rng.uniform(-3, 4.1, 8)
+        p = og.mix_pdf(np.repeat(raw[None, :], N, axis=0), dirs)
+        assert abs(p.mean() * 4 * np.pi - 1) < 2e-3
+
+
+def test_features_layout():
+    """features 24d + 2j + c are sin/cos(π 2^j x_d), then the raw inputs"""
+    x = np.array([[0.1, 0.25, 0.5, 0.75, 0.9, 0.3]], np.float32)
+    F = og.features(x)[0]
+    assert F.shape == (150,)
+    assert F[144:].tolist() == og.bf16(x[0]).tolist()
+    assert abs(F[24 * 2 + 2 * 0] - np.sin(np.pi * 0.5)) < 1e-2       # sin(π x_2), x_2 = 0.5
+    assert abs(F[24 * 1 + 2 * 1 + 1] - np.cos(2 * np.pi * 0.25)) < 1e-2  # cos(2π x_1)
+
+
+# ---- GPU ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = la.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.gpu
+def test_infer_matches_oracle(ctx):
+    th = _theta(3)
+    ctx.guide_set_params(th)
+    assert np.array_equal(ctx.guide_get_params(), th)
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(0, 1, (4093, 6)), np.zeros((1, 6)), np.ones((1, 6))]).astype(np.float32)
+    g = ctx.guide_infer(x).astype(np.float64)
+    o = og.forward(th, x)
+    scale = np.abs(o).max()
+    err = np.abs(g - o)
+    # MFMA fp32 sums in another order; a bf16 activation can round the other way
+    assert np.quantile(err, 0.99) <= 1e-3 * scale, np.quantile(err, 0.99) / scale
+    assert err.max() <= 3e-2 * scale, err.max() / scale
+
+
+@pytest.mark.gpu
+def test_loss_grad_matches_oracle(ctx):
+    th = _theta(6)
+    ctx.guide_set_params(th)
+    rows = _rows(np.random.default_rng(7), 2048)
+    loss, g = ctx.guide_loss_grad(rows)
+    lo, go = og.loss_grad(th, rows)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    assert rel <= 5e-3, rel
+    # every layer, not just the largest
+    off = 0
+    for name, shape in og._SIZES:
+        n = int(np.prod(shape))
+        a, b = g[off:off + n], go[off:off + n]
+        assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-7, name
+        off += n
+
+
+@pytest.mark.gpu
+def test_adam_first_step(ctx):
+    """Adam's first step moves every parameter by lr · g/(|g| + ε)"""
+    th = _theta(8)
+    ctx.guide_set_params(th)
+    rows = _rows(np.random.default_rng(9), 1024)
+    _, g = ctx.guide_loss_grad(rows)
+    lr = 1e-3
+    ctx.guide_train(rows, 1, lr)
+    th1 = ctx.guide_get_params().astype(np.float64)
+    want = th - lr * g / (np.abs(g) + 1e-8)
+    assert np.allclose(th1, want, rtol=0, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_training_fits_a_direction_field(ctx):
+    """rows whose ω concentrates around a direction that depends on the
+    position: training lowers the loss and the mixture learns the directions"""
+    rng = np.random.default_rng(10)
+    n = 8192
+    inp = rng.uniform(0, 1, (n, 6)).astype(np.float32)
+    ang = 2 * np.pi * inp[:, 0]
+    mu = np.stack([np.cos(ang), np.sin(ang), 0.3 * np.ones(n)], axis=1)
+    mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+    om = mu + 0.15 * rng.normal(size=(n, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    rows = np.concatenate([inp, om, np.ones((n, 1))], axis=1).astype(np.float32)
+    ctx.guide_reset(11)
+    l0, _ = ctx.guide_loss_grad(rows)
+    for _ in range(10):
+        ctx.guide_train(rows, 40, 3e-3)
+    l1, _ = ctx.guide_loss_grad(rows)
+    assert l1 < l0 - 2.0, (l0, l1)
+    # the fitted density is high at the true mean direction
+    raw = ctx.guide_infer(inp[:512])
+    p_true = og.mix_pdf(raw, mu[:512])
+    p_anti = og.mix_pdf(raw, -mu[:512])
+    assert np.median(p_true) > 20 * np.median(p_anti)
+
+
+@pytest.fixture(scope="module")
+def glossy(tmp_path_factory):
+    from lambda_amd import scenes
+    p = scenes.glossy_interior(str(tmp_path_factory.mktemp("glossy")), seed=4, width=48, n_boxes=24,
+                               box_subdiv=3)
+    return la.load_nls(p)
+
+
+@pytest.mark.gpu
+def test_guided_alpha0_is_plain(ctx, glossy):
+    ctx.upload(glossy)
+    p = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=4, seed=3)
+    plain = ctx.render(p)
+    ctx.guide_reset(1)
+    g = ctx.render_guided(p, la.guide_params(alpha=0.0, train_passes=0))
+    assert np.array_equal(plain, g)
+
+
+@pytest.mark.gpu
+def test_guided_render_unbiased(ctx, glossy):
+    """guided and plain renders are independent estimates of one image: the
+    per-pixel difference of their means is noise (z-test), with a trained
+    network and with an untrained one"""
+    ctx.upload(glossy)
+    spp = 256
+    p = la.params(spp=spp, max_scatterings=8, sample_p=0.8, hero=4, seed=5)
+    plain = []
+    for half in range(2):  # two halves for a per-pixel variance estimate
+        plain.append(ctx.render(la.params(spp=spp // 2, max_scatterings=8, sample_p=0.8, hero=4, seed=5,
+                                          spp_offset=half * (spp // 2))).astype(np.float64))
+    for trained in (False, True):
+        ctx.guide_reset(2)
+        gp = la.guide_params(alpha=0.5, train_passes=8 if trained else 0, steps_per_pass=8, batch=4096,
+                             lr=5e-3, seed=7)
+        guided = []
+        for half in range(2):
+            pg = la.params(spp=spp // 2, max_scatterings=8, sample_p=0.8, hero=4, seed=99 + half,
+                           spp_offset=0)
+            guided.append(ctx.render_guided(pg, gp if half == 0 else
+                                            la.guide_params(alpha=0.5, train_passes=0, seed=7)).astype(np.float64))
+        mp, mg = (plain[0] + plain[1]) / 2, (guided[0] + guided[1]) / 2
+        # variance of a half-mean from the two halves; of the full mean: /2
+        vp = (plain[0] - plain[1]) ** 2 / 4
+        vg = (guided[0] - guided[1]) ** 2 / 4
+        z = (mg - mp) / np.sqrt(vp + vg + 1e-30)
+        # two-sample estimates are heavy-tailed: compare image-level sums
+        tot_p, tot_g = mp.sum(), mg.sum()
+        sd = np.sqrt(vp.sum() + vg.sum())
+        assert abs(tot_g - tot_p) < 4 * sd, (trained, tot_p, tot_g, sd)
+        assert np.median(np.abs(z)) < 1.2, (trained, np.median(np.abs(z)))
