@@ -287,8 +287,9 @@ int lam_guide_get_params(lam_ctx *ctx, float *theta);
 /* Network outputs (n x 40) for inputs (n x 6), through the MFMA kernel. */
 int lam_guide_infer(lam_ctx *ctx, const float *in, size_t n, float *raw);
 /* Training loss -Σ (w_i/Σw) log p(ω_i | in_i) and its gradient (param_count
- * floats) for rows (n x 10: in[6], ω[3], w), 1 <= n <= 65536. */
-int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad);
+ * floats) for rows (n x 10: in[6], ω[3], w), 1 <= n <= 65536; raw
+ * (optional, n x 40): the training forward's network outputs. */
+int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad, float *raw);
 /* `steps` Adam steps (β1 0.9, β2 0.99) on one batch of rows; *loss = the
  * last step's loss before its update. */
 int lam_guide_train(lam_ctx *ctx, const float *rows, size_t n, uint32_t steps, float lr, float *loss);

@@ -176,7 +176,7 @@ def lib():
         "lam_guide_set_params": (C.c_int, [vp, fp]),
         "lam_guide_get_params": (C.c_int, [vp, fp]),
         "lam_guide_infer": (C.c_int, [vp, fp, sz, fp]),
-        "lam_guide_loss_grad": (C.c_int, [vp, fp, sz, fp, fp]),
+        "lam_guide_loss_grad": (C.c_int, [vp, fp, sz, fp, fp, fp]),
         "lam_guide_train": (C.c_int, [vp, fp, sz, C.c_uint32, C.c_float, fp]),
         "lam_extend": (C.c_int, [vp, vp, sz, vp]),
         "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
@@ -603,12 +603,14 @@ class Context:
         self._check(self._lib.lam_guide_infer(self._h, _fptr(x), len(x), _fptr(raw)))
         return raw
 
-    def guide_loss_grad(self, rows):
+    def guide_loss_grad(self, rows, want_raw=False):
         r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 10)
         loss = C.c_float()
         grad = np.zeros(self.guide_params_count(), dtype=np.float32)
-        self._check(self._lib.lam_guide_loss_grad(self._h, _fptr(r), len(r), C.byref(loss), _fptr(grad)))
-        return loss.value, grad
+        raw = np.zeros((len(r), GUIDE_OUT), dtype=np.float32) if want_raw else None
+        self._check(self._lib.lam_guide_loss_grad(self._h, _fptr(r), len(r), C.byref(loss), _fptr(grad),
+                                                  _fptr(raw) if want_raw else None))
+        return (loss.value, grad, raw) if want_raw else (loss.value, grad)
 
     def guide_train(self, rows, steps, lr):
         r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 10)

@@ -880,15 +880,18 @@ static int guide_rows_in(lam_ctx *ctx, const float *rows, size_t n) {
   return LAM_OK;
 }
 
-int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad) {
+int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, float *grad, float *raw) {
   if (!ctx) return LAM_E_INVALID;
   int rc;
   if ((rc = guide_rows_in(ctx, rows, n))) return rc;
-  HIPC(lam::guide_grad_rows(ctx->g_thq, static_cast<const float *>(ctx->scratch[0]), uint32_t(n), ctx->g_ws,
-                            ctx->g_grad, ctx->g_scal, ctx->stream));
+  HIPC(lam::guide_grad_rows(ctx->g_th, ctx->g_thq, static_cast<const float *>(ctx->scratch[0]), uint32_t(n),
+                            ctx->g_ws, ctx->g_grad, ctx->g_scal, ctx->stream));
   float sc[2] = {0.f, 0.f};
   HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
   if (grad) HIPC(hipMemcpyAsync(grad, ctx->g_grad, size_t(lam::kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (raw)
+    HIPC(hipMemcpyAsync(raw, lam::guide_train_raw(ctx->g_ws, uint32_t(n)), n * lam::kGOut * 4,
+                        hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   if (loss) *loss = sc[1];
   return LAM_OK;

@@ -64,6 +64,23 @@ __host__ __device__ inline int g_perm(int k) {
   return 32 * c + 16 * (j >> 2) + 4 * g + (j & 3);
 }
 
+// sin and cos of 2π fr for fr in [0, 1): reduction to the nearest quarter
+// turn (exact), then Taylor polynomials of degree 9 (sin) and 10 (cos) on
+// [-π/4, π/4] (truncation < 2e-9), float arithmetic in this fixed order, so
+// the host restatement gives the same bits
+__host__ __device__ inline void g_sincos(float fr, float &s, float &c) {
+  const float qf = floorf(fr * 4.f + .5f);
+  const float t = fr - qf * .25f;
+  const float x = t * 6.28318530717958647692f;
+  const float x2 = x * x;
+  const float sp = x + x * x2 * (-1.f / 6.f + x2 * (1.f / 120.f + x2 * (-1.f / 5040.f + x2 * (1.f / 362880.f))));
+  const float cp =
+      1.f + x2 * (-.5f + x2 * (1.f / 24.f + x2 * (-1.f / 720.f + x2 * (1.f / 40320.f + x2 * (-1.f / 3628800.f)))));
+  const int q = int(qf) & 3;
+  s = q == 0 ? sp : (q == 1 ? cp : (q == 2 ? -sp : -cp));
+  c = q == 0 ? cp : (q == 1 ? -sp : (q == 2 ? -cp : sp));
+}
+
 // Feature f of the encoding: f = 24 d + 2 j + c (f < 144) is sin (c = 0) or
 // cos (c = 1) of π 2^j x_d; f = 144 + d is x_d itself; 150..159 are zero.
 // Any 8-aligned block of features holds one input and four frequencies.
@@ -71,10 +88,11 @@ __device__ inline float g_feature(int f, const float *in) {
   if (f >= kGFeat) return 0.f;
   if (f >= 144) return in[f - 144];
   const int d = f / 24, j = (f % 24) >> 1;
-  // π 2^j x = 2π (2^(j-1) x): v_sin/v_cos take revolutions
+  // π 2^j x = 2π (2^(j-1) x)
   const float x = in[d] * float(1 << j) * .5f;
-  const float fr = x - floorf(x);
-  return (f & 1) ? __builtin_amdgcn_cosf(fr) : __builtin_amdgcn_sinf(fr);
+  float sv, cv;
+  g_sincos(x - floorf(x), sv, cv);
+  return (f & 1) ? cv : sv;
 }
 
 __device__ inline uint32_t pack_bf2(float a, float b) { return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16); }
@@ -112,9 +130,7 @@ __device__ inline uint4 feature_chunk(int c, uint32_t g, const GIn &q) {
     float x = gin_at(q, d) * float(1 << j0) * .5f;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const float fr = x - floorf(x);
-      v[2 * k] = __builtin_amdgcn_sinf(fr);
-      v[2 * k + 1] = __builtin_amdgcn_cosf(fr);
+      g_sincos(x - floorf(x), v[2 * k], v[2 * k + 1]);
       x *= 2.f;
     }
   } else if (f0 == 144) {

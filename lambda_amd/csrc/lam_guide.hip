@@ -57,14 +57,12 @@ __global__ void k_guide_pack(const float *__restrict__ th, uint4 *__restrict__ f
     }
     bias[tid] = b;
   }
-  // bf16-rounded weights (biases stay fp32) for the training forward/backward
-  if (tid < uint32_t(kGParams)) {
-    const bool is_bias = (tid >= uint32_t(kGB1) && tid < uint32_t(kGW2)) ||
-                         (tid >= uint32_t(kGB2) && tid < uint32_t(kGW3)) ||
-                         (tid >= uint32_t(kGB3) && tid < uint32_t(kGW4)) ||
-                         (tid >= uint32_t(kGB4) && tid < uint32_t(kGW5)) || tid >= uint32_t(kGB5);
-    thq[tid] = is_bias ? th[tid] : bfround(th[tid]);
-  }
+  // bf16-rounded copy of every parameter for the training passes, which take
+  // the weights from it and the (fp32) biases from th.  (A per-range select
+  // between th and its rounding here was compiled for gfx950 with the store
+  // address of the bias ranges clobbered to the first lane's; a plain copy
+  // has no such select.)
+  if (tid < uint32_t(kGParams)) thq[tid] = bfround(th[tid]);
 }
 
 // ---- batched inference (test hook) ------------------------------------------------------
@@ -334,13 +332,19 @@ static void gemm(uint32_t M, uint32_t N, uint32_t K, const float *A, uint32_t ld
                      ldb, C, ldc, kchunk, split ? 1 : 0);
 }
 
+const float *guide_train_raw(const float *ws, uint32_t B) {
+  return ws + size_t(B) * (kGFeatPad + 8 * kGHid);
+}
+
 size_t guide_train_bytes(uint32_t B) {
   return size_t(B) * (kGFeatPad + 9 * kGHid + 2 * kGOut + 4) * 4 + 64;
 }
 
 // One forward + backward over a prepared batch (X0, tgt in ws): grad
-// (kGParams, zeroed here) and scal[1] (loss) are written.
-static void guide_fwd_bwd(const float *thq, float *ws, uint32_t B, float *grad, float *scal, hipStream_t st) {
+// (kGParams, zeroed here) and scal[1] (loss) are written.  Weights come from
+// thq (bf16-rounded), biases from th (fp32), as in the MFMA network.
+static void guide_fwd_bwd(const float *th, const float *thq, float *ws, uint32_t B, float *grad, float *scal,
+                          hipStream_t st) {
   float *X0 = ws;
   float *Z[5], *X[5];
   float *p = X0 + size_t(B) * kGFeatPad;
@@ -359,10 +363,10 @@ static void guide_fwd_bwd(const float *thq, float *ws, uint32_t B, float *grad, 
   for (int l = 0; l < 5; l++) {
     const float *Xin = l == 0 ? X0 : X[l - 1];
     gemm<false, true>(B, Out[l], In[l], Xin, ldx[l], thq + Wo[l], In[l], Z[l], Out[l], false, st);
-    hipLaunchKernelGGL(k_bias_act, dim3(cdiv(size_t(B) * Out[l], 256)), dim3(256), 0, st, Z[l], thq + Bo[l], B,
+    hipLaunchKernelGGL(k_bias_act, dim3(cdiv(size_t(B) * Out[l], 256)), dim3(256), 0, st, Z[l], th + Bo[l], B,
                        Out[l], l < 4 ? X[l] : nullptr, l < 4 ? 1 : 0);
   }
-  (void)launch_zero(scal, 8, st);
+  (void)launch_zero(scal, 16, st);  // whole 16-byte words
   (void)launch_zero(grad, size_t(kGParams) * 4, st);
   hipLaunchKernelGGL(k_sum_w, dim3(std::min(cdiv(B, 256), 64u)), dim3(256), 0, st, tgt, B, scal);
   hipLaunchKernelGGL(k_guide_loss, dim3(cdiv(B, 256)), dim3(256), 0, st, Z[4], tgt, B, scal, dZo);
@@ -399,11 +403,11 @@ hipError_t guide_infer(const void *pack, const float *in, uint32_t n, float *raw
   return hipGetLastError();
 }
 
-hipError_t guide_grad_rows(const float *thq, const float *rows, uint32_t B, float *ws, float *grad, float *scal,
-                           hipStream_t st) {
+hipError_t guide_grad_rows(const float *th, const float *thq, const float *rows, uint32_t B, float *ws,
+                           float *grad, float *scal, hipStream_t st) {
   float4 *tgt = reinterpret_cast<float4 *>(ws + size_t(B) * (kGFeatPad + 8 * kGHid + 2 * kGOut + kGHid));
   hipLaunchKernelGGL(k_guide_feat_rows, dim3(cdiv(B, 256)), dim3(256), 0, st, rows, B, ws, tgt);
-  guide_fwd_bwd(thq, ws, B, grad, scal, st);
+  guide_fwd_bwd(th, thq, ws, B, grad, scal, st);
   return hipGetLastError();
 }
 
@@ -415,7 +419,7 @@ hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t
     gt.step++;
     hipLaunchKernelGGL(k_guide_gather, dim3(cdiv(B, 256)), dim3(256), 0, st, rec, rec_count, rec_cap, B,
                        gt.seed ^ (uint64_t(gt.step) << 20), gt.ws, tgt);
-    guide_fwd_bwd(gt.thq, gt.ws, B, gt.grad, gt.scal, st);
+    guide_fwd_bwd(gt.th, gt.thq, gt.ws, B, gt.grad, gt.scal, st);
     const float c1 = 1.f - powf(gt.b1, float(gt.step)), c2 = 1.f - powf(gt.b2, float(gt.step));
     hipLaunchKernelGGL(k_adam, dim3(cdiv(kGParams, 256)), dim3(256), 0, st, gt.th, gt.m, gt.v, gt.grad,
                        uint32_t(kGParams), gt.lr, gt.b1, gt.b2, gt.eps, c1, c2);
@@ -427,7 +431,7 @@ hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t
 hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t B, uint32_t steps, hipStream_t st) {
   for (uint32_t s = 0; s < steps; s++) {
     gt.step++;
-    (void)guide_grad_rows(gt.thq, rows, B, gt.ws, gt.grad, gt.scal, st);
+    (void)guide_grad_rows(gt.th, gt.thq, rows, B, gt.ws, gt.grad, gt.scal, st);
     const float c1 = 1.f - powf(gt.b1, float(gt.step)), c2 = 1.f - powf(gt.b2, float(gt.step));
     hipLaunchKernelGGL(k_adam, dim3(cdiv(kGParams, 256)), dim3(256), 0, st, gt.th, gt.m, gt.v, gt.grad,
                        uint32_t(kGParams), gt.lr, gt.b1, gt.b2, gt.eps, c1, c2);

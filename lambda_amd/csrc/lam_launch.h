@@ -23,6 +23,7 @@ hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, 
                               float *film, uint32_t *err, hipStream_t st);
 hipError_t launch_resolve(const DevScene &sc, const float *film, uint32_t w, uint32_t h, float inv_n, float *rgb,
                           float *xyz, hipStream_t st);
+// zeroes bytes / 16 float4 words at p (16-byte aligned; a tail below 16 bytes is left)
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 hipError_t launch_math(int fn, const float *a, const float *b, size_t n, float *out, hipStream_t st);
 
@@ -80,10 +81,12 @@ struct GuideTrain {
   uint64_t seed;
 };
 size_t guide_train_bytes(uint32_t batch);
+// the training forward's raw outputs (batch x 40) inside the workspace
+const float *guide_train_raw(const float *ws, uint32_t batch);
 hipError_t guide_pack(const float *th, void *pack, float *thq, hipStream_t st);
 hipError_t guide_infer(const void *pack, const float *in, uint32_t n, float *raw, uint32_t blocks, hipStream_t st);
-hipError_t guide_grad_rows(const float *thq, const float *rows, uint32_t batch, float *ws, float *grad, float *scal,
-                           hipStream_t st);
+hipError_t guide_grad_rows(const float *th, const float *thq, const float *rows, uint32_t batch, float *ws,
+                           float *grad, float *scal, hipStream_t st);
 hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t batch, uint32_t steps, hipStream_t st);
 hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t *rec_count, uint32_t rec_cap,
                                uint32_t steps, hipStream_t st);

@@ -7,8 +7,9 @@ its BSDF samplers only, src/renderer.cpp:116-145); this file restates the
 library's own definition:
 
   inputs   6 values in [0,1]
-  features f < 144: sin (f even) / cos (f odd) of π 2^j x_d, f = 24 d + 2 j + c;
-           f = 144 + d: x_d; rounded to bf16
+  features f < 144: sin (f even) / cos (f odd) of π 2^j x_d, f = 24 d + 2 j + c,
+           evaluated as the device does (sincos_rev); f = 144 + d: x_d;
+           rounded to bf16
   layers   150 -> 64 -> 64 -> 64 -> 64 (ReLU, activations rounded to bf16)
            -> 40; weights rounded to bf16, biases fp32, sums exact (float64)
   outputs  per lobe L: raw[5L:5L+3] mean direction (normalised), raw[5L+3]
@@ -52,6 +53,33 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def sincos_rev(fr):
+    """sin and cos of 2π fr, fr in [0,1), in float32 with the library's
+    reduction (nearest quarter turn) and Taylor polynomials, each operation
+    rounded as the device rounds it (no fused multiply-adds)"""
+    f = np.float32
+    fr = np.asarray(fr, dtype=f)
+    qf = np.floor(fr * f(4) + f(0.5))
+    t = fr - qf * f(0.25)
+    x = t * f(6.28318530717958647692)
+    x2 = x * x
+    ps = f(1) / f(362880)
+    ps = f(-1) / f(5040) + x2 * ps
+    ps = f(1) / f(120) + x2 * ps
+    ps = f(-1) / f(6) + x2 * ps
+    sp = x + x * x2 * ps
+    pc = f(-1) / f(3628800)
+    pc = f(1) / f(40320) + x2 * pc
+    pc = f(-1) / f(720) + x2 * pc
+    pc = f(1) / f(24) + x2 * pc
+    pc = f(-0.5) + x2 * pc
+    cp = f(1) + x2 * pc
+    q = qf.astype(np.int64) & 3
+    s = np.select([q == 0, q == 1, q == 2], [sp, cp, -sp], -cp)
+    c = np.select([q == 0, q == 1, q == 2], [cp, -sp, -cp], sp)
+    return s.astype(f), c.astype(f)
+
+
 def features(inp):
     """encoded, bf16-rounded features (n x 150)"""
     x = np.asarray(inp, dtype=np.float32).reshape(-1, N_IN)
@@ -59,8 +87,8 @@ def features(inp):
     for f in range(144):
         d, j = f // 24, (f % 24) // 2
         r = x[:, d] * np.float32(2.0 ** j) * np.float32(0.5)
-        fr = (r - np.floor(r)).astype(np.float64)
-        cols.append(np.cos(2 * np.pi * fr) if f % 2 else np.sin(2 * np.pi * fr))
+        s, c = sincos_rev(r - np.floor(r))
+        cols.append((c if f % 2 else s).astype(np.float64))
     for d in range(N_IN):
         cols.append(x[:, d].astype(np.float64))
     return bf16(np.stack(cols, axis=1))

@@ -159,8 +159,14 @@ def test_adam_first_step(ctx):
     lr = 1e-3
     ctx.guide_train(rows, 1, lr)
     th1 = ctx.guide_get_params().astype(np.float64)
-    want = th - lr * g / (np.abs(g) + 1e-8)
-    assert np.allclose(th1, want, rtol=0, atol=1e-6)
+    step = th1 - th
+    # where |g| >> ε the step is -lr sign(g); the gradient is recomputed
+    # inside the step (float atomics in another order), so near-zero
+    # components are only bounded
+    big = np.abs(g) > 1e-5
+    assert big.mean() > 0.5
+    assert np.allclose(step[big], -lr * np.sign(g[big]), rtol=0, atol=1e-7 + 1e-4 * lr)
+    assert np.all(np.abs(step) <= lr * (1 + 1e-5) + 1e-7)
 
 
 @pytest.mark.gpu
