@@ -192,6 +192,12 @@ typedef struct {
   uint64_t extend_launches, shadow_launches, shade_launches;
   uint64_t extend_rays, shadow_rays;
   double render_ms; /* wall time of the last lam_render on the stream */
+  /* lam_render_guided: training of the last call */
+  uint64_t guide_records;  /* training records written (all training passes) */
+  double guide_train_ms;   /* training passes + Adam steps, on the stream */
+  uint64_t guide_dropped;  /* continuations dropped: weight not finite */
+  uint64_t guide_zero_pdf; /* guide samples whose mixture density was zero */
+  double guide_loss_first, guide_loss_last; /* training loss, first and last step */
 } lam_timing;
 
 /* ---- context ---------------------------------------------------------- */
@@ -270,6 +276,8 @@ typedef struct {
   uint32_t batch;          /* training records per step, multiple of 256, <= 65536 */
   float lr;                /* Adam step size */
   uint64_t seed;           /* network initialisation (if none yet) and minibatches */
+  uint32_t materials;      /* material types guided: bit (1 << LAM_MAT_*); 0 = all supported */
+  uint32_t pad;
 } lam_guide_params;
 
 /* lam_render with the learned proposal.  The network persists in the

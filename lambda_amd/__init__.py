@@ -109,12 +109,15 @@ class DiffParamsC(C.Structure):
 
 class GuideParams(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train_passes", C.c_uint32), ("steps_per_pass", C.c_uint32),
-                ("batch", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint64)]
+                ("batch", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint64), ("materials", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
-def guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1):
-    """lam_guide_params: guide-branch probability and online-training schedule."""
-    return GuideParams(float(alpha), int(train_passes), int(steps_per_pass), int(batch), float(lr), int(seed))
+def guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1, materials=0):
+    """lam_guide_params: guide-branch probability, online-training schedule,
+    guided material types (bit 1 << MAT_*; 0 = every supported type)."""
+    return GuideParams(float(alpha), int(train_passes), int(steps_per_pass), int(batch), float(lr), int(seed),
+                       int(materials), 0)
 
 
 GUIDE_IN, GUIDE_OUT, GUIDE_LOBES = 6, 40, 8
@@ -137,7 +140,10 @@ class Timing(C.Structure):
                 ("raygen_ms", C.c_double), ("film_ms", C.c_double), ("other_ms", C.c_double),
                 ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
                 ("shade_launches", C.c_uint64), ("extend_rays", C.c_uint64),
-                ("shadow_rays", C.c_uint64), ("render_ms", C.c_double)]
+                ("shadow_rays", C.c_uint64), ("render_ms", C.c_double),
+                ("guide_records", C.c_uint64), ("guide_train_ms", C.c_double),
+                ("guide_dropped", C.c_uint64), ("guide_zero_pdf", C.c_uint64),
+                ("guide_loss_first", C.c_double), ("guide_loss_last", C.c_double)]
 
 
 def params(spp=4, max_scatterings=4, sample_p=0.5, hero=16, seed=0, spp_offset=0, flags=0):

@@ -324,8 +324,8 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   const lam::PathArgs pa = path_args(p);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
-  const lam::WaveLaunch wl = launch_for(ctx, p);
-  const uint32_t iters = lam::wave_iterations(p->max_scatterings, wl);
+  const lam::WaveLaunch wl_all = launch_for(ctx, p);
+  const uint32_t iters = lam::wave_iterations(p->max_scatterings, wl_all);
   lam::WaveProfile prof{};
   if (profile) {
     const size_t need = 4 + 5 * size_t(iters);
@@ -343,23 +343,40 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   }
   for (size_t off = 0; off < npix; off += cap) {
     const uint32_t n = uint32_t(std::min(cap, npix - off));
+    // the finish kernel takes over below 1/16 of the chunk's paths (and at
+    // most 131,072): guided passes then shade most vertices with the
+    // learned proposal, and small frames keep a full wavefront longer
+    lam::WaveLaunch wl = wl_all;
+    wl.tail_threshold = std::min(wl.tail_threshold, n / 16);
     uint32_t first = 0;  // passes of this chunk already traced
     if (gr && off == 0) {
       // training passes: trace with records, weight them, then Adam steps
       const lam_guide_params *gp = gr->gp;
       const uint32_t tp = std::min(gp->train_passes, p->spp);
       lam::GuideTrain gt = guide_train_state(ctx, gp->batch, gp->lr, gp->seed);
+      ctx->timing.guide_records = 0;
+      const auto t0 = std::chrono::steady_clock::now();
       for (; first < tp; first++) {
-        HIPC(lam::launch_zero(gr->gd_train.rec_count, 4, ctx->stream));
+        HIPC(lam::launch_zero(gr->gd_train.rec_count, 16, ctx->stream));
         HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + first, ctx->stream));
         HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
                                film, nullptr, wl, ctx->stream, nullptr, &gr->gd_train, true));
         HIPC(lam::guide_weights(int(p->hero), ctx->ws, gr->gd_train, ctx->stream));
         HIPC(lam::guide_train_records(gt, gr->gd_train.rec, gr->gd_train.rec_count, gr->gd_train.rec_cap,
                                       gp->steps_per_pass, ctx->stream));
+        uint32_t cnt = 0;
+        float sc[2] = {0.f, 0.f};
+        HIPC(hipMemcpyAsync(&cnt, gr->gd_train.rec_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipStreamSynchronize(ctx->stream));
+        ctx->timing.guide_records += std::min(cnt, gr->gd_train.rec_cap);
+        if (first == 0) ctx->timing.guide_loss_first = sc[1];
+        ctx->timing.guide_loss_last = sc[1];
       }
       ctx->g_step = gt.step;
       HIPC(hipStreamSynchronize(ctx->stream));
+      ctx->timing.guide_train_ms =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     const lam::GuideDev *gd = gr ? &gr->gd_render : nullptr;
     if (first >= p->spp) continue;
@@ -959,6 +976,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   gd.frag = static_cast<const uint4 *>(ctx->g_pack);
   gd.mix = static_cast<uint4 *>(ctx->g_mix);
   gd.alpha = g->alpha;
+  gd.mat_mask = g->materials ? g->materials
+                             : (1u << LAM_MAT_LAMBERTIAN) | (1u << LAM_MAT_BLINN) | (1u << LAM_MAT_THINFILM);
   gd.lo = lam::f3{ctx->box_lo[0], ctx->box_lo[1], ctx->box_lo[2]};
   gd.inv_ext = lam::f3{1.f / (ctx->box_hi[0] - ctx->box_lo[0]), 1.f / (ctx->box_hi[1] - ctx->box_lo[1]),
                        1.f / (ctx->box_hi[2] - ctx->box_lo[2])};
@@ -976,6 +995,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   if ((rc = render_wavefront(ctx, p, film, npix, &gr))) return rc;
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  uint32_t gc[2] = {0, 0};
+  HIPC(hipMemcpyAsync(gc, ctx->ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
@@ -983,6 +1004,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  ctx->timing.guide_dropped = gc[0];
+  ctx->timing.guide_zero_pdf = gc[1];
   if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }

@@ -2,8 +2,8 @@
 // direction, hero wavelength) to a mixture of 8 von Mises-Fisher lobes over
 // the continuation direction.  It is evaluated with bf16 MFMA
 // (v_mfma_f32_16x16x32_bf16, fp32 accumulate) one wave at a time, and mixed
-// with the reference's BSDF sampler under one-sample MIS so the estimator
-// keeps the expectation of the plain render loop.
+// with the cosine sampling of diffuse lobes under one-sample MIS, so the
+// estimator keeps the expectation of the plain render loop.
 //
 // Network: 6 inputs in [0,1] -> frequency encoding (sin/cos(π 2^j x) for
 // j = 0..11 and the raw inputs: 150 features, padded to 160) -> 64 -> 64 ->
@@ -288,120 +288,20 @@ __device__ inline f3 mix_sample(const GuideMix &m, float ul, float u1, float u2)
   return vmf_sample(mu, k, u1, u2);
 }
 
-// ---- the plain continuation as a density over directions ---------------------------------
-// Where guiding applies: the reference's Lambertian sampler, the Blinn
-// sampler when it never transmits (w_t = 0, so fresnelSplit gives p_r = 1)
-// and the thin-film sampler where it continues at all (front side, o·n >= 0).
-__device__ inline bool guide_supported(const lam_material &m, const Surf &s, f3 o) {
-  if (m.type == LAM_MAT_LAMBERTIAN) return true;
-  if (m.type == LAM_MAT_BLINN) return m.w_t == 0.f;
-  if (m.type == LAM_MAT_THINFILM) return s.front && !(dot(o, s.F.z) < 0.f);
-  return false;
+// ---- where the proposal applies ----------------------------------------------------------
+// The learned proposal stands in for the cosine sampling of a diffuse lobe:
+// all of a Lambertian material (cg.hpp:1755-1778) and the Kd lobe of a Blinn
+// material (cg.hpp:1853-1861).  Glossy and thin-film lobes keep their own
+// samplers.
+__device__ inline bool guide_supported(const lam_material &m) {
+  return m.type == LAM_MAT_LAMBERTIAN || (m.type == LAM_MAT_BLINN && m.p_d > 0.f);
 }
 
-// q(ω): density with which sample_material proposes ω (roulette included).
-// G(ω) = Σ_lobes mult_l(ω) w_l(ω) q_l(ω) / prob_l(ω): the plain continuation
-// (mult_l / prob_l · w_l for lobe l's sample) has expectation ∫ G(ω) L_i(ω) dω,
-// where prob_l is the pdf value the reference computes for lobe l (not always
-// the true density: the Blinn and thin-film half-vector pdfs use
-// (n·h)^(α+1)/4 for (n·h)^α / (4 o·h)) and w_l its MIS weight against the
-// light, pL = that light's pdf for ω.  G is written only when want_G.
-template <int H>
-__device__ void guide_eval(const DevScene &sc, const lam_material &m, const Lam<H> &L, const Surf &s, f3 o, f3 wi,
-                           float p, float pL, bool want_G, float *G, float &q) {
-  q = 0.f;
-  if (want_G) {
-#pragma unroll
-    for (int k = 0; k < H; k++) G[k] = 0.f;
-  }
-  const f3 ns = s.n();
-  const float pL2 = pL * pL;
-  if (m.type == LAM_MAT_LAMBERTIAN) {
-    const float c = dot(wi, ns);
-    if (!(c > 0.f)) return;
-    const float pi = p * c * kInvPi;
-    q = pi;
-    if (want_G) {
-      float rho[H];
-      spec_eval<H>(sc, m.spec[0], L, rho);
-      const float w = pi * pi / (pi * pi + pL2);
-#pragma unroll
-      for (int k = 0; k < H; k++) G[k] = rho[k] * kInvPi * c * w;
-    }
-    return;
-  }
-  // reflection half vector on the sampled side (h·ns >= 0)
-  const f3 hv = add(o, wi);
-  const float l2 = len2(hv);
-  f3 h = mk(0.f, 0.f, 1.f);
-  const bool has_h = l2 > 1e-12f;
-  if (has_h) {
-    h = mul(hv, 1.f / sqrtf(l2));
-    if (dot(h, ns) < 0.f) h = neg(h);
-  }
-  if (m.type == LAM_MAT_BLINN) {
-    // lobe choice by xl in [0, p): glossy below p p_spec, diffuse below
-    // p (p_spec + p_d); sel_* are the true selection probabilities
-    const float pp_spec = p * m.p_spec, pp_d = p * m.p_d;
-    const float sel_s = fminf(p, pp_spec), sel_d = fmaxf(0.f, fminf(p, pp_spec + pp_d) - sel_s);
-    const float c = dot(wi, ns);
-    if (c > 0.f && pp_d > 0.f) {  // diffuse lobe
-      const float pi = pp_d * (c * kInvPi), qd = sel_d * (c * kInvPi);
-      q += qd;
-      if (want_G) {
-        float kd[H];
-        spec_eval<H>(sc, m.spec[0], L, kd);
-        const float w = pi * pi / (pi * pi + pL2) * (qd / pi);
-#pragma unroll
-        for (int k = 0; k < H; k++) G[k] += kd[k] * kInvPi * c * w;
-      }
-    }
-    if (has_h && sel_s > 0.f) {  // glossy reflection lobe
-      const float wz = dot(h, s.F.z);  // the sampler's local z (sign flipped on the back)
-      const float aw = fabsf(wz), ci = dot(o, h);
-      if (aw > 0.f && ci != 0.f) {
-        const float qr = sel_s * (m.alpha + 1) * .5f * kInvPi * lam_powf(aw, m.alpha) * .25f / fabsf(ci);
-        q += qr;
-        if (want_G) {
-          const float pr = pp_spec * blinnh_prob(m.alpha, wz) * .25f;
-          float ior[H], ks[H], kt[H];
-          spec_eval<H>(sc, m.spec[4], L, ior);
-          spec_eval<H>(sc, m.spec[1], L, ks);
-          spec_eval<H>(sc, m.spec[2], L, kt);
-          const float pw5 = lam_powf(1.f - ci, 5.f);
-          const float pw = lam_powf(wz, m.alpha);
-          const float w = pr * pr / (pr * pr + pL2);
-          const float ratio = pr > 0.f ? w * qr / pr : 0.f;
-#pragma unroll
-          for (int k = 0; k < H; k++) {
-            const float eta = s.front ? 1.f / ior[k] : ior[k];
-            const float ct2 = 1.f - eta * eta * (1.f - ci * ci);
-            const float f0 = F0s(eta);
-            const float F = ct2 < 0.f ? 1.f : f0 + (1.f - f0) * pw5;
-            G[k] += (ks[k] + F * kt[k]) * (m.alpha + 2) * .125f * kInvPi * pw * ratio;
-          }
-        }
-      }
-    }
-    return;
-  }
-  if (m.type == LAM_MAT_THINFILM) {
-    if (!has_h) return;
-    const f3 n = s.F.z;
-    const float nh = dot(n, h), oh = dot(o, h);
-    if (!(nh > 0.f) || oh == 0.f) return;
-    const float qf = p * (m.alpha + 1) * .5f * kInvPi * lam_powf(nh, m.alpha) * .25f / fabsf(oh);
-    q = qf;
-    if (want_G) {
-      const float pf = p * lam_powf(nh, m.alpha + 1) * (m.alpha + 1) * .5f * kInvPi * .25f;
-      float f[H];
-      thinfilm_f<H>(sc, m, L, wi, o, n, s.front, f);
-      const float w = pf * pf / (pf * pf + pL2);
-      const float ratio = pf > 0.f ? w * qf / pf : 0.f;
-#pragma unroll
-      for (int k = 0; k < H; k++) G[k] = f[k] * ratio;
-    }
-  }
+// power-heuristic weight a² / (a² + b²) of a strategy with pdf a against one
+// with pdf b; 1 where b = 0 (a pdf that underflowed to 0 leaves 0/0 otherwise)
+__device__ inline float mis_w(float a, float b2) {
+  const float a2 = a * a;
+  return b2 > 0.f ? a2 / (a2 + b2) : 1.f;
 }
 
 // network inputs of a vertex, each in [0,1]: position in the scene box,

@@ -66,6 +66,7 @@ struct GuideDev {
   const uint4 *frag;      // packed bf16 fragments + fp32 biases
   uint4 *mix;             // network outputs per extend-queue entry: 40 fp16 (5 uint4)
   float alpha;            // probability of the guide branch at supported vertices
+  uint32_t mat_mask;      // guided material types (bit 1 << type)
   f3 lo, inv_ext;         // scene box
   float4 *rec;            // training records (4 float4 each); null when not training
   uint32_t *rec_count;    // records appended (may exceed rec_cap; the excess is dropped)

@@ -39,6 +39,8 @@ enum : uint32_t {
   CTRL_TAIL = 5,    // 1: the current queue is finished by k_finish
   CTRL_FETCH_FIN = 6,
   CTRL_PASS = 8,    // current pass (sample) index
+  CTRL_GUIDE_DROP = 12,  // guided continuations dropped (weight not finite)
+  CTRL_GUIDE_ZERO = 13,  // guide-branch samples with a zero mixture density
   CTRL_WORDS = 16
 };
 
@@ -282,18 +284,18 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds
 }
 
 // ---- guided shade ----------------------------------------------------------------------
-// k_shade with the learned proposal.  At a vertex whose material supports it
-// the continuation is drawn from the vMF mixture with probability α
-// (dimension GUIDE) and from the reference's sampler otherwise; both branches
-// weight it by one-sample MIS over the mixture
-//   M(ω) = (1-α) q(ω) + α P_s p_g(ω),
-// q = the sampler's true density, P_s = its survival probability:
-//   guide branch:   β ← β ⊙ G(ω) / M(ω)        (survives with probability P_s)
-//   sampler branch: β ← β ⊙ mult/prob w_mat · q(ω) / M(ω)
-// with G the plain continuation's integrand (guide_eval), so the expectation
-// of every vertex is that of k_shade.  The NEE term, its MIS weight and its
-// dependence on survival are unchanged.  α = 0 reproduces k_shade.  The
-// mixture of each vertex comes from k_guide_queue.
+// k_shade with the learned proposal.  Where the reference's sampler picks a
+// diffuse lobe (a Lambertian material, or the Kd lobe of a Blinn material:
+// the same lobe choice from dimension LOBE), the direction is drawn from the
+// vMF mixture with probability α (dimension GUIDE) and from the lobe's
+// cosine distribution otherwise, and both are weighted by one-sample MIS
+// over the mixture of the two:
+//   β ← β ⊙ f_d(ω) w_d(ω) / (pp_d M(ω)),  M = (1-α) cos/π + α p_g,
+// pp_d = the lobe's selection probability, f_d its BSDF·cos and w_d the
+// reference's MIS weight against the light (with its pdf pp_d cos/π).  The
+// expectation of the lobe's term is the plain one: pp_d E_M[f_d w_d L/(pp_d M)]
+// = ∫ f_d w_d L.  Other lobes, the NEE term and its weight, and roulette
+// are unchanged; α = 0 reproduces k_shade.
 // TRAIN: each guided continuation appends a training record (network inputs,
 // ω, Σβ, ΣL before the vertex, ΣNEE term, slot; k_shadow marks a visible
 // NEE term).
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
             ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
         } else if (s <= pa.max_scat) {
           live = true;
-          guided = gd.alpha > 0.f && guide_supported(mat, sf, o);
+          guided = gd.alpha > 0.f && ((gd.mat_mask >> mat.type) & 1u) && guide_supported(mat);
           if (TRAIN && guided) guide_inputs<H>(gd, sc, add(ro, mul(rd, hit.t)), o, lam, gin);
         }
       }
@@ -448,34 +450,55 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
       for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
       const float alpha = guided ? gd.alpha : 0.f;
       const float P = pa.sample_p;
-      // survival probability of the reference's sampler at this material
-      const float Ps = mat.type == LAM_MAT_BLINN ? fminf(P, P * (mat.p_spec + mat.p_d)) : P;
-      const uint4 *mixq = gd.mix + 5 * size_t(q);
+      const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
+      // does the reference's sampler take the diffuse lobe (cg.hpp:1768, 1853)?
+      bool diffuse = false;
+      float pp_d = 0.f;
+      if (alpha > 0.f) {
+        if (mat.type == LAM_MAT_LAMBERTIAN) {
+          diffuse = !(xl > P);
+          pp_d = P;
+        } else {
+          const float pp_spec = P * mat.p_spec;
+          pp_d = P * mat.p_d;
+          diffuse = xl < P && !(xl < pp_spec) && xl < pp_spec + pp_d;
+        }
+      }
       bool ok = false;
       f3 wi = mk(0.f, 0.f, 0.f);
       float fac[H];
-      const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
-      if (lam_draw_f(key, lam_vdim(s, LAM_VDIM_GUIDE)) < alpha) {
-        if (xl < Ps) {
-          ok = true;
-          float pg;
-          {
-            GuideMix gm;
-            float raw[kGOut];
-            load_mix(mixq, raw);
-            mix_from_raw(raw, gm);
-            wi = mix_sample(gm, lam_draw_f(key, lam_vdim(s, LAM_VDIM_SPARE)),
-                            lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)),
-                            lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)));
-            pg = mix_pdf(gm, wi);
-          }
+      if (diffuse) {
+        ok = true;
+        const float xa = lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), xb = lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B));
+        GuideMix gm;
+        {
+          float raw[kGOut];
+          load_mix(gd.mix + 5 * size_t(q), raw);
+          mix_from_raw(raw, gm);
+        }
+        if (lam_draw_f(key, lam_vdim(s, LAM_VDIM_GUIDE)) < alpha) {
+          wi = mix_sample(gm, lam_draw_f(key, lam_vdim(s, LAM_VDIM_SPARE)), xa, xb);
+        } else {  // the lobe's own cosine sample (cg.hpp:1762-1770)
+          f3 dir = cos_hemi(xa, xb);
+          if (!sf.front) dir.z *= -1;
+          wi = to_basis(sf.F, dir);
+        }
+        const float c = dot(wi, nn);
+        const float pg = mix_pdf(gm, wi);
+        const float M = (1.f - alpha) * fmax_ref(c, 0.f) * kInvPi + alpha * pg;
+        if (c > 0.f && M > 0.f) {
+          float kd[H];
+          spec_eval<H>(sc, mat.spec[0], lam, kd);
+          const float pi = pp_d * (c * kInvPi);
           const float pLw = prob_light(sc, lt, sf.p, wi) * pl;
-          float G[H], qv;
-          guide_eval<H>(sc, mat, lam, sf, o, wi, P, pLw, true, G, qv);
-          const float M = (1.f - alpha) * qv + alpha * Ps * pg;
-          const float inv = M > 0.f ? 1.f / M : 0.f;
+          const float w_d = mis_w(pi, pLw * pLw);
+          const float g = w_d / (pp_d * M);
 #pragma unroll
-          for (int k = 0; k < H; k++) fac[k] = G[k] * inv;
+          for (int k = 0; k < H; k++) fac[k] = kd[k] * kInvPi * c * g;
+        } else {
+#pragma unroll
+          for (int k = 0; k < H; k++) fac[k] = 0.f;
+          if (!(M > 0.f)) atomicAdd(ctrl + CTRL_GUIDE_ZERO, 1u);
         }
       } else {
         MatSample<H> ms;
@@ -486,22 +509,8 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
           wi = ms.dir;
           const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
           const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
-          float scale = 1.f;
-          if (alpha > 0.f) {
-            float qv, pg;
-            guide_eval<H>(sc, mat, lam, sf, o, wi, P, 0.f, false, nullptr, qv);
-            {
-              GuideMix gm;
-              float raw[kGOut];
-              load_mix(mixq, raw);
-              mix_from_raw(raw, gm);
-              pg = mix_pdf(gm, wi);
-            }
-            const float M = (1.f - alpha) * qv + alpha * Ps * pg;
-            scale = M > 0.f ? qv / M : 1.f / (1.f - alpha);
-          }
 #pragma unroll
-          for (int k = 0; k < H; k++) fac[k] = ms.mult[k] / ms.prob * w_mat * scale;
+          for (int k = 0; k < H; k++) fac[k] = ms.mult[k] / ms.prob * w_mat;
         }
       }
       float pend[H];
@@ -512,21 +521,24 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
         const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], P);
         const float p_L_i = ls.prob * pl;
         const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
-        bool any = false;
+        bool any = false, finite = true;
 #pragma unroll
         for (int k = 0; k < H; k++) {
           pend[k] = beta[k] * (lis[k] * w_L);
           const float nb = beta[k] * fac[k];
           any = any || nb != 0.f;
+          finite = finite && isfinite(fac[k]);
           ws.beta[size_t(k) * cap + p] = nb;
         }
         // a zero-weight continuation carries nothing: drop it (the plain
-        // kernel traces it with β = 0)
-        push_next = any || alpha == 0.f;
+        // kernel traces it with β = 0); so is one whose weight overflowed in a
+        // degenerate configuration (never with α = 0; counted)
+        push_next = (any && finite) || alpha == 0.f;
+        if (!finite && alpha > 0.f) atomicAdd(ctrl + CTRL_GUIDE_DROP, 1u);
         nxt.o = sf.p;
         nxt.d = wi;
       }
-      if (TRAIN) want_rec = guided && push_next;
+      if (TRAIN) want_rec = diffuse && push_next;
       if (TRAIN && want_rec) {
 #pragma unroll
         for (int k = 0; k < H; k++) {
@@ -586,7 +598,8 @@ __global__ void k_guide_weights(WaveState ws, float4 *__restrict__ rec, const ui
 #pragma unroll
   for (int k = 0; k < H; k++) Lf += ws.L[size_t(k) * ws.cap + p];
   const float dl = Lf - b.w - d.y * c.w;
-  rec[4 * size_t(r) + 3].z = b.z > 0.f ? fmaxf(dl, 0.f) / b.z : 0.f;
+  const float w = b.z > 0.f ? fmaxf(dl, 0.f) / b.z : 0.f;
+  rec[4 * size_t(r) + 3].z = isfinite(w) ? w : 0.f;
 }
 
 hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipStream_t st) {
