@@ -24,6 +24,16 @@ Also reported:
                 parameters x (1 +- U[0,0.15]); L2 image loss; forward 64 spp
                 + path-replay adjoint 64 spp, 16 bounces, p=0.8.  grads/s =
                 complete gradients per second at that sample count.
+  guiding       the learned proposal on the glossy interior (200 instanced
+                2,352-tri rounded boxes with Blinn lobes, room shell, one
+                hidden light; 470,424 tris), 1024x1024, 32 bounces, p=0.8:
+                two independent plain renders and two independent guided
+                renders (each with a fresh network trained online on its
+                first 32 passes, training time included) of the same spp.
+                relMSE = mean over pixels of Var(Y) / (Ȳ² + 0.01 mean(Ȳ)²),
+                Y = luminance, Var estimated from the two runs (both
+                estimators are unbiased); time to relMSE 1e-3 =
+                run time x relMSE / 1e-3 (variance falls as 1/spp).
 """
 import argparse
 import ctypes as C
@@ -200,6 +210,59 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
             "grad_finite": bool(np.isfinite(grad).all())}
 
 
+def guiding_bench(spp=128, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5):
+    d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c4_seed{seed}_{width}")
+    path = os.path.join(d, "glossy_interior.nls")
+    if not os.path.exists(path):
+        scenes.glossy_interior(d, seed=seed, width=width)
+    scene = la.load_nls(path)
+    ctx = la.Context(0)
+    ctx.upload(scene)
+
+    def lum(img):
+        return img.astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
+
+    def relmse(a, b):
+        ya, yb = lum(a), lum(b)
+        mu = 0.5 * (ya + yb)
+        # variance of one run from the two: (a - b)^2 / 2
+        return float(np.mean(0.5 * (ya - yb) ** 2 / (mu ** 2 + 0.01 * np.mean(mu) ** 2)))
+
+    pr = lambda s: la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=s)
+    ctx.render(la.params(spp=2, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=99))  # warm-up
+    plain, tp = [], []
+    for s in (1, 2):
+        t0 = time.perf_counter()
+        plain.append(ctx.render(pr(s)))
+        tp.append(time.perf_counter() - t0)
+    guided, tg, tel = [], [], []
+    for s in (1, 2):
+        ctx.guide_reset(s)
+        gp = la.guide_params(alpha=alpha, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=s)
+        t0 = time.perf_counter()
+        guided.append(ctx.render_guided(pr(100 + s), gp))
+        tg.append(time.perf_counter() - t0)
+        tel.append(ctx.timing())
+    rp, rg = relmse(*plain), relmse(*guided)
+    t_p, t_g = float(np.mean(tp)), float(np.mean(tg))
+    ctx.close()
+    samples = scene.width * scene.height * spp
+    return {"workload": f"glossy interior ({scene.n_tris:,} tris: room shell + 200 Blinn rounded boxes, "
+                        f"hidden light), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
+            "spp_per_run": spp, "alpha": alpha,
+            "plain": {"run_s": round(t_p, 3), "samples_per_s": round(samples / t_p, 1), "relmse": round(rp, 5),
+                      "time_to_relmse_1e-3_s": round(t_p * rp / 1e-3, 2)},
+            "guided": {"run_s": round(t_g, 3), "samples_per_s": round(samples / t_g, 1), "relmse": round(rg, 5),
+                       "time_to_relmse_1e-3_s": round(t_g * rg / 1e-3, 2),
+                       "train_ms": round(float(np.mean([t.guide_train_ms for t in tel])), 1),
+                       "train_records": int(np.mean([t.guide_records for t in tel])),
+                       "loss_first": round(float(np.mean([t.guide_loss_first for t in tel])), 4),
+                       "loss_last": round(float(np.mean([t.guide_loss_last for t in tel])), 4)},
+            "speedup_to_relmse_1e-3": round((t_p * rp) / (t_g * rg), 3),
+            "mean_luminance": {"plain": round(float(lum(plain[0] + plain[1]).mean() / 2), 3),
+                               "guided": round(float(lum(guided[0] + guided[1]).mean() / 2), 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-guiding", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -267,6 +331,9 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
         log(f"adjoint: {out['adjoint']}")
+    if not args.no_guiding:
+        out["guiding"] = guiding_bench()
+        log(f"guiding: {out['guiding']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, 1024, bounces, p, hero)
         log(f"cpu: {out['cpu_baseline']}")

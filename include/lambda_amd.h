@@ -260,15 +260,15 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
 /* ---- learned path guiding -------------------------------------------- */
 /* No reference counterpart: a proposal distribution for the continuation
  * direction of tracePath's BSDF sample (src/renderer.cpp:116-145), learned
- * online.  A small MLP (6 inputs: position in the scene box, outgoing
- * direction as two angles, hero wavelength, each in [0,1]; 12-frequency
+ * online.  A small MLP (6 inputs: position in the scene box, shading normal
+ * as two angles, hero wavelength, each in [0,1]; 12-frequency
  * encoding; 4 hidden layers of 64; bf16 weights on MFMA with fp32
  * accumulation) outputs 8 von Mises-Fisher lobes (40 raw values: per lobe
- * an unnormalised mean direction, log κ and a weight logit).  At a vertex
- * whose material's sampler it can stand in for (Lambertian, Blinn without
- * transmission, thin film), the continuation is drawn from the mixture with
- * probability alpha and from the reference's sampler otherwise, weighted by
- * one-sample MIS over both, so the image converges to lam_render's. */
+ * an unnormalised mean direction, log κ and a weight logit).  Where the
+ * reference's sampler picks a diffuse lobe (Lambertian; the Kd lobe of
+ * Blinn), the direction is drawn from the mixture with probability alpha and
+ * from the lobe's cosine distribution otherwise, weighted by one-sample MIS
+ * over both, so the image converges to lam_render's. */
 typedef struct {
   float alpha;             /* guide-branch probability, [0, 1); 0 = lam_render */
   uint32_t train_passes;   /* the first passes train the network online */

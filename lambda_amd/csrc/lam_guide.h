@@ -304,16 +304,18 @@ __device__ inline float mis_w(float a, float b2) {
   return b2 > 0.f ? a2 / (a2 + b2) : 1.f;
 }
 
-// network inputs of a vertex, each in [0,1]: position in the scene box,
-// outgoing direction as (θ/π, φ/2π), hero wavelength on the grid
+// network inputs of a vertex, each in [0,1]: position in the scene box, the
+// shading normal on the visible side as (θ/π, φ/2π) (the guided lobes are
+// diffuse: their target cos·L_i depends on the normal, not on the outgoing
+// direction), hero wavelength on the grid
 template <int H>
-__device__ inline void guide_inputs(const GuideDev &gd, const DevScene &sc, f3 x, f3 o, const Lam<H> &L,
+__device__ inline void guide_inputs(const GuideDev &gd, const DevScene &sc, f3 x, f3 n, const Lam<H> &L,
                                     float *in) {
   in[0] = fminf(fmaxf((x.x - gd.lo.x) * gd.inv_ext.x, 0.f), 1.f);
   in[1] = fminf(fmaxf((x.y - gd.lo.y) * gd.inv_ext.y, 0.f), 1.f);
   in[2] = fminf(fmaxf((x.z - gd.lo.z) * gd.inv_ext.z, 0.f), 1.f);
-  in[3] = acosf(fminf(fmaxf(o.z, -1.f), 1.f)) * kInvPi;
-  in[4] = (atan2f(o.y, o.x) + kPi) * (.5f * kInvPi);
+  in[3] = acosf(fminf(fmaxf(n.z, -1.f), 1.f)) * kInvPi;
+  in[4] = (atan2f(n.y, n.x) + kPi) * (.5f * kInvPi);
   in[5] = (L.l[0] - sc.grid_inf) / (sc.grid_sup - sc.grid_inf);
 }
 

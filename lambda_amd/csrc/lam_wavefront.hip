@@ -300,8 +300,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds
 // ω, Σβ, ΣL before the vertex, ΣNEE term, slot; k_shadow marks a visible
 // NEE term).
 // The network for every entry of the extend queue (after k_extend): inputs
-// from the hit (position ≈ o + t d), the ray and the path's hero wavelength;
-// 40 outputs stored as fp16.  One wave evaluates 64 entries with MFMA, the
+// from the hit's surface (position, shading normal) and the path's hero
+// wavelength; 40 outputs stored as fp16.  One wave evaluates 64 entries with MFMA, the
 // packed parameters staged in LDS once per block.
 template <int H>
 __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
@@ -327,13 +327,13 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
       const Hit hit = ws.hits[q];
       if (hit.obj != LAM_NONE) {
         const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
-        const f3 d = mk(b.x, b.y, b.z);
-        const f3 x = add(mk(a.x, a.y, a.z), mul(d, hit.t));
+        const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
+        const Surf sf = surface(sc, hit, ro, rd);
         uint32_t pixel, sample;
         path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
         Lam<H> lam;
         lam_for<H>(sc, lam_path_key(pa.seed, pixel, sample), lam);
-        guide_inputs<H>(gd, sc, x, neg(normalized(d)), lam, gin);
+        guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
         valid = true;
       }
     }
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
         } else if (s <= pa.max_scat) {
           live = true;
           guided = gd.alpha > 0.f && ((gd.mat_mask >> mat.type) & 1u) && guide_supported(mat);
-          if (TRAIN && guided) guide_inputs<H>(gd, sc, add(ro, mul(rd, hit.t)), o, lam, gin);
+          if (TRAIN && guided) guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
         }
       }
     }
