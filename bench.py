@@ -272,6 +272,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-guiding", action="store_true")
+    ap.add_argument("--legs", default="render,roofline,adjoint,guiding,cpu",
+                    help="comma list of legs to run (profiling runs pick one)")
+    ap.add_argument("--guide-spp", type=int, default=128)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -281,6 +284,23 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(the sharding design is described in DESIGN.md)"}))
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+    if args.no_roofline:
+        legs.discard("roofline")
+    if args.no_adjoint:
+        legs.discard("adjoint")
+    if args.no_guiding:
+        legs.discard("guiding")
+    if "render" not in legs and not args.extend_only:
+        out = {}
+        if "adjoint" in legs:
+            out["adjoint"] = adjoint_bench()
+        if "guiding" in legs:
+            out["guiding"] = guiding_bench(spp=args.guide_spp)
+        print(json.dumps(out), flush=True)
+        return
     bounces, p, hero = 32, 0.8, 4
     t0 = time.perf_counter()
     scene = la.load_nls(scene_path())
@@ -324,17 +344,17 @@ def main():
                    "width": w, "height": h, "spp_per_step": 1, "bounces": bounces, "bounce_prob": p,
                    "triangles": scene.n_tris},
     }
-    if not args.no_roofline:
+    if "roofline" in legs:
         out["roofline"] = extend_roofline(ctx, scene.desc)
         log(f"extend: {out['roofline']}")
     ctx.close()
-    if not args.no_adjoint:
+    if "adjoint" in legs:
         out["adjoint"] = adjoint_bench()
         log(f"adjoint: {out['adjoint']}")
-    if not args.no_guiding:
-        out["guiding"] = guiding_bench()
+    if "guiding" in legs:
+        out["guiding"] = guiding_bench(spp=args.guide_spp)
         log(f"guiding: {out['guiding']}")
-    if not args.no_cpu:
+    if "cpu" in legs:
         out["cpu_baseline"] = cpu_baseline(scene, 1024, bounces, p, hero)
         log(f"cpu: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
