@@ -166,6 +166,7 @@ typedef struct {
 #define LAM_FLAG_NO_GRAPH 0x2u   /* launch the bounce loop without a hipGraph */
 #define LAM_FLAG_PROFILE 0x4u    /* time every kernel with HIP events */
 #define LAM_FLAG_NO_TAIL 0x8u    /* keep short queues in the wavefront (no finish kernel) */
+#define LAM_FLAG_NO_PIPE 0x10u   /* trace passes one after another on one stream (no overlap) */
 
 typedef struct {
   uint32_t spp;             /* Renderer::SPP            (renderer.hpp:36) */

@@ -41,6 +41,7 @@ LAM_FLAG_MEGAKERNEL = 0x1
 LAM_FLAG_NO_GRAPH = 0x2
 LAM_FLAG_PROFILE = 0x4
 LAM_FLAG_NO_TAIL = 0x8
+LAM_FLAG_NO_PIPE = 0x10
 
 OBJ_SPHERE, OBJ_PLANE, OBJ_TRIMESH = 1, 2, 3
 (MAT_NONE, MAT_LAMBERTIAN, MAT_BLINN, MAT_THINFILM, MAT_MICROFACET, MAT_EMITTER,

@@ -30,11 +30,18 @@ struct lam_ctx {
   size_t scratch_size[6] = {0, 0, 0, 0, 0, 0};
   lam_timing timing{};
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // wavefront path state, queues and control block
-  lam::WaveState ws{};
-  int ws_hero = 0;
-  std::vector<void *> ws_bufs;
-  uint32_t *ctrl = nullptr;
+  // wavefront path state, queues and control block of the two pipeline
+  // lanes (lane 0 runs on `stream`; lane 1 only traces when passes overlap)
+  struct Lane {
+    lam::WaveState ws{};
+    int hero = 0;
+    std::vector<void *> bufs;
+    uint32_t *ctrl = nullptr;
+    void *mix = nullptr;  // guiding-network outputs per queue entry
+    size_t mix_cap = 0;
+  } lane[2];
+  hipStream_t stream1 = nullptr;  // lane 1
+  hipEvent_t ev_film[2] = {nullptr, nullptr}, ev_fork = nullptr;
   lam::WaveLaunch wl{0, 0, 0, 0, 0};
   std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
   float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // scene box (guiding inputs)
@@ -46,12 +53,11 @@ struct lam_ctx {
   uint32_t g_step = 0;
   float *g_ws = nullptr;  // training batch workspace
   size_t g_ws_bytes = 0;
-  // guided-render buffers: network outputs per queue entry, records
-  void *g_mix = nullptr;
-  size_t g_mix_cap = 0;
+  // guided-render training records
   float4 *g_rec = nullptr;
   uint32_t g_rec_cap = 0;
   uint32_t *g_rec_count = nullptr, *g_rec_of_slot = nullptr;
+  size_t g_rec_slot_cap = 0;
 };
 
 // Path slots per wavefront pass.  The frame is processed in chunks of this
@@ -136,16 +142,19 @@ int check_params(lam_ctx *ctx, const lam_render_params *p) {
   return LAM_OK;
 }
 
-void free_wave(lam_ctx *ctx) {
-  for (void *p : ctx->ws_bufs) (void)hipFree(p);
-  ctx->ws_bufs.clear();
-  ctx->ws = lam::WaveState{};
-  ctx->ws_hero = 0;
+void free_wave(lam_ctx *ctx, int l) {
+  lam_ctx::Lane &ln = ctx->lane[l];
+  for (void *p : ln.bufs) (void)hipFree(p);
+  ln.bufs.clear();
+  ln.ws = lam::WaveState{};
+  ln.hero = 0;
 }
 
-int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
-  if (!ctx->ctrl) {
-    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->ctrl), lam::kCtrlWords * 4));
+// Launch shape, control block and path state of lane l for chunks of up to
+// cap paths.  Lane 1's stream and events are created on first use.
+int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
+  if (!ctx->lane[0].ctrl) {
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->lane[0].ctrl), lam::kCtrlWords * 4));
     int cus = 0;
     HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     // extend/shadow: 16 KB of LDS stack per 64-lane block, 10 per CU;
@@ -160,32 +169,51 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero) {
     // guiding network: 51 KB of LDS per 256-lane block, 3 blocks per CU
     ctx->wl.guide_blocks = uint32_t(cus > 0 ? cus : 256) * 3u;
   }
-  if (ctx->ws.cap >= cap && ctx->ws_hero >= hero) return LAM_OK;
-  free_wave(ctx);
+  lam_ctx::Lane &ln = ctx->lane[l];
+  if (l == 1 && !ln.ctrl) {
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ln.ctrl), lam::kCtrlWords * 4));
+    HIPC(hipStreamCreateWithFlags(&ctx->stream1, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&ctx->ev_film[0], &ctx->ev_film[1], &ctx->ev_fork})
+      HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
+  if (ln.ws.cap >= cap && ln.hero >= hero) return LAM_OK;
+  free_wave(ctx, l);
   cap = std::max(cap, size_t(64));
   auto alloc = [&](size_t bytes, void **p) -> int {
     HIPC(hipMalloc(p, bytes));
-    ctx->ws_bufs.push_back(*p);
+    ln.bufs.push_back(*p);
     return LAM_OK;
   };
   int rc;
   void *p;
   const size_t hf = size_t(hero) * cap * 4;
   if ((rc = alloc(hf, &p))) return rc;
-  ctx->ws.beta = static_cast<float *>(p);
+  ln.ws.beta = static_cast<float *>(p);
   if ((rc = alloc(hf, &p))) return rc;
-  ctx->ws.L = static_cast<float *>(p);
+  ln.ws.L = static_cast<float *>(p);
   if ((rc = alloc(hf, &p))) return rc;
-  ctx->ws.pend = static_cast<float *>(p);
-  if ((rc = alloc(cap * 32, &ctx->ws.ray[0]))) return rc;
-  if ((rc = alloc(cap * 32, &ctx->ws.ray[1]))) return rc;
+  ln.ws.pend = static_cast<float *>(p);
+  if ((rc = alloc(cap * 32, &ln.ws.ray[0]))) return rc;
+  if ((rc = alloc(cap * 32, &ln.ws.ray[1]))) return rc;
   if ((rc = alloc(cap * 16, &p))) return rc;
-  ctx->ws.hits = static_cast<lam::Hit *>(p);
-  if ((rc = alloc(cap * 32, &ctx->ws.sray))) return rc;
+  ln.ws.hits = static_cast<lam::Hit *>(p);
+  if ((rc = alloc(cap * 32, &ln.ws.sray))) return rc;
   if ((rc = alloc(cap * 4, &p))) return rc;
-  ctx->ws.spath = static_cast<uint32_t *>(p);
-  ctx->ws.cap = cap;
-  ctx->ws_hero = hero;
+  ln.ws.spath = static_cast<uint32_t *>(p);
+  ln.ws.cap = cap;
+  ln.hero = hero;
+  return LAM_OK;
+}
+
+// guiding-network output buffer of lane l (80 B per path slot)
+int ensure_mix(lam_ctx *ctx, int l) {
+  lam_ctx::Lane &ln = ctx->lane[l];
+  if (ln.mix_cap >= ln.ws.cap) return LAM_OK;
+  if (ln.mix) HIPC(hipFree(ln.mix));
+  ln.mix = nullptr;
+  ln.mix_cap = 0;
+  HIPC(hipMalloc(&ln.mix, ln.ws.cap * 80));
+  ln.mix_cap = ln.ws.cap;
   return LAM_OK;
 }
 
@@ -199,26 +227,32 @@ lam::WaveLaunch launch_for(const lam_ctx *ctx, const lam_render_params *p) {
 }
 
 int check_ctrl_overflow(lam_ctx *ctx) {
-  if (!ctx->ctrl) return LAM_OK;
-  uint32_t flag = 0;
-  HIPC(hipMemcpy(&flag, ctx->ctrl + lam::kCtrlOverflow, 4, hipMemcpyDeviceToHost));
-  if (flag) return fail(ctx, LAM_E_BVH, "BVH traversal stack overflow");
+  for (const lam_ctx::Lane &ln : ctx->lane) {
+    if (!ln.ctrl) continue;
+    uint32_t flag = 0;
+    HIPC(hipMemcpy(&flag, ln.ctrl + lam::kCtrlOverflow, 4, hipMemcpyDeviceToHost));
+    if (flag) return fail(ctx, LAM_E_BVH, "BVH traversal stack overflow");
+  }
   return LAM_OK;
 }
 
 void free_guide(lam_ctx *ctx) {
-  void *bufs[] = {ctx->g_th, ctx->g_pack, ctx->g_ws, ctx->g_mix, ctx->g_rec, ctx->g_rec_count, ctx->g_rec_of_slot};
+  void *bufs[] = {ctx->g_th, ctx->g_pack, ctx->g_ws, ctx->lane[0].mix, ctx->lane[1].mix, ctx->g_rec,
+                  ctx->g_rec_count, ctx->g_rec_of_slot};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   ctx->g_th = ctx->g_m = ctx->g_v = ctx->g_grad = ctx->g_thq = ctx->g_scal = nullptr;
   ctx->g_pack = nullptr;
   ctx->g_ws = nullptr;
   ctx->g_ws_bytes = 0;
-  ctx->g_mix = nullptr;
-  ctx->g_mix_cap = 0;
+  for (lam_ctx::Lane &ln : ctx->lane) {
+    ln.mix = nullptr;
+    ln.mix_cap = 0;
+  }
   ctx->g_rec = nullptr;
   ctx->g_rec_cap = 0;
   ctx->g_rec_count = ctx->g_rec_of_slot = nullptr;
+  ctx->g_rec_slot_cap = 0;
   ctx->guide_ready = false;
 }
 
@@ -311,19 +345,43 @@ struct GuideRun {
 };
 
 // Wavefront render of every pixel for passes spp_offset .. spp_offset+spp-1.
-// Each chunk's pass is captured once as a hipGraph (its first node advances
-// the pass index held in the control block) and replayed spp times.
-// LAM_FLAG_PROFILE launches without a graph and times every extend and
-// shadow launch with events; LAM_FLAG_NO_GRAPH launches without a graph.
+//
+// Consecutive passes alternate between two lanes, each with its own path
+// state, control block and stream, so that the tail of one pass (a few
+// paths with many bounces, finished by k_finish on a handful of CUs) shares
+// the GPU with the full wavefront of the next.  Each lane's pass is captured
+// once per chunk as a hipGraph (its first node advances the lane's pass
+// index by the number of lanes) and replayed.  The film kernels stay in pass
+// order: each runs after its own pass and after the previous pass's film on
+// the other lane, so the Kahan film receives the same sums in the same order
+// as a one-lane render and the image is bit-identical to one.
+//
+// LAM_FLAG_NO_PIPE keeps every pass on lane 0.  LAM_FLAG_PROFILE (one lane,
+// no graph) times every extend and shadow launch with events;
+// LAM_FLAG_NO_GRAPH launches without a graph.
 int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix,
                      const GuideRun *gr = nullptr) {
   const size_t cap = std::min(npix, kMaxWaveCap);
-  int rc;
-  if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
-  HIPC(lam::launch_zero(ctx->ctrl, lam::kCtrlWords * 4, ctx->stream));
-  const lam::PathArgs pa = path_args(p);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
+  const int lanes = (p->flags & (LAM_FLAG_NO_PIPE | LAM_FLAG_PROFILE)) || p->spp < 2 ? 1 : 2;
+  int rc;
+  for (int l = 0; l < lanes; l++) {
+    if ((rc = ensure_wave(ctx, cap, int(p->hero), l))) return rc;
+    if (gr && (rc = ensure_mix(ctx, l))) return rc;
+  }
+  hipStream_t st[2] = {ctx->stream, ctx->stream1};
+  lam::WaveState ws[2] = {ctx->lane[0].ws, ctx->lane[1].ws};
+  uint32_t *ctrl[2] = {ctx->lane[0].ctrl, ctx->lane[1].ctrl};
+  lam::GuideDev gdl[2];
+  for (int l = 0; l < 2; l++) {
+    if (ctrl[l]) HIPC(lam::launch_zero(ctrl[l], lam::kCtrlWords * 4, st[0]));
+    if (gr) {
+      gdl[l] = gr->gd_render;
+      gdl[l].mix = static_cast<uint4 *>(ctx->lane[l].mix);
+    }
+  }
+  const lam::PathArgs pa = path_args(p);
   const lam::WaveLaunch wl_all = launch_for(ctx, p);
   const uint32_t iters = lam::wave_iterations(p->max_scatterings, wl_all);
   lam::WaveProfile prof{};
@@ -350,84 +408,115 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
     wl.tail_threshold = std::min(wl.tail_threshold, n / 16);
     uint32_t first = 0;  // passes of this chunk already traced
     if (gr && off == 0) {
-      // training passes: trace with records, weight them, then Adam steps
+      // training passes (lane 0): trace with records, weight them, then Adam
       const lam_guide_params *gp = gr->gp;
       const uint32_t tp = std::min(gp->train_passes, p->spp);
       lam::GuideTrain gt = guide_train_state(ctx, gp->batch, gp->lr, gp->seed);
+      lam::GuideDev gtr = gr->gd_train;
+      gtr.mix = gdl[0].mix;
       ctx->timing.guide_records = 0;
       const auto t0 = std::chrono::steady_clock::now();
       for (; first < tp; first++) {
-        HIPC(lam::launch_zero(gr->gd_train.rec_count, 16, ctx->stream));
-        HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + first, ctx->stream));
-        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
-                               film, nullptr, wl, ctx->stream, nullptr, &gr->gd_train, true));
-        HIPC(lam::guide_weights(int(p->hero), ctx->ws, gr->gd_train, ctx->stream));
-        HIPC(lam::guide_train_records(gt, gr->gd_train.rec, gr->gd_train.rec_count, gr->gd_train.rec_cap,
-                                      gp->steps_per_pass, ctx->stream));
+        HIPC(lam::launch_zero(gtr.rec_count, 16, st[0]));
+        HIPC(lam::enqueue_pass_set(ctrl[0], p->spp_offset + first, st[0]));
+        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[0], n, ctrl[0],
+                               film, nullptr, wl, st[0], nullptr, &gtr, true));
+        HIPC(lam::guide_weights(int(p->hero), ws[0], gtr, st[0]));
+        HIPC(lam::guide_train_records(gt, gtr.rec, gtr.rec_count, gtr.rec_cap, gp->steps_per_pass, st[0]));
         uint32_t cnt = 0;
         float sc[2] = {0.f, 0.f};
-        HIPC(hipMemcpyAsync(&cnt, gr->gd_train.rec_count, 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(hipStreamSynchronize(ctx->stream));
-        ctx->timing.guide_records += std::min(cnt, gr->gd_train.rec_cap);
+        HIPC(hipMemcpyAsync(&cnt, gtr.rec_count, 4, hipMemcpyDeviceToHost, st[0]));
+        HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, st[0]));
+        HIPC(hipStreamSynchronize(st[0]));
+        ctx->timing.guide_records += std::min(cnt, gtr.rec_cap);
         if (first == 0) ctx->timing.guide_loss_first = sc[1];
         ctx->timing.guide_loss_last = sc[1];
       }
       ctx->g_step = gt.step;
-      HIPC(hipStreamSynchronize(ctx->stream));
+      HIPC(hipStreamSynchronize(st[0]));
       ctx->timing.guide_train_ms =
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    const lam::GuideDev *gd = gr ? &gr->gd_render : nullptr;
     if (first >= p->spp) continue;
+    const uint32_t nl = p->spp - first >= 2 ? uint32_t(lanes) : 1u;
+    // with two lanes the film kernel is enqueued on its own, after the wait
+    float *pass_film = nl == 2 ? nullptr : film;
+    if (nl == 2) {  // lane 1 starts after everything enqueued on lane 0 so far
+      HIPC(hipEventRecord(ctx->ev_fork, st[0]));
+      HIPC(hipStreamWaitEvent(st[1], ctx->ev_fork, 0));
+    }
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
     if (graph) {
-      hipGraph_t g = nullptr;
-      hipGraphExec_t ge = nullptr;
-      HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t e1 = lam::enqueue_pass_next(ctx->ctrl, ctx->stream);
-      hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n,
-                                        ctx->ctrl, film, nullptr, wl, ctx->stream, nullptr, gd, false);
-      HIPC(hipStreamEndCapture(ctx->stream, &g));
-      HIPC(e1);
-      HIPC(e2);
-      hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HIPC(e3);
-      HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + first - 1u, ctx->stream));
-      hipError_t e4 = hipSuccess;
-      for (uint32_t s = first; s < p->spp && e4 == hipSuccess; s++) e4 = hipGraphLaunch(ge, ctx->stream);
-      hipError_t e5 = hipStreamSynchronize(ctx->stream);
-      (void)hipGraphExecDestroy(ge);
-      HIPC(e4);
-      HIPC(e5);
-    } else {
-      for (uint32_t s = first; s < p->spp; s++) {
-        HIPC(lam::enqueue_pass_set(ctx->ctrl, p->spp_offset + s, ctx->stream));
-        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ctx->ws, n, ctx->ctrl,
-                               film, nullptr, wl, ctx->stream, profile ? &prof : nullptr, gd, false));
-        if (profile) {
-          HIPC(hipStreamSynchronize(ctx->stream));
-          float ms = 0.f;
-          for (uint32_t it = 0; it < iters; it++) {
-            HIPC(hipEventElapsedTime(&ms, prof.ext0[it], prof.ext1[it]));
-            ctx->timing.extend_ms += ms;
-            HIPC(hipEventElapsedTime(&ms, prof.shd0[it], prof.shd1[it]));
-            ctx->timing.shadow_ms += ms;
-            HIPC(hipEventElapsedTime(&ms, prof.fin0[it], prof.ext0[it]));
-            ctx->timing.other_ms += ms;  // iteration reset + finish kernel
-            HIPC(hipEventElapsedTime(&ms, prof.ext1[it], prof.shd0[it]));
-            ctx->timing.shade_ms += ms;
-          }
-          ctx->timing.extend_launches += iters;
-          ctx->timing.shadow_launches += iters;
-          ctx->timing.shade_launches += iters;
-          HIPC(hipEventElapsedTime(&ms, prof.ev[0], prof.ev[1]));
-          ctx->timing.raygen_ms += ms;
-          HIPC(hipEventElapsedTime(&ms, prof.ev[2], prof.ev[3]));
-          ctx->timing.film_ms += ms;
-        }
+      for (uint32_t l = 0; l < nl && e == hipSuccess; l++) {
+        hipGraph_t g = nullptr;
+        if ((e = hipStreamBeginCapture(st[l], hipStreamCaptureModeThreadLocal)) != hipSuccess) break;
+        hipError_t e1 = lam::enqueue_pass_next(ctrl[l], nl, st[l]);
+        hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], n,
+                                          ctrl[l], pass_film, nullptr, wl, st[l], nullptr, gr ? &gdl[l] : nullptr,
+                                          false);
+        hipError_t e3 = hipStreamEndCapture(st[l], &g);
+        e = e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge[l], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        // the graph's first node advances the pass by nl: start one step back
+        if (e == hipSuccess) e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + first + l - nl, st[l]);
       }
     }
+    for (uint32_t s = first; s < p->spp && e == hipSuccess; s++) {
+      const uint32_t l = (s - first) % nl;
+      if (graph) {
+        e = hipGraphLaunch(ge[l], st[l]);
+      } else {
+        e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + s, st[l]);
+        if (e == hipSuccess)
+          e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], n, ctrl[l],
+                                pass_film, nullptr, wl, st[l], profile ? &prof : nullptr, gr ? &gdl[l] : nullptr,
+                                false);
+      }
+      if (nl == 2 && e == hipSuccess) {
+        if (s > first) e = hipStreamWaitEvent(st[l], ctx->ev_film[l ^ 1u], 0);
+        if (e == hipSuccess) e = lam::enqueue_film(int(p->hero), ctx->sc, pa, uint32_t(off), ws[l], n, ctrl[l],
+                                                   film, st[l]);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_film[l], st[l]);
+      }
+      if (profile && e == hipSuccess) {
+        HIPC(hipStreamSynchronize(st[0]));
+        float ms = 0.f;
+        for (uint32_t it = 0; it < iters; it++) {
+          HIPC(hipEventElapsedTime(&ms, prof.ext0[it], prof.ext1[it]));
+          ctx->timing.extend_ms += ms;
+          HIPC(hipEventElapsedTime(&ms, prof.shd0[it], prof.shd1[it]));
+          ctx->timing.shadow_ms += ms;
+          HIPC(hipEventElapsedTime(&ms, prof.fin0[it], prof.ext0[it]));
+          ctx->timing.other_ms += ms;  // iteration reset + finish kernel
+          HIPC(hipEventElapsedTime(&ms, prof.ext1[it], prof.shd0[it]));
+          ctx->timing.shade_ms += ms;
+        }
+        ctx->timing.extend_launches += iters;
+        ctx->timing.shadow_launches += iters;
+        ctx->timing.shade_launches += iters;
+        HIPC(hipEventElapsedTime(&ms, prof.ev[0], prof.ev[1]));
+        ctx->timing.raygen_ms += ms;
+        HIPC(hipEventElapsedTime(&ms, prof.ev[2], prof.ev[3]));
+        ctx->timing.film_ms += ms;
+      }
+    }
+    // join: lane 0 (and whatever the caller enqueues next) waits for lane 1
+    if (nl == 2) {
+      hipError_t ej = hipEventRecord(ctx->ev_fork, st[1]);
+      if (ej == hipSuccess) ej = hipStreamWaitEvent(st[0], ctx->ev_fork, 0);
+      if (e == hipSuccess) e = ej;
+    }
+    hipError_t es = hipStreamSynchronize(st[0]);
+    if (nl == 2) {
+      hipError_t e1 = hipStreamSynchronize(st[1]);
+      if (es == hipSuccess) es = e1;
+    }
+    for (hipGraphExec_t x : ge)
+      if (x) (void)hipGraphExecDestroy(x);
+    HIPC(e);
+    HIPC(es);
   }
   return LAM_OK;
 }
@@ -467,9 +556,15 @@ void lam_destroy(lam_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_scene(ctx);
-  free_wave(ctx);
+  if (ctx->stream1) (void)hipStreamSynchronize(ctx->stream1);
+  free_wave(ctx, 0);
+  free_wave(ctx, 1);
   free_guide(ctx);
-  if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+  for (lam_ctx::Lane &ln : ctx->lane)
+    if (ln.ctrl) (void)hipFree(ln.ctrl);
+  for (hipEvent_t e : {ctx->ev_film[0], ctx->ev_film[1], ctx->ev_fork})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream1) (void)hipStreamDestroy(ctx->stream1);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 6; i++)
     if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
@@ -730,10 +825,11 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pi
   } else {
     const size_t cap = std::min(n, kMaxWaveCap);
     if ((rc = ensure_wave(ctx, cap, int(H)))) return rc;
-    HIPC(lam::launch_zero(ctx->ctrl, lam::kCtrlWords * 4, ctx->stream));
+    HIPC(lam::launch_zero(ctx->lane[0].ctrl, lam::kCtrlWords * 4, ctx->stream));
     for (size_t off = 0; off < n; off += cap) {
       const uint32_t m = uint32_t(std::min(cap, n - off));
-      HIPC(lam::enqueue_pass(int(H), ctx->sc, path_args(p), dpix + off, dsmp + off, 0, ctx->ws, m, ctx->ctrl,
+      HIPC(lam::enqueue_pass(int(H), ctx->sc, path_args(p), dpix + off, dsmp + off, 0, ctx->lane[0].ws, m,
+                             ctx->lane[0].ctrl,
                              nullptr, dout + off * H, launch_for(ctx, p), ctx->stream, nullptr));
     }
   }
@@ -948,18 +1044,16 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
   const size_t cap = std::min(npix, kMaxWaveCap);
   if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
-  if (ctx->g_mix_cap < ctx->ws.cap) {
-    if (ctx->g_mix) HIPC(hipFree(ctx->g_mix));
-    if (ctx->g_rec_of_slot) HIPC(hipFree(ctx->g_rec_of_slot));
-    ctx->g_mix = nullptr;
-    ctx->g_rec_of_slot = nullptr;
-    ctx->g_mix_cap = 0;
-    HIPC(hipMalloc(&ctx->g_mix, ctx->ws.cap * 80));
-    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_rec_of_slot), ctx->ws.cap * 4));
-    ctx->g_mix_cap = ctx->ws.cap;
-  }
+  const size_t wcap = ctx->lane[0].ws.cap;
   if (g->train_passes) {
-    const uint32_t rcap = uint32_t(std::min<size_t>(ctx->ws.cap * 8, size_t(1) << 25));
+    if (ctx->g_rec_slot_cap < wcap) {
+      if (ctx->g_rec_of_slot) HIPC(hipFree(ctx->g_rec_of_slot));
+      ctx->g_rec_of_slot = nullptr;
+      ctx->g_rec_slot_cap = 0;
+      HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->g_rec_of_slot), wcap * 4));
+      ctx->g_rec_slot_cap = wcap;
+    }
+    const uint32_t rcap = uint32_t(std::min<size_t>(wcap * 8, size_t(1) << 25));
     if (ctx->g_rec_cap < rcap) {
       if (ctx->g_rec) HIPC(hipFree(ctx->g_rec));
       ctx->g_rec = nullptr;
@@ -974,7 +1068,7 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   gr.gp = g;
   lam::GuideDev &gd = gr.gd_render;
   gd.frag = static_cast<const uint4 *>(ctx->g_pack);
-  gd.mix = static_cast<uint4 *>(ctx->g_mix);
+  gd.mix = nullptr;  // per lane, set by render_wavefront
   gd.alpha = g->alpha;
   gd.mat_mask = g->materials ? g->materials
                              : (1u << LAM_MAT_LAMBERTIAN) | (1u << LAM_MAT_BLINN) | (1u << LAM_MAT_THINFILM);
@@ -995,8 +1089,9 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   if ((rc = render_wavefront(ctx, p, film, npix, &gr))) return rc;
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
-  uint32_t gc[2] = {0, 0};
-  HIPC(hipMemcpyAsync(gc, ctx->ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t gc[2][2] = {{0, 0}, {0, 0}};
+  for (int l = 0; l < 2; l++)
+    if (ctx->lane[l].ctrl) HIPC(hipMemcpyAsync(gc[l], ctx->lane[l].ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
@@ -1004,8 +1099,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
-  ctx->timing.guide_dropped = gc[0];
-  ctx->timing.guide_zero_pdf = gc[1];
+  ctx->timing.guide_dropped = gc[0][0] + gc[1][0];
+  ctx->timing.guide_zero_pdf = gc[0][1] + gc[1][1];
   if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }

@@ -94,12 +94,17 @@ hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t
 // training weights of the records of the pass just traced (needs its final L)
 hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipStream_t st);
 
-// gd: null for the plain render loop; train: write training records
+// gd: null for the plain render loop; train: write training records.
+// With film and out both null the pass ends before its film kernel
+// (enqueue_film then splats it, e.g. after waiting for another stream).
 hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
                         const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
                         WaveProfile *prof, const GuideDev *gd = nullptr, bool train = false);
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
+// the film kernel of a pass over pixels pixel_offset .. pixel_offset+n-1
+hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset,
+                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st);
 
 // ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------
 constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block
@@ -107,6 +112,6 @@ hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, cons
                           const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
                           uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
                           uint32_t blocks, hipStream_t st);
-hipError_t enqueue_pass_next(uint32_t *ctrl, hipStream_t st);
+hipError_t enqueue_pass_next(uint32_t *ctrl, uint32_t stride, hipStream_t st);
 
 }  // namespace lam

@@ -111,8 +111,8 @@ __global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q, uint3
 __global__ void k_pass_set(uint32_t *__restrict__ ctrl, uint32_t pass) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = pass;
 }
-__global__ void k_pass_next(uint32_t *__restrict__ ctrl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = ctrl[CTRL_PASS] + 1;
+__global__ void k_pass_next(uint32_t *__restrict__ ctrl, uint32_t stride) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = ctrl[CTRL_PASS] + stride;
 }
 
 // Work distribution of the persistent trace grids (one 64-lane wave per
@@ -773,7 +773,8 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     q ^= 1u;
   }
   ev(2);
-  hipLaunchKernelGGL((k_film<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, out);
+  if (film || out)
+    hipLaunchKernelGGL((k_film<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, out);
   ev(3);
   return hipGetLastError();
 }
@@ -787,12 +788,23 @@ hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const 
   return enqueue_pass_t<4>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof, gd, train);
 }
 
+hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset,
+                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st) {
+  const PathIds ids{nullptr, nullptr, pixel_offset};
+  const uint32_t b256 = (n + 255) / 256;
+  if (hero == 16)
+    hipLaunchKernelGGL((k_film<16>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
+  else
+    hipLaunchKernelGGL((k_film<4>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st) {
   hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, st, ctrl, pass);
   return hipGetLastError();
 }
-hipError_t enqueue_pass_next(uint32_t *ctrl, hipStream_t st) {
-  hipLaunchKernelGGL(k_pass_next, dim3(1), dim3(64), 0, st, ctrl);
+hipError_t enqueue_pass_next(uint32_t *ctrl, uint32_t stride, hipStream_t st) {
+  hipLaunchKernelGGL(k_pass_next, dim3(1), dim3(64), 0, st, ctrl, stride);
   return hipGetLastError();
 }
 

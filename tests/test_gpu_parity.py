@@ -165,6 +165,32 @@ def test_tail_handoff_is_exact(ctx, tmp_path):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_GRAPH, la.LAM_FLAG_NO_TAIL], ids=["graph", "nograph", "notail"])
+@pytest.mark.parametrize("spp,offset", [(2, 0), (5, 3), (8, 0)])
+def test_pipelined_passes_are_exact(ctx, box_scene, flags, spp, offset):
+    """passes alternating between two lanes (streams) splat into the film in
+    pass order: the image equals the one-lane render bit for bit"""
+    ctx.upload(box_scene)
+    a = ctx.render(la.params(spp=spp, max_scatterings=8, sample_p=0.8, hero=4, seed=7, spp_offset=offset,
+                             flags=flags))
+    b = ctx.render(la.params(spp=spp, max_scatterings=8, sample_p=0.8, hero=4, seed=7, spp_offset=offset,
+                             flags=flags | la.LAM_FLAG_NO_PIPE))
+    assert np.array_equal(a, b)
+
+
+def test_pipelined_chunks_are_exact(ctx, tmp_path):
+    """a frame of more than one chunk of path slots (4,410,000 pixels > 4 Mi):
+    both lanes per chunk, the same image as one lane"""
+    from lambda_amd import scenes
+    s = la.load_nls(scenes.box_room(str(tmp_path), seed=2, width=2100))
+    assert s.width * s.height > 1 << 22
+    ctx.upload(s)
+    a = ctx.render(la.params(spp=3, max_scatterings=6, sample_p=0.8, hero=4, seed=5))
+    b = ctx.render(la.params(spp=3, max_scatterings=6, sample_p=0.8, hero=4, seed=5, flags=la.LAM_FLAG_NO_PIPE))
+    assert np.isfinite(a).all() and a.mean() > 0
+    assert np.array_equal(a, b)
+
+
 def test_render_against_fp64_truth(ctx, box_scene):
     """GPU at 8x8 spp vs the FP64 oracle at 64 independent spp: per-pixel
     z-test on luminance and relMSE within the noise of both estimates."""

@@ -215,6 +215,10 @@ def test_guided_alpha0_is_plain(ctx, glossy, flags):
     # with α > 0 the guided vertices take other paths (the proposal is used)
     g5 = ctx.render_guided(p, la.guide_params(alpha=0.5, train_passes=0))
     assert not np.array_equal(plain, g5)
+    # two pipeline lanes (each with its own network-output buffer) give the
+    # one-lane image
+    p1 = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=4, seed=3, flags=flags | la.LAM_FLAG_NO_PIPE)
+    assert np.array_equal(g5, ctx.render_guided(p1, la.guide_params(alpha=0.5, train_passes=0)))
 
 
 @pytest.mark.gpu
