@@ -314,9 +314,9 @@ int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n,
  * triangles tested per ray. */
 int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
                    lam_hit *d_hits, uint32_t *d_counts);
-/* Measurement hook: `reps` back-to-back closest-hit launches over device
- * rays between one pair of HIP events on the context's stream; *avg_ms
- * receives the mean duration of one launch. */
+/* Measurement hook: `reps` (<= 4096) back-to-back closest-hit launches
+ * over device rays between one pair of HIP events on the context's stream;
+ * *avg_ms receives the mean duration of one launch. */
 int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
                          lam_hit *d_hits, uint32_t reps, double *avg_ms);
 
@@ -327,6 +327,21 @@ int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t 
 
 int lam_get_timing(const lam_ctx *ctx, lam_timing *out);
 int lam_reset_timing(lam_ctx *ctx);
+
+/* ---- wavefront launch configuration ------------------------------------
+ * No counterpart in the reference (its render loop is one thread per
+ * pixel, src/renderer.cpp:35-62).  The defaults suit frames of 10^5..10^7
+ * pixels; images do not depend on these values (same paths, same film
+ * order), only the time does. */
+typedef struct {
+  uint32_t lanes;          /* passes in flight, each on its own stream: 1..4 (default 3) */
+  uint32_t tail_threshold; /* the finish kernel takes over a queue at or below this length (default 131072) */
+  uint32_t tail_div;       /* ... capped at 1/tail_div of the chunk's paths, >= 1 (default 16) */
+  uint32_t max_iters;      /* synchronised bounces before the finish kernel takes the rest, >= 1 (default 12) */
+  uint32_t trace_per_cu;   /* extend / shadow blocks (64 lanes) per CU: 1..32 (default 24) */
+} lam_wave_config;
+int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out);
+int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg);
 
 #ifdef __cplusplus
 }

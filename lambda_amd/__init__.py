@@ -136,6 +136,12 @@ class SceneStats(C.Structure):
                 ("device_bytes", C.c_uint64), ("sah_cost", C.c_double)]
 
 
+class WaveConfig(C.Structure):
+    """lam_wave_config"""
+    _fields_ = [("lanes", C.c_uint32), ("tail_threshold", C.c_uint32), ("tail_div", C.c_uint32),
+                ("max_iters", C.c_uint32), ("trace_per_cu", C.c_uint32)]
+
+
 class Timing(C.Structure):
     _fields_ = [("extend_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("raygen_ms", C.c_double), ("film_ms", C.c_double), ("other_ms", C.c_double),
@@ -191,6 +197,8 @@ def lib():
         "lam_extend_dev_timed": (C.c_int, [vp, vp, sz, vp, C.c_uint32, C.POINTER(C.c_double)]),
         "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
         "lam_reset_timing": (C.c_int, [vp]),
+        "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
+        "lam_set_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
         "lamh_load_nls": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
         "lamh_load_nls_text": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
@@ -632,6 +640,18 @@ class Context:
 
     def reset_timing(self):
         self._check(self._lib.lam_reset_timing(self._h))
+
+    def wave_config(self):
+        """the wavefront launch configuration as a dict (lam_get_wave_config)"""
+        c = WaveConfig()
+        self._check(self._lib.lam_get_wave_config(self._h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in WaveConfig._fields_}
+
+    def set_wave_config(self, **kw):
+        """change fields of the launch configuration (lanes, tail_threshold,
+        tail_div, max_iters, trace_per_cu); images do not depend on them"""
+        c = WaveConfig(**{**self.wave_config(), **kw})
+        self._check(self._lib.lam_set_wave_config(self._h, C.byref(c)))
 
     def debug_math(self, fn, a, b=None):
         a = np.ascontiguousarray(a, dtype=np.float32)

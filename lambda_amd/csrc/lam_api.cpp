@@ -16,6 +16,10 @@
 #include "lam_launch.h"
 #include "lambda_amd.h"
 
+static constexpr int kMaxLanes = 4;
+// launches of the batch trace kernel that one timed call may queue
+static constexpr uint32_t kTraceCounters = 4096;
+
 struct lam_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -30,19 +34,29 @@ struct lam_ctx {
   size_t scratch_size[6] = {0, 0, 0, 0, 0, 0};
   lam_timing timing{};
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // wavefront path state, queues and control block of the two pipeline
-  // lanes (lane 0 runs on `stream`; lane 1 only traces when passes overlap)
+  // wavefront path state, queues, control block and stream of each pipeline
+  // lane (lane 0 runs on `stream`; the others only trace when passes overlap)
   struct Lane {
     lam::WaveState ws{};
     int hero = 0;
     std::vector<void *> bufs;
     uint32_t *ctrl = nullptr;
+    hipStream_t stream = nullptr;  // lanes 1.. (lane 0 uses ctx->stream)
+    hipEvent_t ev_film = nullptr;  // recorded after the lane's latest film kernel
+    hipEvent_t ev_join = nullptr;
     void *mix = nullptr;  // guiding-network outputs per queue entry
     size_t mix_cap = 0;
-  } lane[2];
-  hipStream_t stream1 = nullptr;  // lane 1
-  hipEvent_t ev_film[2] = {nullptr, nullptr}, ev_fork = nullptr;
-  lam::WaveLaunch wl{0, 0, 0, 0, 0};
+    uint32_t *spill = nullptr;  // traversal-stack spill columns (WaveState::spill)
+    uint32_t spill_lanes = 0;
+  } lane[kMaxLanes];
+  // block counts are set on the first render (they depend on the CU count);
+  // the finish kernel takes over below 131,072 paths or after 12 bounces
+  lam::WaveLaunch wl{0, 0, 131072u, 12u, 0, 0};
+  int cus = 0;
+  uint32_t trace_per_cu = 24;  // extend / shadow blocks per CU
+  uint32_t *trace_ctr = nullptr;  // fetch counters of the batch trace kernel (one per launch)
+  uint32_t lanes = 3;     // passes in flight
+  uint32_t tail_div = 16;  // the finish kernel's threshold is at most 1/tail_div of a chunk
   std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
   float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // scene box (guiding inputs)
   // guiding network: fp32 master parameters, Adam moments, gradient,
@@ -150,32 +164,62 @@ void free_wave(lam_ctx *ctx, int l) {
   ln.hero = 0;
 }
 
-// Launch shape, control block and path state of lane l for chunks of up to
-// cap paths.  Lane 1's stream and events are created on first use.
-int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
+// Launch shape (from the CU count), lane 0's control block and the fetch
+// counters of the batch trace kernel, on first use.
+int ensure_launch(lam_ctx *ctx) {
   if (!ctx->lane[0].ctrl) {
     HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->lane[0].ctrl), lam::kCtrlWords * 4));
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->trace_ctr), kTraceCounters * 4));
     int cus = 0;
     HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    // extend/shadow: 16 KB of LDS stack per 64-lane block, 10 per CU;
-    // shade: 256-lane blocks, grid-stride, one occupancy round (its VGPR
-    // count allows 2 waves per SIMD = 2 blocks per CU)
-    ctx->wl.trace_blocks = uint32_t(cus > 0 ? cus : 256) * 10u;
-    ctx->wl.shade_blocks = uint32_t(cus > 0 ? cus : 256) * 2u;
-    // below ~128 Ki paths a synchronised bounce is bounded by its slowest
-    // ray; the finish kernel then traces each remaining path to its end
-    ctx->wl.tail_threshold = 131072u;
-    ctx->wl.max_iters = 12u;
+    ctx->cus = cus > 0 ? cus : 256;
+    const uint32_t c = uint32_t(ctx->cus);
+    // extend/shadow: 6 KB of LDS stack per 64-lane block (the rest of the
+    // stack spills to HBM), trace_per_cu blocks per CU (24: the VGPR limit);
+    // finish: 16 KB of LDS stack per block, 10 per CU;
+    // shade: 256-lane blocks, grid-stride, one occupancy round (its launch
+    // bounds give 3 waves per SIMD = 3 blocks per CU)
+    ctx->wl.trace_blocks = c * ctx->trace_per_cu;
+    ctx->wl.finish_blocks = c * 10u;
+    ctx->wl.shade_blocks = c * 3u;
     // guiding network: 51 KB of LDS per 256-lane block, 3 blocks per CU
-    ctx->wl.guide_blocks = uint32_t(cus > 0 ? cus : 256) * 3u;
+    ctx->wl.guide_blocks = c * 3u;
   }
+  return LAM_OK;
+}
+
+// Traversal-stack spill columns of lane l for the current trace grid.
+int ensure_spill(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
-  if (l == 1 && !ln.ctrl) {
-    HIPC(hipMalloc(reinterpret_cast<void **>(&ln.ctrl), lam::kCtrlWords * 4));
-    HIPC(hipStreamCreateWithFlags(&ctx->stream1, hipStreamNonBlocking));
-    for (hipEvent_t *e : {&ctx->ev_film[0], &ctx->ev_film[1], &ctx->ev_fork})
-      HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  const uint32_t glanes = ctx->wl.trace_blocks * 64u;
+  if (ln.spill_lanes < glanes) {
+    if (ln.spill) HIPC(hipFree(ln.spill));
+    ln.spill = nullptr;
+    ln.spill_lanes = 0;
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ln.spill), size_t(lam::kStack - lam::kLdsStack) * glanes * 4));
+    ln.spill_lanes = glanes;
   }
+  ln.ws.spill = ln.spill;
+  ln.ws.spill_lanes = ln.spill_lanes;
+  return LAM_OK;
+}
+
+// Launch shape, control block, spill columns and path state of lane l for
+// chunks of up to cap paths.  A lane's stream and events are created on its
+// first use.
+int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
+  int rc;
+  if ((rc = ensure_launch(ctx))) return rc;
+  lam_ctx::Lane &ln = ctx->lane[l];
+  if (!ln.ev_film) {
+    if (l > 0) {
+      HIPC(hipMalloc(reinterpret_cast<void **>(&ln.ctrl), lam::kCtrlWords * 4));
+      HIPC(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+    }
+    HIPC(hipEventCreateWithFlags(&ln.ev_join, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&ln.ev_film, hipEventDisableTiming));
+  }
+  if ((rc = ensure_spill(ctx, l))) return rc;
   if (ln.ws.cap >= cap && ln.hero >= hero) return LAM_OK;
   free_wave(ctx, l);
   cap = std::max(cap, size_t(64));
@@ -184,7 +228,6 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
     ln.bufs.push_back(*p);
     return LAM_OK;
   };
-  int rc;
   void *p;
   const size_t hf = size_t(hero) * cap * 4;
   if ((rc = alloc(hf, &p))) return rc;
@@ -201,6 +244,8 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
   if ((rc = alloc(cap * 4, &p))) return rc;
   ln.ws.spath = static_cast<uint32_t *>(p);
   ln.ws.cap = cap;
+  ln.ws.spill = ln.spill;
+  ln.ws.spill_lanes = ln.spill_lanes;
   ln.hero = hero;
   return LAM_OK;
 }
@@ -237,10 +282,11 @@ int check_ctrl_overflow(lam_ctx *ctx) {
 }
 
 void free_guide(lam_ctx *ctx) {
-  void *bufs[] = {ctx->g_th, ctx->g_pack, ctx->g_ws, ctx->lane[0].mix, ctx->lane[1].mix, ctx->g_rec,
-                  ctx->g_rec_count, ctx->g_rec_of_slot};
+  void *bufs[] = {ctx->g_th, ctx->g_pack, ctx->g_ws, ctx->g_rec, ctx->g_rec_count, ctx->g_rec_of_slot};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
+  for (lam_ctx::Lane &ln : ctx->lane)
+    if (ln.mix) (void)hipFree(ln.mix);
   ctx->g_th = ctx->g_m = ctx->g_v = ctx->g_grad = ctx->g_thq = ctx->g_scal = nullptr;
   ctx->g_pack = nullptr;
   ctx->g_ws = nullptr;
@@ -346,15 +392,15 @@ struct GuideRun {
 
 // Wavefront render of every pixel for passes spp_offset .. spp_offset+spp-1.
 //
-// Consecutive passes alternate between two lanes, each with its own path
-// state, control block and stream, so that the tail of one pass (a few
+// Consecutive passes go round-robin to `lanes` lanes, each with its own
+// path state, control block and stream, so that the tail of one pass (a few
 // paths with many bounces, finished by k_finish on a handful of CUs) shares
-// the GPU with the full wavefront of the next.  Each lane's pass is captured
-// once per chunk as a hipGraph (its first node advances the lane's pass
-// index by the number of lanes) and replayed.  The film kernels stay in pass
-// order: each runs after its own pass and after the previous pass's film on
-// the other lane, so the Kahan film receives the same sums in the same order
-// as a one-lane render and the image is bit-identical to one.
+// the GPU with the full wavefronts of the next.  Each lane's pass is
+// captured once per chunk as a hipGraph (its first node advances the lane's
+// pass index by the number of lanes) and replayed.  The film kernels stay in
+// pass order: each runs after its own pass and after the previous pass's
+// film on the previous lane, so the Kahan film receives the same sums in the
+// same order as a one-lane render and the image is bit-identical to one.
 //
 // LAM_FLAG_NO_PIPE keeps every pass on lane 0.  LAM_FLAG_PROFILE (one lane,
 // no graph) times every extend and shadow launch with events;
@@ -364,21 +410,27 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   const size_t cap = std::min(npix, kMaxWaveCap);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
-  const int lanes = (p->flags & (LAM_FLAG_NO_PIPE | LAM_FLAG_PROFILE)) || p->spp < 2 ? 1 : 2;
+  const uint32_t lanes = (p->flags & (LAM_FLAG_NO_PIPE | LAM_FLAG_PROFILE)) ? 1u : std::min(ctx->lanes, p->spp);
   int rc;
-  for (int l = 0; l < lanes; l++) {
-    if ((rc = ensure_wave(ctx, cap, int(p->hero), l))) return rc;
-    if (gr && (rc = ensure_mix(ctx, l))) return rc;
+  for (uint32_t l = 0; l < lanes; l++) {
+    if ((rc = ensure_wave(ctx, cap, int(p->hero), int(l)))) return rc;
+    if (gr && (rc = ensure_mix(ctx, int(l)))) return rc;
   }
-  hipStream_t st[2] = {ctx->stream, ctx->stream1};
-  lam::WaveState ws[2] = {ctx->lane[0].ws, ctx->lane[1].ws};
-  uint32_t *ctrl[2] = {ctx->lane[0].ctrl, ctx->lane[1].ctrl};
-  lam::GuideDev gdl[2];
-  for (int l = 0; l < 2; l++) {
+  hipStream_t st[kMaxLanes];
+  lam::WaveState ws[kMaxLanes];
+  uint32_t *ctrl[kMaxLanes];
+  lam::GuideDev gdl[kMaxLanes];
+  for (int l = 0; l < kMaxLanes; l++) {
+    lam_ctx::Lane &ln = ctx->lane[l];
+    st[l] = l == 0 ? ctx->stream : ln.stream;
+    ws[l] = ln.ws;
+    ctrl[l] = ln.ctrl;
+    // every control block (also of lanes this render leaves idle) starts at
+    // zero, so that the counters read after the render are this render's
     if (ctrl[l]) HIPC(lam::launch_zero(ctrl[l], lam::kCtrlWords * 4, st[0]));
     if (gr) {
       gdl[l] = gr->gd_render;
-      gdl[l].mix = static_cast<uint4 *>(ctx->lane[l].mix);
+      gdl[l].mix = static_cast<uint4 *>(ln.mix);
     }
   }
   const lam::PathArgs pa = path_args(p);
@@ -401,11 +453,12 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   }
   for (size_t off = 0; off < npix; off += cap) {
     const uint32_t n = uint32_t(std::min(cap, npix - off));
-    // the finish kernel takes over below 1/16 of the chunk's paths (and at
-    // most 131,072): guided passes then shade most vertices with the
-    // learned proposal, and small frames keep a full wavefront longer
+    // the finish kernel takes over below 1/tail_div of the chunk's paths
+    // (and at most tail_threshold): guided passes then shade most vertices
+    // with the learned proposal, and small frames keep a full wavefront
+    // longer
     lam::WaveLaunch wl = wl_all;
-    wl.tail_threshold = std::min(wl.tail_threshold, n / 16);
+    wl.tail_threshold = std::min(wl.tail_threshold, n / ctx->tail_div);
     uint32_t first = 0;  // passes of this chunk already traced
     if (gr && off == 0) {
       // training passes (lane 0): trace with records, weight them, then Adam
@@ -438,14 +491,14 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (first >= p->spp) continue;
-    const uint32_t nl = p->spp - first >= 2 ? uint32_t(lanes) : 1u;
-    // with two lanes the film kernel is enqueued on its own, after the wait
-    float *pass_film = nl == 2 ? nullptr : film;
-    if (nl == 2) {  // lane 1 starts after everything enqueued on lane 0 so far
-      HIPC(hipEventRecord(ctx->ev_fork, st[0]));
-      HIPC(hipStreamWaitEvent(st[1], ctx->ev_fork, 0));
+    const uint32_t nl = std::min(lanes, p->spp - first);
+    // with several lanes the film kernel is enqueued on its own, after the wait
+    float *pass_film = nl > 1 ? nullptr : film;
+    if (nl > 1) {  // the other lanes start after everything enqueued on lane 0 so far
+      HIPC(hipEventRecord(ctx->lane[0].ev_join, st[0]));
+      for (uint32_t l = 1; l < nl; l++) HIPC(hipStreamWaitEvent(st[l], ctx->lane[0].ev_join, 0));
     }
-    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    hipGraphExec_t ge[kMaxLanes] = {};
     hipError_t e = hipSuccess;
     if (graph) {
       for (uint32_t l = 0; l < nl && e == hipSuccess; l++) {
@@ -474,11 +527,11 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
                                 pass_film, nullptr, wl, st[l], profile ? &prof : nullptr, gr ? &gdl[l] : nullptr,
                                 false);
       }
-      if (nl == 2 && e == hipSuccess) {
-        if (s > first) e = hipStreamWaitEvent(st[l], ctx->ev_film[l ^ 1u], 0);
+      if (nl > 1 && e == hipSuccess) {
+        if (s > first) e = hipStreamWaitEvent(st[l], ctx->lane[(l + nl - 1) % nl].ev_film, 0);
         if (e == hipSuccess) e = lam::enqueue_film(int(p->hero), ctx->sc, pa, uint32_t(off), ws[l], n, ctrl[l],
                                                    film, st[l]);
-        if (e == hipSuccess) e = hipEventRecord(ctx->ev_film[l], st[l]);
+        if (e == hipSuccess) e = hipEventRecord(ctx->lane[l].ev_film, st[l]);
       }
       if (profile && e == hipSuccess) {
         HIPC(hipStreamSynchronize(st[0]));
@@ -502,15 +555,15 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
         ctx->timing.film_ms += ms;
       }
     }
-    // join: lane 0 (and whatever the caller enqueues next) waits for lane 1
-    if (nl == 2) {
-      hipError_t ej = hipEventRecord(ctx->ev_fork, st[1]);
-      if (ej == hipSuccess) ej = hipStreamWaitEvent(st[0], ctx->ev_fork, 0);
+    // join: lane 0 (and whatever the caller enqueues next) waits for the others
+    for (uint32_t l = 1; l < nl; l++) {
+      hipError_t ej = hipEventRecord(ctx->lane[l].ev_join, st[l]);
+      if (ej == hipSuccess) ej = hipStreamWaitEvent(st[0], ctx->lane[l].ev_join, 0);
       if (e == hipSuccess) e = ej;
     }
-    hipError_t es = hipStreamSynchronize(st[0]);
-    if (nl == 2) {
-      hipError_t e1 = hipStreamSynchronize(st[1]);
+    hipError_t es = hipSuccess;
+    for (uint32_t l = 0; l < nl; l++) {
+      hipError_t e1 = hipStreamSynchronize(st[l]);
       if (es == hipSuccess) es = e1;
     }
     for (hipGraphExec_t x : ge)
@@ -556,15 +609,18 @@ void lam_destroy(lam_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_scene(ctx);
-  if (ctx->stream1) (void)hipStreamSynchronize(ctx->stream1);
-  free_wave(ctx, 0);
-  free_wave(ctx, 1);
-  free_guide(ctx);
   for (lam_ctx::Lane &ln : ctx->lane)
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+  for (int l = 0; l < kMaxLanes; l++) free_wave(ctx, l);
+  free_guide(ctx);
+  if (ctx->trace_ctr) (void)hipFree(ctx->trace_ctr);
+  for (lam_ctx::Lane &ln : ctx->lane) {
     if (ln.ctrl) (void)hipFree(ln.ctrl);
-  for (hipEvent_t e : {ctx->ev_film[0], ctx->ev_film[1], ctx->ev_fork})
-    if (e) (void)hipEventDestroy(e);
-  if (ctx->stream1) (void)hipStreamDestroy(ctx->stream1);
+    if (ln.spill) (void)hipFree(ln.spill);
+    for (hipEvent_t e : {ln.ev_film, ln.ev_join})
+      if (e) (void)hipEventDestroy(e);
+    if (ln.stream) (void)hipStreamDestroy(ln.stream);
+  }
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 6; i++)
     if (ctx->scratch[i]) (void)hipFree(ctx->scratch[i]);
@@ -729,13 +785,15 @@ static int trace_rays_host(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit 
   int rc;
   if ((rc = ensure_scratch(ctx, 0, n * sizeof(lam_ray)))) return rc;
   if ((rc = ensure_scratch(ctx, 1, n * sizeof(lam_hit)))) return rc;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
   HIPC(hipMemcpyAsync(ctx->scratch[0], rays, n * sizeof(lam_ray), hipMemcpyHostToDevice, ctx->stream));
   if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(lam::launch_zero(ctx->trace_ctr, 16, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   HIPC(lam::launch_trace_rays(ctx->sc, static_cast<lam_ray *>(ctx->scratch[0]), n,
                               occl ? nullptr : static_cast<lam_hit *>(ctx->scratch[1]),
                               occl ? static_cast<uint32_t *>(ctx->scratch[1]) : nullptr, nullptr, ctx->d_err,
-                              ctx->stream));
+                              ctx->trace_ctr, ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   if (occl)
     HIPC(hipMemcpyAsync(occl, ctx->scratch[1], n * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -765,9 +823,12 @@ int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hit
   if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
   HIPC(hipSetDevice(ctx->device));
   int rc;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
   if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(lam::launch_zero(ctx->trace_ctr, 16, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
-  HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, d_counts, ctx->d_err, ctx->stream));
+  HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, d_counts, ctx->d_err, ctx->trace_ctr,
+                              ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.extend_ms += event_ms(ctx);
@@ -782,12 +843,17 @@ int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit 
   if (!d_rays || !d_hits || !avg_ms || reps == 0 || n == 0) return fail(ctx, LAM_E_INVALID, "bad argument");
   if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
   if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  if (reps > kTraceCounters) return fail(ctx, LAM_E_LIMIT, "at most 4096 launches per call");
   HIPC(hipSetDevice(ctx->device));
   int rc;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
   if ((rc = clear_err_flag(ctx))) return rc;
+  // one zeroed fetch counter per launch, cleared before the timed region
+  HIPC(lam::launch_zero(ctx->trace_ctr, size_t(kTraceCounters) * 4, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   for (uint32_t r = 0; r < reps; r++)
-    HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, nullptr, ctx->d_err, ctx->stream));
+    HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, nullptr, ctx->d_err, ctx->trace_ctr + r,
+                                ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   const double ms = event_ms(ctx);
@@ -1089,8 +1155,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   if ((rc = render_wavefront(ctx, p, film, npix, &gr))) return rc;
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
-  uint32_t gc[2][2] = {{0, 0}, {0, 0}};
-  for (int l = 0; l < 2; l++)
+  uint32_t gc[kMaxLanes][2] = {};
+  for (int l = 0; l < kMaxLanes; l++)
     if (ctx->lane[l].ctrl) HIPC(hipMemcpyAsync(gc[l], ctx->lane[l].ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
@@ -1099,8 +1165,11 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
-  ctx->timing.guide_dropped = gc[0][0] + gc[1][0];
-  ctx->timing.guide_zero_pdf = gc[0][1] + gc[1][1];
+  ctx->timing.guide_dropped = ctx->timing.guide_zero_pdf = 0;
+  for (int l = 0; l < kMaxLanes; l++) {
+    ctx->timing.guide_dropped += gc[l][0];
+    ctx->timing.guide_zero_pdf += gc[l][1];
+  }
   if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
@@ -1133,6 +1202,36 @@ int lam_get_timing(const lam_ctx *ctx, lam_timing *out) {
 int lam_reset_timing(lam_ctx *ctx) {
   if (!ctx) return LAM_E_INVALID;
   ctx->timing = lam_timing{};
+  return LAM_OK;
+}
+
+int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out) {
+  if (!ctx || !out) return LAM_E_INVALID;
+  out->lanes = ctx->lanes;
+  out->tail_threshold = ctx->wl.tail_threshold;
+  out->tail_div = ctx->tail_div;
+  out->max_iters = ctx->wl.max_iters;
+  out->trace_per_cu = ctx->trace_per_cu;
+  return LAM_OK;
+}
+
+int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!cfg) return fail(ctx, LAM_E_INVALID, "NULL config");
+  if (cfg->lanes < 1 || cfg->lanes > uint32_t(kMaxLanes)) return fail(ctx, LAM_E_INVALID, "lanes must be 1..4");
+  if (cfg->tail_div < 1) return fail(ctx, LAM_E_INVALID, "tail_div must be >= 1");
+  // the finish kernel runs only where max_iters is set (LAM_FLAG_NO_TAIL
+  // gives the pure wavefront)
+  if (cfg->max_iters < 1) return fail(ctx, LAM_E_INVALID, "max_iters must be >= 1");
+  if (cfg->trace_per_cu < 1 || cfg->trace_per_cu > 32) return fail(ctx, LAM_E_INVALID, "trace_per_cu must be 1..32");
+  ctx->lanes = cfg->lanes;
+  ctx->wl.tail_threshold = cfg->tail_threshold;
+  ctx->tail_div = cfg->tail_div;
+  ctx->wl.max_iters = cfg->max_iters;
+  ctx->trace_per_cu = cfg->trace_per_cu;
+  // before the first render the CU count is not known yet: ensure_wave
+  // sets the grid from trace_per_cu then
+  if (ctx->cus > 0) ctx->wl.trace_blocks = uint32_t(ctx->cus) * cfg->trace_per_cu;
   return LAM_OK;
 }
 

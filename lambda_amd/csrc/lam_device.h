@@ -218,131 +218,245 @@ struct TravCounts {
   uint32_t nodes, tris;
 };
 
-// Closest hit (ANY=false) or any hit (ANY=true) for t in [BIAS, tmax],
-// objects `skip` excluded.  Ties go to the lowest (obj, prim).
-// `stk` points at this lane's LDS stack column: entry k at stk[k*64].
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, uint32_t *stk,
-                                     TravCounts &cnt, bool &overflow) {
+// Traversal stacks; entry k of a lane's stack lives at:
+//   LdsStack    the lane's LDS column, lds[k*64] (kStack entries, 16 KB per
+//               64-lane block);
+//   SplitStack  the LDS column for k < kLdsStack, else the lane's column of
+//               a global spill buffer, spill[(k - kLdsStack) * stride].
+// Deep entries are rare (a push needs several overlapping children at every
+// level above), so the split stack trades a few global accesses for a block
+// footprint of kLdsStack*256 B and ~2.5x the resident waves per CU.
+constexpr int kLdsStack = 24;
+struct LdsStack {
+  uint32_t *lds;
+  __device__ __forceinline__ void put(int k, uint32_t v) const { lds[k * 64] = v; }
+  __device__ __forceinline__ uint32_t get(int k) const { return lds[k * 64]; }
+};
+struct SplitStack {
+  uint32_t *lds;
+  uint32_t *spill;
+  uint32_t stride;
+  __device__ __forceinline__ void put(int k, uint32_t v) const {
+    if (k < kLdsStack)
+      lds[k * 64] = v;
+    else
+      spill[size_t(k - kLdsStack) * stride] = v;
+  }
+  __device__ __forceinline__ uint32_t get(int k) const {
+    return k < kLdsStack ? lds[k * 64] : spill[size_t(k - kLdsStack) * stride];
+  }
+};
+
+// Traversal state of one ray: the ray, its slab-test constants, the best
+// hit so far, the current node and the stack depth (the stack itself is
+// the lane's LDS column).  Closest hit (ANY=false) or any hit (ANY=true)
+// for t in [BIAS, tmax], objects `skip` excluded; ties go to the lowest
+// (obj, prim).
+struct Trav {
+  Ray r;
+  float ix, iy, iz;
+  f3 mr;
   Hit best;
-  best.t = r.tmax;
-  best.slot = LAM_NONE;
-  best.obj = LAM_NONE;
-  best.prim = LAM_NONE;
-  // spheres and planes: linear loop as intersect::scene (cg.hpp:1477-1487)
+  uint32_t cur;
+  int sp;
+};
+
+// Starts a ray: the analytic objects (spheres, planes) as the linear loop
+// of intersect::scene (cg.hpp:1477-1487), then the BVH root.  Returns false
+// when the ray is already decided (any-hit found, or no BVH).
+template <bool ANY>
+__device__ __forceinline__ bool trav_begin(const DevScene &sc, const Ray &r, Trav &t) {
+  t.r = r;
+  t.best.t = r.tmax;
+  t.best.slot = LAM_NONE;
+  t.best.obj = LAM_NONE;
+  t.best.prim = LAM_NONE;
   for (uint32_t a = 0; a < sc.n_analytic; a++) {
     const uint32_t oi = sc.analytic[a];
     if (oi == r.skip) continue;
-    const float t = analytic_t(sc.objects[oi], r.o, r.d);
-    if (t >= kBias && better(t, oi, 0u, best)) {
-      best.t = t;
-      best.slot = LAM_NONE;
-      best.obj = oi;
-      best.prim = 0;
-      if (ANY) return best;
+    const float th = analytic_t(sc.objects[oi], r.o, r.d);
+    if (th >= kBias && better(th, oi, 0u, t.best)) {
+      t.best.t = th;
+      t.best.slot = LAM_NONE;
+      t.best.obj = oi;
+      t.best.prim = 0;
+      if (ANY) return false;
     }
   }
-  if (sc.n_nodes == 0) return best;
-
+  if (sc.n_nodes == 0) return false;
   // ray constructor inv_u (cg.hpp:357-361)
-  const float ix = r.d.x != 0.f ? 1.f / r.d.x : kFltMax;
-  const float iy = r.d.y != 0.f ? 1.f / r.d.y : kFltMax;
-  const float iz = r.d.z != 0.f ? 1.f / r.d.z : kFltMax;
-  const f3 mr = cross(r.o, r.d);
-  int sp = 0;
-  uint32_t cur = 0;  // root is always an interior record
-  while (true) {
-    if (!(cur & kLeafBit)) {
-      const Node8d *nd = sc.nodes + cur;
-      const float4 h0 = *reinterpret_cast<const float4 *>(&nd->ox);
-      const uint4 c0 = *reinterpret_cast<const uint4 *>(&nd->child[0]);
-      const uint4 c1 = *reinterpret_cast<const uint4 *>(&nd->child[4]);
-      const uint4 q0 = *reinterpret_cast<const uint4 *>(&nd->qlo[0]);
-      const uint4 q1 = *reinterpret_cast<const uint4 *>(&nd->qlo[4]);  // qlo[4..5], qhi[0..1]
-      const uint4 q2 = *reinterpret_cast<const uint4 *>(&nd->qhi[2]);  // qhi[2..5]
-      if (COUNT) cnt.nodes++;
-      const uint32_t en = __float_as_uint(h0.w);
-      const float sx = scale_of(en & 0xffu), sy = scale_of((en >> 8) & 0xffu),
-                  sz = scale_of((en >> 16) & 0xffu);
-      const uint32_t nchild = en >> 24;
-      const uint32_t qlo[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-      const uint32_t qhi[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-      const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-      const float kInf = __builtin_inff();
-      float tk[8];
-      uint32_t ck[8];
+  t.ix = r.d.x != 0.f ? 1.f / r.d.x : kFltMax;
+  t.iy = r.d.y != 0.f ? 1.f / r.d.y : kFltMax;
+  t.iz = r.d.z != 0.f ? 1.f / r.d.z : kFltMax;
+  t.mr = cross(r.o, r.d);
+  t.sp = 0;
+  t.cur = 0;  // root is always an interior record
+  return true;
+}
+
+// One traversal step: an interior node (test the 8 children, push the hit
+// ones but the nearest, farthest first, descend into the nearest) or a leaf
+// (test its triangles), then pop if nothing was descended into.  Returns
+// false when the ray is finished (stack empty, any-hit found or overflow).
+template <bool ANY, bool COUNT, class Stack>
+__device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Stack &stk, TravCounts &cnt,
+                                          bool &overflow) {
+  const uint32_t cur = t.cur;
+  if (!(cur & kLeafBit)) {
+    const Node8d *nd = sc.nodes + cur;
+    const float4 h0 = *reinterpret_cast<const float4 *>(&nd->ox);
+    const uint4 c0 = *reinterpret_cast<const uint4 *>(&nd->child[0]);
+    const uint4 c1 = *reinterpret_cast<const uint4 *>(&nd->child[4]);
+    const uint4 q0 = *reinterpret_cast<const uint4 *>(&nd->qlo[0]);
+    const uint4 q1 = *reinterpret_cast<const uint4 *>(&nd->qlo[4]);  // qlo[4..5], qhi[0..1]
+    const uint4 q2 = *reinterpret_cast<const uint4 *>(&nd->qhi[2]);  // qhi[2..5]
+    if (COUNT) cnt.nodes++;
+    const uint32_t en = __float_as_uint(h0.w);
+    const float sx = scale_of(en & 0xffu), sy = scale_of((en >> 8) & 0xffu), sz = scale_of((en >> 16) & 0xffu);
+    const uint32_t nchild = en >> 24;
+    const uint32_t qlo[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+    const uint32_t qhi[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+    const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    const float kInf = __builtin_inff();
+    const f3 o = t.r.o;
+    float tk[8];
+    uint32_t ck[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int w = k >> 2, sh = (k & 3) * 8;
-        const float lx = h0.x + float((qlo[0 + w] >> sh) & 0xffu) * sx;
-        const float ly = h0.y + float((qlo[2 + w] >> sh) & 0xffu) * sy;
-        const float lz = h0.z + float((qlo[4 + w] >> sh) & 0xffu) * sz;
-        const float hx = h0.x + float((qhi[0 + w] >> sh) & 0xffu) * sx;
-        const float hy = h0.y + float((qhi[2 + w] >> sh) & 0xffu) * sy;
-        const float hz = h0.z + float((qhi[4 + w] >> sh) & 0xffu) * sz;
-        const float tx0 = (lx - r.o.x) * ix, tx1 = (hx - r.o.x) * ix;
-        const float ty0 = (ly - r.o.y) * iy, ty1 = (hy - r.o.y) * iy;
-        const float tz0 = (lz - r.o.z) * iz, tz1 = (hz - r.o.z) * iz;
-        const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
-        const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), best.t));
-        const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobust);
-        tk[k] = hit ? fminf(tn, kFltMax) : kInf;
-        ck[k] = ch[k];
-      }
-      // sorting network for 8 (19 comparators), nearest first
-      cswap(tk[0], ck[0], tk[1], ck[1]); cswap(tk[2], ck[2], tk[3], ck[3]);
-      cswap(tk[4], ck[4], tk[5], ck[5]); cswap(tk[6], ck[6], tk[7], ck[7]);
-      cswap(tk[0], ck[0], tk[2], ck[2]); cswap(tk[1], ck[1], tk[3], ck[3]);
-      cswap(tk[4], ck[4], tk[6], ck[6]); cswap(tk[5], ck[5], tk[7], ck[7]);
-      cswap(tk[1], ck[1], tk[2], ck[2]); cswap(tk[5], ck[5], tk[6], ck[6]);
-      cswap(tk[0], ck[0], tk[4], ck[4]); cswap(tk[3], ck[3], tk[7], ck[7]);
-      cswap(tk[1], ck[1], tk[5], ck[5]); cswap(tk[2], ck[2], tk[6], ck[6]);
-      cswap(tk[1], ck[1], tk[4], ck[4]); cswap(tk[3], ck[3], tk[6], ck[6]);
-      cswap(tk[2], ck[2], tk[4], ck[4]); cswap(tk[3], ck[3], tk[5], ck[5]);
-      cswap(tk[3], ck[3], tk[4], ck[4]);
-      if (tk[0] != kInf) {
-        // push the hit children except the nearest, farthest first
+    for (int k = 0; k < 8; k++) {
+      const int w = k >> 2, sh = (k & 3) * 8;
+      const float lx = h0.x + float((qlo[0 + w] >> sh) & 0xffu) * sx;
+      const float ly = h0.y + float((qlo[2 + w] >> sh) & 0xffu) * sy;
+      const float lz = h0.z + float((qlo[4 + w] >> sh) & 0xffu) * sz;
+      const float hx = h0.x + float((qhi[0 + w] >> sh) & 0xffu) * sx;
+      const float hy = h0.y + float((qhi[2 + w] >> sh) & 0xffu) * sy;
+      const float hz = h0.z + float((qhi[4 + w] >> sh) & 0xffu) * sz;
+      const float tx0 = (lx - o.x) * t.ix, tx1 = (hx - o.x) * t.ix;
+      const float ty0 = (ly - o.y) * t.iy, ty1 = (hy - o.y) * t.iy;
+      const float tz0 = (lz - o.z) * t.iz, tz1 = (hz - o.z) * t.iz;
+      const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+      const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t.best.t));
+      const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobust);
+      tk[k] = hit ? fminf(tn, kFltMax) : kInf;
+      ck[k] = ch[k];
+    }
+    // sorting network for 8 (19 comparators), nearest first
+    cswap(tk[0], ck[0], tk[1], ck[1]); cswap(tk[2], ck[2], tk[3], ck[3]);
+    cswap(tk[4], ck[4], tk[5], ck[5]); cswap(tk[6], ck[6], tk[7], ck[7]);
+    cswap(tk[0], ck[0], tk[2], ck[2]); cswap(tk[1], ck[1], tk[3], ck[3]);
+    cswap(tk[4], ck[4], tk[6], ck[6]); cswap(tk[5], ck[5], tk[7], ck[7]);
+    cswap(tk[1], ck[1], tk[2], ck[2]); cswap(tk[5], ck[5], tk[6], ck[6]);
+    cswap(tk[0], ck[0], tk[4], ck[4]); cswap(tk[3], ck[3], tk[7], ck[7]);
+    cswap(tk[1], ck[1], tk[5], ck[5]); cswap(tk[2], ck[2], tk[6], ck[6]);
+    cswap(tk[1], ck[1], tk[4], ck[4]); cswap(tk[3], ck[3], tk[6], ck[6]);
+    cswap(tk[2], ck[2], tk[4], ck[4]); cswap(tk[3], ck[3], tk[5], ck[5]);
+    cswap(tk[3], ck[3], tk[4], ck[4]);
+    if (tk[0] != kInf) {
 #pragma unroll
-        for (int k = 7; k >= 1; k--) {
-          if (tk[k] != kInf) {
-            if (sp >= kStack) { overflow = true; return best; }
-            stk[sp * 64] = ck[k];
-            sp++;
+      for (int k = 7; k >= 1; k--) {
+        if (tk[k] != kInf) {
+          if (t.sp >= kStack) {
+            overflow = true;
+            return false;
           }
+          stk.put(t.sp, ck[k]);
+          t.sp++;
         }
-        cur = ck[0];
-        continue;
       }
-    } else {
-      const uint32_t first = cur & 0x07FFFFFFu;
-      const uint32_t count = ((cur >> 27) & 7u) + 1u;
-      for (uint32_t i = 0; i < count; i++) {
-        const uint32_t s = first + i;
-        const float4 a = sc.tris[3 * s + 0];
-        const float4 b = sc.tris[3 * s + 1];
-        const float4 c = sc.tris[3 * s + 2];
-        if (COUNT) cnt.tris++;
-        const uint32_t obj = __float_as_uint(c.y);
-        if (obj == r.skip) continue;
-        const f3 v0 = mk(a.x, a.y, a.z), v1 = mk(a.w, b.x, b.y), v2 = mk(b.z, b.w, c.x);
-        const float t = tri_t(v0, v1, v2, r.o, r.d, mr);
-        if (t >= kBias) {
-          const uint32_t prim = __float_as_uint(c.z);
-          if (better(t, obj, prim, best)) {
-            best.t = t;
-            best.slot = s;
-            best.obj = obj;
-            best.prim = prim;
-            if (ANY) return best;
-          }
+      t.cur = ck[0];
+      return true;
+    }
+  } else {
+    const uint32_t first = cur & 0x07FFFFFFu;
+    const uint32_t count = ((cur >> 27) & 7u) + 1u;
+    for (uint32_t i = 0; i < count; i++) {
+      const uint32_t s = first + i;
+      const float4 a = sc.tris[3 * s + 0];
+      const float4 b = sc.tris[3 * s + 1];
+      const float4 c = sc.tris[3 * s + 2];
+      if (COUNT) cnt.tris++;
+      const uint32_t obj = __float_as_uint(c.y);
+      if (obj == t.r.skip) continue;
+      const f3 v0 = mk(a.x, a.y, a.z), v1 = mk(a.w, b.x, b.y), v2 = mk(b.z, b.w, c.x);
+      const float th = tri_t(v0, v1, v2, t.r.o, t.r.d, t.mr);
+      if (th >= kBias) {
+        const uint32_t prim = __float_as_uint(c.z);
+        if (better(th, obj, prim, t.best)) {
+          t.best.t = th;
+          t.best.slot = s;
+          t.best.obj = obj;
+          t.best.prim = prim;
+          if (ANY) return false;
         }
       }
     }
-    if (sp == 0) break;
-    sp--;
-    cur = stk[sp * 64];
   }
-  return best;
+  if (t.sp == 0) return false;
+  t.sp--;
+  t.cur = stk.get(t.sp);
+  return true;
+}
+
+// The whole traversal of one ray.  `stk` points at this lane's LDS stack
+// column: entry k at stk[k*64].
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, uint32_t *stk, TravCounts &cnt,
+                                     bool &overflow) {
+  Trav t;
+  const LdsStack st{stk};
+  if (trav_begin<ANY>(sc, r, t))
+    while (trav_step<ANY, COUNT>(sc, t, st, cnt, overflow)) {
+    }
+  return t.best;
+}
+
+// Persistent traversal of a queue of n rays with per-lane refill: a lane
+// whose ray has ended takes the next ray of the queue once at least
+// kRefill lanes of its wave are idle, so a wave keeps most lanes tracing
+// until the queue runs dry instead of waiting for its longest ray.
+// Block b starts with rays [64 b, 64 b + 64) (no atomic); later rays come
+// from *fetch (ray count past 64 gridDim.x, one atomic per refill; zero at
+// launch).  load(i) returns ray i; done(i, hit, overflow, counts) consumes
+// its result (counts: nodes and triangles of this ray when COUNT).
+constexpr uint32_t kRefill = 32;
+template <bool ANY, bool COUNT, class Stack, class Load, class Done>
+__device__ __forceinline__ void trace_queue(const DevScene &sc, uint32_t n, uint32_t *fetch, const Stack &stk,
+                                            Load load, Done done) {
+  const uint32_t lane = threadIdx.x % 64;
+  TravCounts cnt{0, 0};
+  Trav t;
+  bool active = false, ovf = false;
+  uint32_t idx = 0;
+  uint32_t next = blockIdx.x * 64u;  // wave-uniform: first index of the next refill
+  bool first = true;
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (next < n && nidle >= (first ? 64u : kRefill)) {
+      uint32_t base = next;
+      if (!first) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(fetch, nidle);
+        base = 64u * gridDim.x + __shfl(k, 0, 64);
+      }
+      first = false;
+      if (!active) {
+        idx = base + __popcll(idle & ((uint64_t(1) << lane) - 1));
+        if (idx < n) {
+          ovf = false;
+          cnt = TravCounts{0, 0};
+          active = trav_begin<ANY>(sc, load(idx), t);
+          if (!active) done(idx, t.best, false, cnt);
+        }
+      }
+      next = base + nidle;  // >= n: the queue has run dry
+    } else if (nidle == 64) {
+      break;
+    }
+    if (active) {
+      active = trav_step<ANY, COUNT>(sc, t, stk, cnt, ovf);
+      if (!active) done(idx, t.best, ovf, cnt);
+    }
+  }
 }
 
 }  // namespace lam

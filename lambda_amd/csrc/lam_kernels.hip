@@ -6,43 +6,46 @@
 namespace lam {
 
 // ---------------------------------------------------------------------------
-// extend / occlusion over a ray batch
+// extend / occlusion over a ray batch: the persistent queue traversal of the
+// wavefront's extend and shadow kernels (split stacks, per-lane refill)
 // ---------------------------------------------------------------------------
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *__restrict__ rays, uint32_t n,
                                                    lam_hit *__restrict__ hits, uint32_t *__restrict__ occl,
-                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ err) {
-  __shared__ uint32_t stack[kStack * 64];
-  const uint32_t lane = threadIdx.x;
-  const uint32_t i = blockIdx.x * 64u + lane;
-  if (i >= n) return;
-  const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-  const float4 a = rp[0], b = rp[1];
-  Ray r;
-  r.o = mk(a.x, a.y, a.z);
-  r.tmax = a.w;
-  r.d = mk(b.x, b.y, b.z);
-  r.skip = __float_as_uint(b.w);
-  TravCounts c{0, 0};
-  bool ovf = false;
-  const Hit h = trace<ANY, COUNT>(sc, r, stack + lane, c, ovf);
-  if (ovf) atomicOr(err, 1u);
-  if (ANY) {
-    occl[i] = h.obj != LAM_NONE ? 1u : 0u;
-  } else {
-    uint32_t tri = LAM_NONE;
-    if (h.slot != LAM_NONE) tri = __float_as_uint(sc.tris[3 * h.slot + 2].w);
-    float4 o;
-    o.x = h.obj == LAM_NONE ? kFltMax : h.t;
-    o.y = __uint_as_float(tri);
-    o.z = __uint_as_float(h.obj);
-    o.w = __uint_as_float(h.obj == LAM_NONE ? LAM_NONE : h.prim);
-    reinterpret_cast<float4 *>(hits)[i] = o;
-  }
-  if (COUNT) {
-    counts[2 * i + 0] = c.nodes;
-    counts[2 * i + 1] = c.tris;
-  }
+                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ err,
+                                                   uint32_t *__restrict__ fetch, uint32_t *__restrict__ spill) {
+  __shared__ uint32_t stack[kLdsStack * 64];
+  auto load = [&](uint32_t i) {
+    const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    Ray r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmax = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.skip = __float_as_uint(b.w);
+    return r;
+  };
+  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &c) {
+    if (ovf) atomicOr(err, 1u);
+    if (ANY) {
+      occl[i] = h.obj != LAM_NONE ? 1u : 0u;
+    } else {
+      uint32_t tri = LAM_NONE;
+      if (h.slot != LAM_NONE) tri = __float_as_uint(sc.tris[3 * h.slot + 2].w);
+      float4 o;
+      o.x = h.obj == LAM_NONE ? kFltMax : h.t;
+      o.y = __uint_as_float(tri);
+      o.z = __uint_as_float(h.obj);
+      o.w = __uint_as_float(h.obj == LAM_NONE ? LAM_NONE : h.prim);
+      reinterpret_cast<float4 *>(hits)[i] = o;
+    }
+    if (COUNT) {
+      counts[2 * i + 0] = c.nodes;
+      counts[2 * i + 1] = c.tris;
+    }
+  };
+  const SplitStack stk{stack + threadIdx.x, spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  trace_queue<ANY, COUNT>(sc, n, fetch, stk, load, done);
 }
 
 // ---------------------------------------------------------------------------
@@ -159,15 +162,19 @@ __global__ void k_zero(float4 *__restrict__ p, size_t n4) {
 static inline uint32_t blocks64(size_t n) { return uint32_t((n + 63) / 64); }
 
 hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, lam_hit *hits, uint32_t *occl,
-                             uint32_t *counts, uint32_t *err, hipStream_t st) {
+                             uint32_t *counts, uint32_t *err, uint32_t *fetch, uint32_t *spill, uint32_t blocks,
+                             hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const dim3 g(blocks64(n)), b(64);
+  const dim3 g(std::min(blocks64(n), blocks)), b(64);
   if (occl) {
-    hipLaunchKernelGGL((k_trace_rays<true, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+    hipLaunchKernelGGL((k_trace_rays<true, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err,
+                       fetch, spill);
   } else if (counts) {
-    hipLaunchKernelGGL((k_trace_rays<false, true>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+    hipLaunchKernelGGL((k_trace_rays<false, true>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err,
+                       fetch, spill);
   } else {
-    hipLaunchKernelGGL((k_trace_rays<false, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err);
+    hipLaunchKernelGGL((k_trace_rays<false, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err,
+                       fetch, spill);
   }
   return hipGetLastError();
 }

@@ -15,8 +15,12 @@ struct PathArgs {
   float sample_p;
 };
 
+// Closest hit (occl null) or occlusion of n rays with the persistent queue
+// traversal on min(blocks, n/64) blocks: *fetch must be zero at launch,
+// spill holds (kStack - kLdsStack) * 64 * blocks words.
 hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, lam_hit *hits, uint32_t *occl,
-                             uint32_t *counts, uint32_t *err, hipStream_t st);
+                             uint32_t *counts, uint32_t *err, uint32_t *fetch, uint32_t *spill, uint32_t blocks,
+                             hipStream_t st);
 hipError_t launch_trace_paths(const DevScene &sc, const PathArgs &pa, int hero, const uint32_t *pixel,
                               const uint32_t *sample, size_t n, float *out, uint32_t *err, hipStream_t st);
 hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, uint32_t sample, uint32_t npix,
@@ -37,13 +41,18 @@ struct WaveState {
   void *sray;     // shadow queue, 32 B per ray (o, tmax | d, skip)
   uint32_t *spath;
   size_t cap;
+  // traversal-stack spill columns of the extend / shadow grid: entry k >=
+  // kLdsStack of lane g at spill[(k - kLdsStack) * spill_lanes + g]
+  uint32_t *spill;
+  uint32_t spill_lanes;
 };
 struct WaveLaunch {
-  uint32_t trace_blocks;    // persistent extend / shadow / finish grid (64-lane blocks)
+  uint32_t trace_blocks;    // persistent extend / shadow grid (64-lane blocks, split stacks)
   uint32_t shade_blocks;    // shade grid (256-lane blocks)
   uint32_t tail_threshold;  // queue length at or below which k_finish takes over
   uint32_t max_iters;       // 0: pure wavefront; else cap on bounce iterations per pass
   uint32_t guide_blocks;    // guiding-network grid (256-lane blocks, 51 KB LDS each)
+  uint32_t finish_blocks;   // persistent finish grid (64-lane blocks, 16 KB LDS stacks)
 };
 // bounce iterations enqueued per pass
 inline uint32_t wave_iterations(uint32_t max_scat, const WaveLaunch &wl) {

@@ -128,37 +128,38 @@ __device__ __forceinline__ uint32_t next_chunk(uint32_t *ctr) {
 // ---- extend: persistent closest hit ---------------------------------------------------
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
-                                               uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ counts) {
-  __shared__ uint32_t stack[kStack * 64];
-  const uint32_t lane = threadIdx.x;
+                                               uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ spill,
+                                               uint32_t *__restrict__ counts) {
+  __shared__ uint32_t stack[kLdsStack * 64];
   if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
-  uint32_t *ctr = ctrl + CTRL_FETCH_EXT;
-  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctr)) {
-    const uint32_t base = chunk * 64u;
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
-      Ray r;
-      r.o = mk(a.x, a.y, a.z);
-      r.tmax = a.w;
-      r.d = mk(b.x, b.y, b.z);
-      r.skip = LAM_NONE;
-      TravCounts c{0, 0};
-      bool ovf = false;
-      Hit h = trace<false, COUNT>(sc, r, stack + lane, c, ovf);
-      if (ovf) {  // cannot happen under the host-checked stack bound
-        atomicOr(ctrl + 15, 1u);
-        h.obj = LAM_NONE;
-      }
-      hits[i] = h;
-      if (COUNT) {
-        atomicAdd(counts + 0, c.nodes);
-        atomicAdd(counts + 1, c.tris);
-        atomicAdd(counts + 2, 1u);
-      }
+  TravCounts c{0, 0};
+  uint32_t nrays = 0;
+  auto load = [&](uint32_t i) {
+    const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
+    Ray r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmax = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.skip = LAM_NONE;
+    return r;
+  };
+  auto done = [&](uint32_t i, Hit h, bool ovf, const TravCounts &rc) {
+    if (ovf) {  // cannot happen under the host-checked stack bound
+      atomicOr(ctrl + 15, 1u);
+      h.obj = LAM_NONE;
     }
+    hits[i] = h;
+    c.nodes += rc.nodes;
+    c.tris += rc.tris;
+    nrays++;
+  };
+  const SplitStack stk{stack + threadIdx.x, spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  trace_queue<false, COUNT>(sc, n, ctrl + CTRL_FETCH_EXT, stk, load, done);
+  if (COUNT) {
+    atomicAdd(counts + 0, c.nodes);
+    atomicAdd(counts + 1, c.tris);
+    atomicAdd(counts + 2, nrays);
   }
 }
 
@@ -177,8 +178,12 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
   return base + rank;
 }
 
+// H = 4: at least 3 waves per SIMD; the register budget then stays under
+// 168 VGPRs without spills (2 waves at the default allocation), and the
+// latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
+// whole register file (1 wave).
 template <int H>
-__global__ __launch_bounds__(256) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
+__global__ __launch_bounds__(256, H == 4 ? 3 : 1) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
@@ -617,42 +622,43 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
                                                uint32_t *__restrict__ counts, float4 *__restrict__ rec = nullptr,
                                                const uint32_t *__restrict__ rec_of_slot = nullptr,
                                                uint32_t rec_cap = 0) {
-  __shared__ uint32_t stack[kStack * 64];
-  const uint32_t lane = threadIdx.x;
+  __shared__ uint32_t stack[kLdsStack * 64];
   if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_SHADOW];
   const float4 *__restrict__ sr = reinterpret_cast<const float4 *>(ws.sray);
   const size_t cap = ws.cap;
-  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctrl + CTRL_FETCH_SHD)) {
-    const uint32_t base = chunk * 64u;
-    if (base >= n) break;
-    const uint32_t i = base + lane;
-    if (i < n) {
-      const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
-      Ray r;
-      r.o = mk(a.x, a.y, a.z);
-      r.tmax = a.w;
-      r.d = mk(b.x, b.y, b.z);
-      r.skip = __float_as_uint(b.w);
-      TravCounts c{0, 0};
-      bool ovf = false;
-      const Hit h = trace<true, COUNT>(sc, r, stack + lane, c, ovf);
-      if (ovf) atomicOr(ctrl + 15, 1u);
-      if (h.obj == LAM_NONE && !ovf) {
-        const uint32_t p = ws.spath[i];
+  TravCounts c{0, 0};
+  uint32_t nrays = 0;
+  auto load = [&](uint32_t i) {
+    const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
+    Ray r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmax = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.skip = __float_as_uint(b.w);
+    return r;
+  };
+  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &rc) {
+    if (ovf) atomicOr(ctrl + 15, 1u);
+    if (h.obj == LAM_NONE && !ovf) {
+      const uint32_t p = ws.spath[i];
 #pragma unroll
-        for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += ws.pend[size_t(k) * cap + p];
-        if (REC) {  // the vertex's training record: its NEE term reached L
-          const uint32_t r = rec_of_slot[p];
-          if (r < rec_cap) rec[4 * size_t(r) + 3].y = 1.f;
-        }
-      }
-      if (COUNT) {
-        atomicAdd(counts + 3, c.nodes);
-        atomicAdd(counts + 4, c.tris);
-        atomicAdd(counts + 5, 1u);
+      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += ws.pend[size_t(k) * cap + p];
+      if (REC) {  // the vertex's training record: its NEE term reached L
+        const uint32_t r = rec_of_slot[p];
+        if (r < rec_cap) rec[4 * size_t(r) + 3].y = 1.f;
       }
     }
+    c.nodes += rc.nodes;
+    c.tris += rc.tris;
+    nrays++;
+  };
+  const SplitStack stk{stack + threadIdx.x, ws.spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  trace_queue<true, COUNT>(sc, n, ctrl + CTRL_FETCH_SHD, stk, load, done);
+  if (COUNT) {
+    atomicAdd(counts + 3, c.nodes);
+    atomicAdd(counts + 4, c.tris);
+    atomicAdd(counts + 5, nrays);
   }
 }
 
@@ -742,15 +748,15 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     if (prof) (void)hipEventRecord(prof->fin0[s], st);
     hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u, wl.tail_threshold, last ? 1u : 0u);
     if (wl.max_iters)
-      hipLaunchKernelGGL((k_finish<H>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, pa, ids, ws, s, q, ctrl);
+      hipLaunchKernelGGL((k_finish<H>), dim3(wl.finish_blocks), dim3(64), 0, st, sc, pa, ids, ws, s, q, ctrl);
     if (prof) {
       (void)hipEventRecord(prof->ext0[s], st);
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
-                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, nullptr);
+                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
       (void)hipEventRecord(prof->ext1[s], st);
     } else {
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
-                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, nullptr);
+                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
     }
     if (gd)
       hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, q, ctrl);

@@ -184,6 +184,31 @@ def box_room(out_dir, seed=1, width=256, hero=4):
     return path
 
 
+def sheet_stack(out_dir, seed=5, width=64, n_sheets=8192, hero=4):
+    """box_room with n_sheets parallel square sheets (0.8 m, facing the
+    camera) stacked along the view axis between y = -0.3 and y = 0.5.  The
+    BVH can separate them only along y, so every interior node a camera ray
+    descends through has all of its children on the ray: the traversal
+    pushes 7 entries per level (a stress case for the traversal stack)."""
+    path = box_room(out_dir, seed=seed, width=width, hero=hero)
+    with open(path) as f:
+        scene = json.load(f)
+    quads = []
+    for i in range(n_sheets):
+        y = -0.3 + 0.8 * i / n_sheets
+        quads.append(((0.4, y, 0.6), (-0.4, y, 0.6), (-0.4, y, 1.4), (0.4, y, 1.4)))  # normal -y
+    V, F, N = _mesh_from_quads(quads)
+    write_obj(os.path.join(out_dir, "sheets.obj"), V, F, N)
+    scene["objects"].append({"name": "sheets", "type": "trimesh", "transform": {}, "material": "mat_block_tall",
+                             "source": os.path.join(out_dir, "sheets.obj")})
+    # the tall block stands behind the sheets; drop it (its material stays)
+    scene["objects"] = [o for o in scene["objects"] if o["name"] != "block_tall"]
+    path = os.path.join(out_dir, "sheet_stack.nls")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
+
+
 def box_room_films(out_dir, seed=3, width=512, hero=4, n_films=64, film_freq=3, film_radius=0.06,
                    glossiness=32):
     os.makedirs(out_dir, exist_ok=True)

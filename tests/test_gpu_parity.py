@@ -178,6 +178,66 @@ def test_pipelined_passes_are_exact(ctx, box_scene, flags, spp, offset):
     assert np.array_equal(a, b)
 
 
+@pytest.fixture(scope="module")
+def sheets(tmp_path_factory):
+    from lambda_amd import scenes
+    return la.load_nls(scenes.sheet_stack(str(tmp_path_factory.mktemp("sheets")), n_sheets=8192, width=64))
+
+
+def test_deep_stacks_bit_exact(ctx, sheets):
+    """8,192 parallel sheets along the view axis: a ray down the stack pushes
+    7 entries per BVH level, past the 24 held in LDS into the spill columns.
+    Closest hits and occlusion stay bit-exact against the oracle; the
+    wavefront render (split stacks) equals the megakernel (LDS stacks)."""
+    ctx.upload(sheets)
+    assert ctx.stats().stack_bound > 24 and ctx.stats().depth >= 6
+    orc = ob.Oracle(sheets)
+    rng = np.random.default_rng(21)
+    n = 60000
+    rays = np.zeros(n, dtype=la.RAY_DTYPE)
+    rays["o"] = np.stack([rng.uniform(-0.35, 0.35, n), np.full(n, -0.9), rng.uniform(0.65, 1.35, n)], axis=1)
+    d = np.stack([rng.normal(0, 0.05, n), np.ones(n), rng.normal(0, 0.05, n)], axis=1)
+    rays["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmax"] = 3.4e38
+    rays["skip"] = la.LAM_NONE
+    o = orc.closest(rays)
+    _assert_hits_equal(ctx.extend(rays), o, "rays down the sheet stack")
+    sheet_obj = np.bincount(o["obj"][o["obj"] != la.LAM_NONE]).argmax()
+    assert (o["obj"] == sheet_obj).mean() > 0.9
+    # shadow rays between the sheets' first and last layer
+    rays["tmax"] = rng.uniform(0.0, 1.6, n).astype(np.float32)
+    assert np.array_equal(ctx.occluded(rays), orc.occluded(rays))
+    pr = la.params(spp=3, max_scatterings=6, sample_p=0.8, hero=4, seed=4)
+    a = ctx.render(la.params(spp=3, max_scatterings=6, sample_p=0.8, hero=4, seed=4, flags=la.LAM_FLAG_NO_TAIL))
+    b = ctx.render(la.params(spp=3, max_scatterings=6, sample_p=0.8, hero=4, seed=4, flags=la.LAM_FLAG_MEGAKERNEL))
+    assert np.array_equal(a, b) and np.array_equal(a, ctx.render(pr))
+    g, og = ctx.render(pr), orc.render(pr)
+    assert np.mean((g - og) ** 2) / np.mean(og ** 2) < 1e-8
+
+
+def test_wave_config_keeps_the_image(ctx, box_scene):
+    """lanes, tail threshold and bounce cap change the schedule, not the image"""
+    ctx.upload(box_scene)
+    pr = la.params(spp=7, max_scatterings=8, sample_p=0.8, hero=4, seed=11, spp_offset=2)
+    base = ctx.render(pr)
+    default = ctx.wave_config()
+    assert default == {"lanes": 3, "tail_threshold": 131072, "tail_div": 16, "max_iters": 12, "trace_per_cu": 24}
+    try:
+        for cfg in [dict(lanes=1), dict(lanes=2, tail_div=2, max_iters=3, trace_per_cu=1),
+                    dict(lanes=4, tail_threshold=0, trace_per_cu=32),
+                    dict(lanes=4, tail_threshold=1 << 30, tail_div=1, max_iters=1)]:
+            ctx.set_wave_config(**{**default, **cfg})
+            assert np.array_equal(ctx.render(pr), base), cfg
+        for bad in [dict(lanes=0), dict(lanes=5), dict(tail_div=0), dict(max_iters=0), dict(trace_per_cu=0),
+                    dict(trace_per_cu=33)]:
+            with pytest.raises(la.LambdaError) as e:
+                ctx.set_wave_config(**{**default, **bad})
+            assert e.value.code == la.LAM_E_INVALID
+    finally:
+        ctx.set_wave_config(**default)
+    assert ctx.wave_config() == default
+
+
 def test_pipelined_chunks_are_exact(ctx, tmp_path):
     """a frame of more than one chunk of path slots (4,410,000 pixels > 4 Mi):
     both lanes per chunk, the same image as one lane"""
