@@ -229,6 +229,8 @@ struct TravCounts {
 constexpr int kLdsStack = 24;
 struct LdsStack {
   uint32_t *lds;
+  // entries [0, top) all in LDS
+  __device__ __forceinline__ bool in_lds(int) const { return true; }
   __device__ __forceinline__ void put(int k, uint32_t v) const { lds[k * 64] = v; }
   __device__ __forceinline__ uint32_t get(int k) const { return lds[k * 64]; }
 };
@@ -236,6 +238,7 @@ struct SplitStack {
   uint32_t *lds;
   uint32_t *spill;
   uint32_t stride;
+  __device__ __forceinline__ bool in_lds(int top) const { return top <= kLdsStack; }
   __device__ __forceinline__ void put(int k, uint32_t v) const {
     if (k < kLdsStack)
       lds[k * 64] = v;
@@ -323,13 +326,15 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
     uint32_t ck[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
+      // origin + q * 2^e: the product is exact (8-bit q, power-of-two
+      // scale), so the fused form rounds once, exactly as the sum does
       const int w = k >> 2, sh = (k & 3) * 8;
-      const float lx = h0.x + float((qlo[0 + w] >> sh) & 0xffu) * sx;
-      const float ly = h0.y + float((qlo[2 + w] >> sh) & 0xffu) * sy;
-      const float lz = h0.z + float((qlo[4 + w] >> sh) & 0xffu) * sz;
-      const float hx = h0.x + float((qhi[0 + w] >> sh) & 0xffu) * sx;
-      const float hy = h0.y + float((qhi[2 + w] >> sh) & 0xffu) * sy;
-      const float hz = h0.z + float((qhi[4 + w] >> sh) & 0xffu) * sz;
+      const float lx = __builtin_fmaf(float((qlo[0 + w] >> sh) & 0xffu), sx, h0.x);
+      const float ly = __builtin_fmaf(float((qlo[2 + w] >> sh) & 0xffu), sy, h0.y);
+      const float lz = __builtin_fmaf(float((qlo[4 + w] >> sh) & 0xffu), sz, h0.z);
+      const float hx = __builtin_fmaf(float((qhi[0 + w] >> sh) & 0xffu), sx, h0.x);
+      const float hy = __builtin_fmaf(float((qhi[2 + w] >> sh) & 0xffu), sy, h0.y);
+      const float hz = __builtin_fmaf(float((qhi[4 + w] >> sh) & 0xffu), sz, h0.z);
       const float tx0 = (lx - o.x) * t.ix, tx1 = (hx - o.x) * t.ix;
       const float ty0 = (ly - o.y) * t.iy, ty1 = (hy - o.y) * t.iy;
       const float tz0 = (lz - o.z) * t.iz, tz1 = (hz - o.z) * t.iz;
@@ -351,17 +356,27 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
     cswap(tk[2], ck[2], tk[4], ck[4]); cswap(tk[3], ck[3], tk[5], ck[5]);
     cswap(tk[3], ck[3], tk[4], ck[4]);
     if (tk[0] != kInf) {
+      // the hit children are ck[0 .. nh-1] (misses sorted last as +inf):
+      // descend into ck[0], push the others farthest first (ck[nh-1] at
+      // sp, ..., ck[1] at sp + nh - 2)
+      int nh = 1;
 #pragma unroll
-      for (int k = 7; k >= 1; k--) {
-        if (tk[k] != kInf) {
-          if (t.sp >= kStack) {
-            overflow = true;
-            return false;
-          }
-          stk.put(t.sp, ck[k]);
-          t.sp++;
-        }
+      for (int k = 1; k < 8; k++) nh += tk[k] != kInf ? 1 : 0;
+      const int top = t.sp + nh - 1;
+      if (top > kStack) {
+        overflow = true;
+        return false;
       }
+      if (stk.in_lds(top)) {
+#pragma unroll
+        for (int k = 1; k < 8; k++)
+          if (k < nh) stk.lds[(top - k) * 64] = ck[k];
+      } else {
+#pragma unroll
+        for (int k = 1; k < 8; k++)
+          if (k < nh) stk.put(top - k, ck[k]);
+      }
+      t.sp = top;
       t.cur = ck[0];
       return true;
     }

@@ -14,8 +14,8 @@
 //                 fetches 64 rays with one atomic
 //   k_shade       hit -> surface, emitter, NEE light sample, BSDF sample,
 //                 MIS weights; appends the continuation ray and the shadow ray
-//                 to compacted queues (wave ballot + prefix count + one atomic
-//                 per wave per queue)
+//                 to compacted queues (wave ballots + a block prefix count in
+//                 LDS + one atomic per block iteration per queue)
 //   k_shadow      persistent any-hit; adds the pending NEE term when visible
 // then k_film splats each path's spectral radiance into the Kahan XYZ film.
 //
@@ -43,6 +43,9 @@ enum : uint32_t {
   CTRL_GUIDE_ZERO = 13,  // guide-branch samples with a zero mixture density
   CTRL_WORDS = 16
 };
+
+// threads per shade block (the block appends aggregate this many entries)
+constexpr uint32_t kShadeBlock = 256;
 
 struct PathIds {
   const uint32_t *pixel;   // nullable: then pixel = slot + pixel_offset
@@ -164,18 +167,49 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
 }
 
 // ---- shade ------------------------------------------------------------------------------
-// wave-aggregated append: returns this lane's slot in a queue whose count
-// lives at *count (valid only where want is true)
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
-  const uint64_t mask = __ballot(want);
-  const uint32_t lane = threadIdx.x % 64;
-  const uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
-  const uint32_t total = __popcll(mask);
-  uint32_t base = 0;
-  const uint32_t leader = mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 0;
-  if (want && rank == 0) base = atomicAdd(count, total);
-  base = __shfl(base, leader, 64);
-  return base + rank;
+// Block-aggregated append to NQ queues: one atomic per queue per block
+// iteration instead of one per wave.  Same-address atomics on the queue
+// counts are device-scope and serialise at the memory side across the 8
+// XCDs; per-wave appends cost the shade kernel more than half its time.
+// Every thread of the block calls this (two barriers).  The per-wave counts
+// are double-buffered by iteration parity `par`, so the next iteration may
+// start writing before every wave has read this one's offsets.  Within a
+// block, queue order is wave order then lane order.
+template <int WAVES, int NQ>
+struct BlockQueues {
+  uint32_t cnt[2][NQ][WAVES];
+  uint32_t base[2][NQ];
+};
+template <int WAVES, int NQ>
+__device__ __forceinline__ void block_append(BlockQueues<WAVES, NQ> &sm, uint32_t par, const bool (&want)[NQ],
+                                             uint32_t *const (&count)[NQ], uint32_t (&slot)[NQ]) {
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  uint64_t m[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) {
+    m[k] = __ballot(want[k]);
+    if (lane == 0) sm.cnt[par][k][wave] = __popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const uint32_t k = threadIdx.x;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; w++) {
+      const uint32_t c = sm.cnt[par][k][w];
+      sm.cnt[par][k][w] = tot;
+      tot += c;
+    }
+    uint32_t *ck = count[0];
+#pragma unroll
+    for (int j = 1; j < NQ; j++)
+      if (k == uint32_t(j)) ck = count[j];
+    sm.base[par][k] = tot ? atomicAdd(ck, tot) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NQ; k++) slot[k] = sm.base[par][k] + sm.cnt[par][k][wave] + __popcll(m[k] & below);
 }
 
 // H = 4: at least 3 waves per SIMD; the register budget then stays under
@@ -183,19 +217,20 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *count) {
 // latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
 // whole register file (1 wave).
 template <int H>
-__global__ __launch_bounds__(256, H == 4 ? 3 : 1) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
+__global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
   const size_t cap = ws.cap;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  // uniform trip count across the wave so the ballots see every lane
-  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t wave_first = first - (threadIdx.x % 64);
-  for (uint32_t wbase = wave_first; wbase < n; wbase += stride) {
-    const uint32_t q = wbase + (threadIdx.x % 64);
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  __shared__ BlockQueues<kShadeBlock / 64, 2> sq;
+  uint32_t par = 0;
+  // uniform trip count across the block: every thread takes part in the
+  // block appends
+  for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
+    const uint32_t q = bbase + threadIdx.x;
     bool push_next = false, push_shadow = false;
     uint32_t p = 0;
     Ray nxt, shd;
@@ -273,17 +308,17 @@ __global__ __launch_bounds__(256, H == 4 ? 3 : 1) void k_shade(DevScene sc, Path
         }
       }
     }
-    const uint32_t qn = wave_append(push_next, ctrl + CTRL_COUNT0 + (qsel ^ 1u));
+    uint32_t slot[2];
+    block_append(sq, par, {push_next, push_shadow}, {ctrl + CTRL_COUNT0 + (qsel ^ 1u), ctrl + CTRL_SHADOW}, slot);
     if (push_next) {
-      rout[2 * size_t(qn)] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
-      rout[2 * size_t(qn) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
+      rout[2 * size_t(slot[0])] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
+      rout[2 * size_t(slot[0]) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
     }
-    const uint32_t qs = wave_append(push_shadow, ctrl + CTRL_SHADOW);
     if (push_shadow) {
       float4 *sr = reinterpret_cast<float4 *>(ws.sray);
-      sr[2 * size_t(qs)] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
-      sr[2 * size_t(qs) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
-      ws.spath[qs] = p;
+      sr[2 * size_t(slot[1])] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
+      sr[2 * size_t(slot[1]) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
+      ws.spath[slot[1]] = p;
     }
   }
 }
@@ -383,7 +418,7 @@ __device__ inline void load_mix(const uint4 *src, float *raw) {
 }
 
 template <int H, bool TRAIN>
-__global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+__global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
                                                       GuideDev gd, uint32_t s, uint32_t qsel,
                                                       uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
@@ -391,11 +426,11 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
   const size_t cap = ws.cap;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t wave_first = first - (threadIdx.x % 64);
-  for (uint32_t wbase = wave_first; wbase < n; wbase += stride) {
-    const uint32_t q = wbase + (threadIdx.x % 64);
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  __shared__ BlockQueues<kShadeBlock / 64, TRAIN ? 3 : 2> sq;
+  uint32_t par = 0;
+  for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
+    const uint32_t q = bbase + threadIdx.x;
     bool push_next = false, push_shadow = false, live = false, guided = false, want_rec = false;
     uint32_t p = 0, mat_id = 0;
     Ray nxt, shd;
@@ -563,8 +598,14 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
         for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += pend[k];
       }
     }
-    if (TRAIN) {
-      const uint32_t r = wave_append(want_rec, gd.rec_count);
+    uint32_t slot[TRAIN ? 3 : 2];
+    uint32_t *const next_count = ctrl + CTRL_COUNT0 + (qsel ^ 1u);
+    if constexpr (TRAIN)
+      block_append(sq, par, {push_next, push_shadow, want_rec}, {next_count, ctrl + CTRL_SHADOW, gd.rec_count}, slot);
+    else
+      block_append(sq, par, {push_next, push_shadow}, {next_count, ctrl + CTRL_SHADOW}, slot);
+    if constexpr (TRAIN) {
+      const uint32_t r = slot[TRAIN ? 2 : 0];
       if (live) gd.rec_of_slot[p] = (want_rec && r < gd.rec_cap) ? r : LAM_NONE;
       if (want_rec && r < gd.rec_cap) {
         float4 *R = gd.rec + 4 * size_t(r);
@@ -574,17 +615,15 @@ __global__ __launch_bounds__(256) void k_shade_guided(DevScene sc, PathArgs pa, 
         R[3] = make_float4(__uint_as_float(p), rec_v, 0.f, 0.f);
       }
     }
-    const uint32_t qn = wave_append(push_next, ctrl + CTRL_COUNT0 + (qsel ^ 1u));
     if (push_next) {
-      rout[2 * size_t(qn)] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
-      rout[2 * size_t(qn) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
+      rout[2 * size_t(slot[0])] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
+      rout[2 * size_t(slot[0]) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
     }
-    const uint32_t qs = wave_append(push_shadow, ctrl + CTRL_SHADOW);
     if (push_shadow) {
       float4 *sr = reinterpret_cast<float4 *>(ws.sray);
-      sr[2 * size_t(qs)] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
-      sr[2 * size_t(qs) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
-      ws.spath[qs] = p;
+      sr[2 * size_t(slot[1])] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
+      sr[2 * size_t(slot[1]) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
+      ws.spath[slot[1]] = p;
     }
   }
 }
@@ -761,12 +800,12 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     if (gd)
       hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, q, ctrl);
     if (!gd)
-      hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, s, q, ctrl);
+      hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s, q, ctrl);
     else if (train)
-      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd,
+      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gd,
                          s, q, ctrl);
     else
-      hipLaunchKernelGGL((k_shade_guided<H, false>), dim3(wl.shade_blocks), dim3(256), 0, st, sc, pa, ids, ws,
+      hipLaunchKernelGGL((k_shade_guided<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws,
                          *gd, s, q, ctrl);
     if (prof) (void)hipEventRecord(prof->shd0[s], st);
     if (gd && train)
