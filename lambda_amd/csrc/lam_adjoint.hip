@@ -137,10 +137,10 @@ __device__ bool differentiable(const DiffDev &dv, const lam_material &m, uint32_
 
 // Replays one path: Ltot is its total radiance, a[k] the adjoint weight of
 // wavelength k (∂loss/∂L_k).
-template <int H>
+template <int H, class Stack>
 __device__ void adjoint_path(const DevScene &sc, const DiffDev &dv, uint32_t max_scat, float sample_p, uint64_t key,
-                             const Lam<H> &lam, Ray ray, const float *Ltot, const float *a, float *gl, uint32_t *stk,
-                             bool &ovf) {
+                             const Lam<H> &lam, Ray ray, const float *Ltot, const float *a, float *gl,
+                             const Stack &stk, bool &ovf) {
   float beta[H], Lrem[H];
 #pragma unroll
   for (int k = 0; k < H; k++) { beta[k] = 1.f; Lrem[k] = Ltot[k]; }
@@ -218,10 +218,12 @@ __device__ void adjoint_path(const DevScene &sc, const DiffDev &dv, uint32_t max
 template <int H>
 __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDev dv, uint32_t sample, uint32_t npix,
                                                 float inv_n, const float *__restrict__ dloss, float *__restrict__ grad,
-                                                uint32_t *__restrict__ ctr, uint32_t *__restrict__ err) {
-  __shared__ uint32_t stack[kStack * 64];
+                                                uint32_t *__restrict__ ctr, uint32_t *__restrict__ err,
+                                                uint32_t *__restrict__ spill) {
+  __shared__ uint32_t stack[kLdsStack * 64];
   extern __shared__ float gl[];
   const uint32_t lane = threadIdx.x;
+  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
   for (uint32_t j = lane; j < dv.n_params; j += 64) gl[j] = 0.f;
   __syncthreads();
   const uint32_t w = sc.cam.width, h = sc.cam.height;
@@ -260,8 +262,8 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
 #pragma unroll
         for (int k = 0; k < H; k++) { Lt[k] = 0.f; beta[k] = 1.f; }
         bool ovf = false;
-        path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, 0u, ray, beta, Lt, stack + lane, ovf);
-        adjoint_path<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, ray, Lt, a, gl, stack + lane, ovf);
+        path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, 0u, ray, beta, Lt, stk, ovf);
+        adjoint_path<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, ray, Lt, a, gl, stk, ovf);
         if (ovf) atomicOr(err, 1u);
       }
     }
@@ -277,15 +279,15 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
 hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
                           const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
                           uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
-                          uint32_t blocks, hipStream_t st) {
+                          uint32_t *spill, uint32_t blocks, hipStream_t st) {
   DiffDev dv{mat_d, mat_ior, mat_albedo, n_params};
   const size_t shmem = size_t(n_params) * 4;
   if (hero == 16)
     hipLaunchKernelGGL((k_adjoint<16>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, npix, inv_n, dloss,
-                       grad, ctr, err);
+                       grad, ctr, err, spill);
   else
     hipLaunchKernelGGL((k_adjoint<4>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, npix, inv_n, dloss,
-                       grad, ctr, err);
+                       grad, ctr, err, spill);
   return hipGetLastError();
 }
 

@@ -188,10 +188,12 @@ int ensure_launch(lam_ctx *ctx) {
   return LAM_OK;
 }
 
-// Traversal-stack spill columns of lane l for the current trace grid.
+// Traversal-stack spill columns of lane l for its largest grid: the trace
+// grid (extend, shadow, batch traces) or the finish grid (finish, and on
+// lane 0 the adjoint, megakernel and path-trace hooks, capped at it).
 int ensure_spill(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
-  const uint32_t glanes = ctx->wl.trace_blocks * 64u;
+  const uint32_t glanes = std::max(ctx->wl.trace_blocks, ctx->wl.finish_blocks) * 64u;
   if (ln.spill_lanes < glanes) {
     if (ln.spill) HIPC(hipFree(ln.spill));
     ln.spill = nullptr;
@@ -887,7 +889,9 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pi
   const uint32_t *dsmp = static_cast<uint32_t *>(ctx->scratch[1]);
   float *dout = static_cast<float *>(ctx->scratch[2]);
   if (p->flags & LAM_FLAG_MEGAKERNEL) {
-    HIPC(lam::launch_trace_paths(ctx->sc, path_args(p), int(H), dpix, dsmp, n, dout, ctx->d_err, ctx->stream));
+    if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
+    HIPC(lam::launch_trace_paths(ctx->sc, path_args(p), int(H), dpix, dsmp, n, dout, ctx->d_err, ctx->lane[0].spill,
+                                 ctx->wl.finish_blocks, ctx->stream));
   } else {
     const size_t cap = std::min(n, kMaxWaveCap);
     if ((rc = ensure_wave(ctx, cap, int(H)))) return rc;
@@ -923,9 +927,10 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   const lam::PathArgs pa = path_args(p);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   if (p->flags & LAM_FLAG_MEGAKERNEL) {
+    if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
     for (uint32_t s = 0; s < p->spp; s++)
       HIPC(lam::launch_render_mega(ctx->sc, pa, int(p->hero), p->spp_offset + s, uint32_t(npix), film,
-                                   ctx->d_err, ctx->stream));
+                                   ctx->d_err, ctx->lane[0].spill, ctx->wl.finish_blocks, ctx->stream));
   } else if ((rc = render_wavefront(ctx, p, film, npix))) {
     return rc;
   }
@@ -978,16 +983,16 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   uint32_t *ctr = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ctx->scratch[2]) + gbytes);
   HIPC(lam::launch_zero(dgrad, gbytes, ctx->stream));
   if ((rc = clear_err_flag(ctx))) return rc;
-  int cus = 0;
-  HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  const uint32_t blocks = uint32_t(cus > 0 ? cus : 256) * 8u;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
+  // 8 blocks per CU (within the finish grid the spill columns are sized for)
+  const uint32_t blocks = std::min(uint32_t(ctx->cus) * 8u, ctx->wl.finish_blocks);
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   for (uint32_t s = 0; s < p->spp; s++) {
     HIPC(lam::launch_zero(ctr, 16, ctx->stream));
     HIPC(lam::launch_adjoint(int(p->hero), ctx->sc, path_args(p), dmaps, dmaps + nm, dmaps + 2 * nm, dp->n_params,
                              p->spp_offset + s, uint32_t(npix), inv_n, static_cast<float *>(ctx->scratch[1]), dgrad,
-                             ctr, ctx->d_err, blocks, ctx->stream));
+                             ctr, ctx->d_err, ctx->lane[0].spill, blocks, ctx->stream));
   }
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipMemcpyAsync(grad, dgrad, size_t(dp->n_params) * 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -21,7 +21,7 @@ constexpr float kEps = 1.1920928955078125e-07f;  // ε<float>
 constexpr float kFltMax = 3.40282346638528859812e+38f;
 constexpr float kInvPi = 0.318309886183790671538f;  // inv_π<float>
 constexpr float kPi = 3.14159265358979323846f;
-constexpr int kStack = 64;  // traversal stack entries per lane (LDS)
+constexpr int kStack = 128;  // traversal stack entries per lane (LDS part + HBM spill)
 constexpr uint32_t kLeafBit = 0x80000000u;
 
 // ---- small vector type with the reference's operator semantics -----------
@@ -218,27 +218,27 @@ struct TravCounts {
   uint32_t nodes, tris;
 };
 
-// Traversal stacks; entry k of a lane's stack lives at:
-//   LdsStack    the lane's LDS column, lds[k*64] (kStack entries, 16 KB per
-//               64-lane block);
-//   SplitStack  the LDS column for k < kLdsStack, else the lane's column of
-//               a global spill buffer, spill[(k - kLdsStack) * stride].
+// Traversal stack of one lane: entry k lives in the lane's LDS column,
+// lds[k*64], for k < kLdsStack, else in the lane's column of a global spill
+// buffer, spill[(k - kLdsStack) * stride] (stride = lanes of the grid).
 // Deep entries are rare (a push needs several overlapping children at every
 // level above), so the split stack trades a few global accesses for a block
-// footprint of kLdsStack*256 B and ~2.5x the resident waves per CU.
+// footprint of kLdsStack*256 B (6 KB) instead of kStack*256 B.
 constexpr int kLdsStack = 24;
-struct LdsStack {
-  uint32_t *lds;
-  // entries [0, top) all in LDS
-  __device__ __forceinline__ bool in_lds(int) const { return true; }
-  __device__ __forceinline__ void put(int k, uint32_t v) const { lds[k * 64] = v; }
-  __device__ __forceinline__ uint32_t get(int k) const { return lds[k * 64]; }
-};
 struct SplitStack {
-  uint32_t *lds;
-  uint32_t *spill;
-  uint32_t stride;
+  uint32_t *lds;    // this lane's LDS column: entry k at lds[k*64]
+  uint32_t *spill;  // this lane's spill column: entry k at spill[(k - kLdsStack)*stride]
+  uint32_t stride;  // lanes of the grid
+  // the stack of lane threadIdx.x % 64 of block `block` in a grid of
+  // `blocks` 64-lane blocks (the spill buffer holds kStack - kLdsStack
+  // words per grid lane)
+  __device__ __forceinline__ static SplitStack of(uint32_t *lds_base, uint32_t *spill_base, uint32_t block,
+                                                  uint32_t blocks) {
+    const uint32_t lane = threadIdx.x % 64;
+    return SplitStack{lds_base + lane, spill_base + size_t(block) * 64 + lane, blocks * 64};
+  }
   __device__ __forceinline__ bool in_lds(int top) const { return top <= kLdsStack; }
+  __device__ __forceinline__ void put_lds(int k, uint32_t v) const { lds[k * 64] = v; }
   __device__ __forceinline__ void put(int k, uint32_t v) const {
     if (k < kLdsStack)
       lds[k * 64] = v;
@@ -252,7 +252,7 @@ struct SplitStack {
 
 // Traversal state of one ray: the ray, its slab-test constants, the best
 // hit so far, the current node and the stack depth (the stack itself is
-// the lane's LDS column).  Closest hit (ANY=false) or any hit (ANY=true)
+// the lane's SplitStack).  Closest hit (ANY=false) or any hit (ANY=true)
 // for t in [BIAS, tmax], objects `skip` excluded; ties go to the lowest
 // (obj, prim).
 struct Trav {
@@ -370,7 +370,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
       if (stk.in_lds(top)) {
 #pragma unroll
         for (int k = 1; k < 8; k++)
-          if (k < nh) stk.lds[(top - k) * 64] = ck[k];
+          if (k < nh) stk.put_lds(top - k, ck[k]);
       } else {
 #pragma unroll
         for (int k = 1; k < 8; k++)
@@ -411,15 +411,13 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
   return true;
 }
 
-// The whole traversal of one ray.  `stk` points at this lane's LDS stack
-// column: entry k at stk[k*64].
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, uint32_t *stk, TravCounts &cnt,
+// The whole traversal of one ray on the lane's stack.
+template <bool ANY, bool COUNT, class Stack>
+__device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, const Stack &stk, TravCounts &cnt,
                                      bool &overflow) {
   Trav t;
-  const LdsStack st{stk};
   if (trav_begin<ANY>(sc, r, t))
-    while (trav_step<ANY, COUNT>(sc, t, st, cnt, overflow)) {
+    while (trav_step<ANY, COUNT>(sc, t, stk, cnt, overflow)) {
     }
   return t.best;
 }

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *_
       counts[2 * i + 1] = c.tris;
     }
   };
-  const SplitStack stk{stack + threadIdx.x, spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
   trace_queue<ANY, COUNT>(sc, n, fetch, stk, load, done);
 }
 
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *_
 // ---------------------------------------------------------------------------
 template <int H>
 __device__ void trace_path(const DevScene &sc, const PathArgs &pa, uint32_t pixel, uint32_t sample, float *Lout,
-                           float &inv_omega, Lam<H> &lam, uint32_t *stk, bool &ovf) {
+                           float &inv_omega, Lam<H> &lam, const SplitStack &stk, bool &ovf) {
   const uint64_t key = lam_path_key(pa.seed, pixel, sample);
   const uint32_t w = sc.cam.width;
   const uint32_t row = pixel / w, colj = pixel - row * w;
@@ -75,41 +75,47 @@ __device__ void trace_path(const DevScene &sc, const PathArgs &pa, uint32_t pixe
   path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, 0u, ray, beta, Lout, stk, ovf);
 }
 
+// grid-stride over the paths; spill holds the traversal-stack spill
+// columns of the grid
 template <int H>
 __global__ __launch_bounds__(64) void k_trace_paths(DevScene sc, PathArgs pa, const uint32_t *__restrict__ pixel,
                                                     const uint32_t *__restrict__ sample, uint32_t n,
-                                                    float *__restrict__ out, uint32_t *__restrict__ err) {
-  __shared__ uint32_t stack[kStack * 64];
+                                                    float *__restrict__ out, uint32_t *__restrict__ err,
+                                                    uint32_t *__restrict__ spill) {
+  __shared__ uint32_t stack[kLdsStack * 64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t i = blockIdx.x * 64u + lane;
-  if (i >= n) return;
-  float L[H];
-  float inv_om;
-  Lam<H> lam;
-  bool ovf = false;
-  trace_path<H>(sc, pa, pixel[i], sample[i], L, inv_om, lam, stack + lane, ovf);
-  if (ovf) atomicOr(err, 1u);
+  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
+  for (uint32_t i = blockIdx.x * 64u + lane; i < n; i += gridDim.x * 64u) {
+    float L[H];
+    float inv_om;
+    Lam<H> lam;
+    bool ovf = false;
+    trace_path<H>(sc, pa, pixel[i], sample[i], L, inv_om, lam, stk, ovf);
+    if (ovf) atomicOr(err, 1u);
 #pragma unroll
-  for (int k = 0; k < H; k++) out[size_t(i) * H + k] = L[k];
+    for (int k = 0; k < H; k++) out[size_t(i) * H + k] = L[k];
+  }
 }
 
 template <int H>
 __global__ __launch_bounds__(64) void k_render_mega(DevScene sc, PathArgs pa, uint32_t sample, uint32_t npix,
-                                                    float *__restrict__ film, uint32_t *__restrict__ err) {
-  __shared__ uint32_t stack[kStack * 64];
+                                                    float *__restrict__ film, uint32_t *__restrict__ err,
+                                                    uint32_t *__restrict__ spill) {
+  __shared__ uint32_t stack[kLdsStack * 64];
   const uint32_t lane = threadIdx.x;
-  const uint32_t pixel = blockIdx.x * 64u + lane;
-  if (pixel >= npix) return;
-  float L[H];
-  float inv_om;
-  Lam<H> lam;
-  bool ovf = false;
-  trace_path<H>(sc, pa, pixel, sample, L, inv_om, lam, stack + lane, ovf);
-  if (ovf) atomicOr(err, 1u);
-  float X = 0.f, Y = 0.f, Z = 0.f;
+  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
+  for (uint32_t pixel = blockIdx.x * 64u + lane; pixel < npix; pixel += gridDim.x * 64u) {
+    float L[H];
+    float inv_om;
+    Lam<H> lam;
+    bool ovf = false;
+    trace_path<H>(sc, pa, pixel, sample, L, inv_om, lam, stk, ovf);
+    if (ovf) atomicOr(err, 1u);
+    float X = 0.f, Y = 0.f, Z = 0.f;
 #pragma unroll
-  for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z);
-  film_add(film, pixel, X, Y, Z);
+    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z);
+    film_add(film, pixel, X, Y, Z);
+  }
 }
 
 // coefλ2linRGB of the normalised film (src/renderer.cpp:52-56,
@@ -180,23 +186,24 @@ hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, 
 }
 
 hipError_t launch_trace_paths(const DevScene &sc, const PathArgs &pa, int hero, const uint32_t *pixel,
-                              const uint32_t *sample, size_t n, float *out, uint32_t *err, hipStream_t st) {
+                              const uint32_t *sample, size_t n, float *out, uint32_t *err, uint32_t *spill,
+                              uint32_t blocks, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const dim3 g(blocks64(n)), b(64);
+  const dim3 g(std::min(blocks64(n), blocks)), b(64);
   if (hero == 16)
-    hipLaunchKernelGGL((k_trace_paths<16>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err);
+    hipLaunchKernelGGL((k_trace_paths<16>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err, spill);
   else
-    hipLaunchKernelGGL((k_trace_paths<4>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err);
+    hipLaunchKernelGGL((k_trace_paths<4>), g, b, 0, st, sc, pa, pixel, sample, uint32_t(n), out, err, spill);
   return hipGetLastError();
 }
 
 hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, uint32_t sample, uint32_t npix,
-                              float *film, uint32_t *err, hipStream_t st) {
-  const dim3 g(blocks64(npix)), b(64);
+                              float *film, uint32_t *err, uint32_t *spill, uint32_t blocks, hipStream_t st) {
+  const dim3 g(std::min(blocks64(npix), blocks)), b(64);
   if (hero == 16)
-    hipLaunchKernelGGL((k_render_mega<16>), g, b, 0, st, sc, pa, sample, npix, film, err);
+    hipLaunchKernelGGL((k_render_mega<16>), g, b, 0, st, sc, pa, sample, npix, film, err, spill);
   else
-    hipLaunchKernelGGL((k_render_mega<4>), g, b, 0, st, sc, pa, sample, npix, film, err);
+    hipLaunchKernelGGL((k_render_mega<4>), g, b, 0, st, sc, pa, sample, npix, film, err, spill);
   return hipGetLastError();
 }
 

@@ -21,10 +21,13 @@ struct PathArgs {
 hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, lam_hit *hits, uint32_t *occl,
                              uint32_t *counts, uint32_t *err, uint32_t *fetch, uint32_t *spill, uint32_t blocks,
                              hipStream_t st);
+// One thread per path on min(blocks, n/64) blocks (grid-stride); spill
+// holds (kStack - kLdsStack) * 64 * blocks words (traversal-stack spill).
 hipError_t launch_trace_paths(const DevScene &sc, const PathArgs &pa, int hero, const uint32_t *pixel,
-                              const uint32_t *sample, size_t n, float *out, uint32_t *err, hipStream_t st);
+                              const uint32_t *sample, size_t n, float *out, uint32_t *err, uint32_t *spill,
+                              uint32_t blocks, hipStream_t st);
 hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, uint32_t sample, uint32_t npix,
-                              float *film, uint32_t *err, hipStream_t st);
+                              float *film, uint32_t *err, uint32_t *spill, uint32_t blocks, hipStream_t st);
 hipError_t launch_resolve(const DevScene &sc, const float *film, uint32_t w, uint32_t h, float inv_n, float *rgb,
                           float *xyz, hipStream_t st);
 // zeroes bytes / 16 float4 words at p (16-byte aligned; a tail below 16 bytes is left)
@@ -120,7 +123,7 @@ constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per blo
 hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
                           const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
                           uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
-                          uint32_t blocks, hipStream_t st);
+                          uint32_t *spill, uint32_t blocks, hipStream_t st);
 hipError_t enqueue_pass_next(uint32_t *ctrl, uint32_t stride, hipStream_t st);
 
 }  // namespace lam

@@ -574,9 +574,9 @@ __device__ __forceinline__ float hero_lambdas(const DevScene &sc, float xi, Lam<
 // equal to the reference up to floating-point reassociation.  Starting at
 // vertex s0 with (ray, β, L) lets the wavefront hand its tail of long paths
 // to one thread each.
-template <int H>
+template <int H, class Stack>
 __device__ void path_loop(const DevScene &sc, uint32_t max_scat, float sample_p, uint64_t key, const Lam<H> &lam,
-                          uint32_t s0, Ray ray, float *beta, float *L, uint32_t *stk, bool &ovf) {
+                          uint32_t s0, Ray ray, float *beta, float *L, const Stack &stk, bool &ovf) {
   TravCounts tc{0, 0};
   for (uint32_t s = s0;; s++) {
     const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
     c.tris += rc.tris;
     nrays++;
   };
-  const SplitStack stk{stack + threadIdx.x, spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
   trace_queue<false, COUNT>(sc, n, ctrl + CTRL_FETCH_EXT, stk, load, done);
   if (COUNT) {
     atomicAdd(counts + 0, c.nodes);
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
     c.tris += rc.tris;
     nrays++;
   };
-  const SplitStack stk{stack + threadIdx.x, ws.spill + blockIdx.x * 64 + threadIdx.x, gridDim.x * 64};
+  const SplitStack stk = SplitStack::of(stack, ws.spill, blockIdx.x, gridDim.x);
   trace_queue<true, COUNT>(sc, n, ctrl + CTRL_FETCH_SHD, stk, load, done);
   if (COUNT) {
     atomicAdd(counts + 3, c.nodes);
@@ -705,9 +705,10 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
 template <int H>
 __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
-  __shared__ uint32_t stack[kStack * 64];
+  __shared__ uint32_t stack[kLdsStack * 64];
   if (!ctrl[CTRL_TAIL]) return;
   const uint32_t lane = threadIdx.x;
+  const SplitStack stk = SplitStack::of(stack, ws.spill, blockIdx.x, gridDim.x);
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   const size_t cap = ws.cap;
@@ -735,7 +736,7 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
         L[k] = ws.L[size_t(k) * cap + p];
       }
       bool ovf = false;
-      path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stack + lane, ovf);
+      path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stk, ovf);
       if (ovf) atomicOr(ctrl + 15, 1u);
 #pragma unroll
       for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] = L[k];

@@ -447,7 +447,7 @@ def rounded_box(k, p=8.0):
     return V, F, N
 
 
-def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14):
+def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14, plate=1.0, plate_gap=0.35):
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     lam_ref = np.arange(355.0, 836.0, 5.0)
@@ -481,19 +481,20 @@ def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv
         write_obj(os.path.join(out_dir, f"{nm}.obj"), V, F, N)
         objects.append({"name": nm, "type": "trimesh", "transform": {}, "material": f"mat_{nm}",
                         "source": os.path.join(out_dir, f"{nm}.obj")})
-    # the light hangs under the ceiling at (lx, ly), facing down; a 2 m plate
-    # 0.3 m below hides it from the room, which is lit by the plate's and
-    # then the ceiling's bounce.  (The exponent is even: with an odd one the
+    # the light hangs under the ceiling at (lx, ly), facing down; a plate
+    # (half-width `plate`, 2 m by default) `plate_gap` below the ceiling
+    # hides it from the room, which is lit by the plate's and then the
+    # ceiling's bounce.  (The exponent is even: with an odd one the
     # reference's quad "dir" light gives negative radiance to points behind
     # it, cg.hpp:1625-1668, and the ceiling is.)
-    lx, ly, ph = 2.5, 2.5, 1.0
+    lx, ly, ph = 2.5, 2.5, plate
     V, F, N = _mesh_from_quads(_block(lx - ph, ly - ph, lx + ph, ly + ph, 0.05))
     write_obj(os.path.join(out_dir, "plate.obj"), V, F, N)
     write_csv(os.path.join(out_dir, "albedo_plate.csv"), lam_ref, 0.8 * np.ones_like(lam_ref))
     materials.append({"name": "mat_plate", "type": "lambertian",
                       "albedo": {"val": os.path.join(out_dir, "albedo_plate.csv")}})
     objects.append({"name": "plate", "type": "trimesh", "material": "mat_plate",
-                    "source": os.path.join(out_dir, "plate.obj"), "transform": {"translate": [0, 0, H - 0.35]}})
+                    "source": os.path.join(out_dir, "plate.obj"), "transform": {"translate": [0, 0, H - plate_gap]}})
     # glossy boxes
     V, F, N = rounded_box(box_subdiv)
     write_obj(os.path.join(out_dir, "rbox.obj"), V, F, N)

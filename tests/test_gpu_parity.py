@@ -215,6 +215,32 @@ def test_deep_stacks_bit_exact(ctx, sheets):
     assert np.mean((g - og) ** 2) / np.mean(og ** 2) < 1e-8
 
 
+def test_deep_bvh_uploads_and_renders(ctx, tmp_path):
+    """the glossy interior with a 1 m plate 0.2 m under the ceiling builds a
+    BVH whose traversal stack bound is 65 entries (above the 64 an all-LDS
+    stack held): it uploads, its hits are bit-exact and every traversal
+    kernel (trace queue, finish, megakernel, path hook) agrees"""
+    from lambda_amd import scenes
+    s = la.load_nls(scenes.glossy_interior(str(tmp_path), width=48, plate=0.5, plate_gap=0.2))
+    ctx.upload(s)
+    assert ctx.stats().stack_bound > 64
+    orc = ob.Oracle(s)
+    rng = np.random.default_rng(8)
+    rays = random_rays(rng, 50000, np.array([-4.9, -4.9, 0.1]), np.array([4.9, 4.9, 3.9]))
+    _assert_hits_equal(ctx.extend(rays), orc.closest(rays), "random rays in the deep-BVH room")
+    pr = la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=6)
+    a = ctx.render(pr)
+    b = ctx.render(la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=6, flags=la.LAM_FLAG_MEGAKERNEL))
+    c = ctx.render(la.params(spp=2, max_scatterings=8, sample_p=0.8, hero=4, seed=6, flags=la.LAM_FLAG_NO_TAIL))
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    o = orc.render(pr)
+    assert np.mean((a - o) ** 2) / np.mean(o ** 2) < 1e-8
+    pix = rng.integers(0, s.width * s.height, 2048).astype(np.uint32)
+    smp = rng.integers(0, 16, 2048).astype(np.uint32)
+    p1 = la.params(spp=1, max_scatterings=8, sample_p=0.8, hero=4, seed=6)
+    _check_paths(ctx.trace_paths(p1, pix, smp), orc.trace_paths(p1, pix, smp))
+
+
 def test_wave_config_keeps_the_image(ctx, box_scene):
     """lanes, tail threshold and bounce cap change the schedule, not the image"""
     ctx.upload(box_scene)
