@@ -210,7 +210,11 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
             "grad_finite": bool(np.isfinite(grad).all())}
 
 
-def guiding_bench(spp=128, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5):
+def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32):
+    """Learned proposal on the glossy interior: two independent plain runs
+    and two guided runs of `spp` samples per pixel (the guided runs train
+    the network online on their first `train_passes` passes, inside the timed
+    call); relMSE of one run from the difference of its two runs."""
     d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c4_seed{seed}_{width}")
     path = os.path.join(d, "glossy_interior.nls")
     if not os.path.exists(path):
@@ -238,7 +242,8 @@ def guiding_bench(spp=128, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5):
     guided, tg, tel = [], [], []
     for s in (1, 2):
         ctx.guide_reset(s)
-        gp = la.guide_params(alpha=alpha, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=s)
+        gp = la.guide_params(alpha=alpha, train_passes=train_passes, steps_per_pass=8, batch=16384, lr=5e-3,
+                             seed=s)
         t0 = time.perf_counter()
         guided.append(ctx.render_guided(pr(100 + s), gp))
         tg.append(time.perf_counter() - t0)
@@ -249,7 +254,7 @@ def guiding_bench(spp=128, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5):
     samples = scene.width * scene.height * spp
     return {"workload": f"glossy interior ({scene.n_tris:,} tris: room shell + 200 Blinn rounded boxes, "
                         f"hidden light), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
-            "spp_per_run": spp, "alpha": alpha,
+            "spp_per_run": spp, "train_passes": train_passes, "alpha": alpha,
             "plain": {"run_s": round(t_p, 3), "samples_per_s": round(samples / t_p, 1), "relmse": round(rp, 5),
                       "time_to_relmse_1e-3_s": round(t_p * rp / 1e-3, 2)},
             "guided": {"run_s": round(t_g, 3), "samples_per_s": round(samples / t_g, 1), "relmse": round(rg, 5),
@@ -274,7 +279,7 @@ def main():
     ap.add_argument("--no-guiding", action="store_true")
     ap.add_argument("--legs", default="render,roofline,adjoint,guiding,cpu",
                     help="comma list of legs to run (profiling runs pick one)")
-    ap.add_argument("--guide-spp", type=int, default=128)
+    ap.add_argument("--guide-spp", type=int, default=1024)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "

@@ -425,35 +425,57 @@ __device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, const Sta
 // Persistent traversal of a queue of n rays with per-lane refill: a lane
 // whose ray has ended takes the next ray of the queue once at least
 // kRefill lanes of its wave are idle, so a wave keeps most lanes tracing
-// until the queue runs dry instead of waiting for its longest ray.
-// Block b starts with rays [64 b, 64 b + 64) (no atomic); later rays come
-// from *fetch (ray count past 64 gridDim.x, one atomic per refill; zero at
-// launch).  load(i) returns ray i; done(i, hit, overflow, counts) consumes
-// its result (counts: nodes and triangles of this ray when COUNT).
-constexpr uint32_t kRefill = 32;
+// until the queue runs dry instead of waiting for its longest ray.  Ray
+// indices come from a wave-local pool: block b starts with rays
+// [64 b, 64 b + 64) (no atomic), and an exhausted pool is refilled with
+// `psize` consecutive indices from *fetch (ray count past 64 gridDim.x,
+// zero at launch), one device atomic per reservation.  The reservation
+// size gives each wave about kPoolRounds reservations of the queue's
+// remainder (32..kPool rays): large enough that the atomic's latency is
+// rare, small enough that the last reservations do not leave a few waves
+// with most of the work.  load(i) returns ray i; done(i, hit, overflow,
+// counts) consumes its result (counts: nodes and triangles of this ray
+// when COUNT).
+constexpr uint32_t kRefill = 16;
+constexpr uint32_t kPool = 256;
+constexpr uint32_t kPoolRounds = 2;
 template <bool ANY, bool COUNT, class Stack, class Load, class Done>
 __device__ __forceinline__ void trace_queue(const DevScene &sc, uint32_t n, uint32_t *fetch, const Stack &stk,
                                             Load load, Done done) {
   const uint32_t lane = threadIdx.x % 64;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
   TravCounts cnt{0, 0};
   Trav t;
   bool active = false, ovf = false;
   uint32_t idx = 0;
-  uint32_t next = blockIdx.x * 64u;  // wave-uniform: first index of the next refill
-  bool first = true;
+  // wave-uniform pool of reserved indices [pool, pool_end); dry: the queue
+  // has no unreserved rays left
+  uint32_t pool = blockIdx.x * 64u;
+  uint32_t pool_end = pool < n ? min(pool + 64u, n) : pool;
+  bool dry = pool >= n;
+  const uint32_t first_round = 64u * gridDim.x;
+  const uint32_t psize =
+      n > first_round ? min(kPool, max(32u, (n - first_round) / (gridDim.x * kPoolRounds) + 1u)) : 32u;
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (next < n && nidle >= (first ? 64u : kRefill)) {
-      uint32_t base = next;
-      if (!first) {
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(fetch, nidle);
-        base = 64u * gridDim.x + __shfl(k, 0, 64);
+    const uint32_t avail = pool_end - pool;
+    if (nidle == 64 && avail == 0 && dry) break;
+    if (nidle >= kRefill && (avail > 0 || !dry)) {
+      const uint32_t k1 = min(nidle, avail);  // from the current pool
+      uint32_t nb = 0, nb_cnt = 0;            // a new reservation
+      if (k1 < nidle && !dry) {
+        uint32_t r = 0;
+        if (lane == 0) r = atomicAdd(fetch, psize);
+        nb = 64u * gridDim.x + __shfl(r, 0, 64);
+        if (nb >= n)
+          dry = true;
+        else
+          nb_cnt = min(psize, n - nb);
       }
-      first = false;
       if (!active) {
-        idx = base + __popcll(idle & ((uint64_t(1) << lane) - 1));
+        const uint32_t rank = __popcll(idle & below);
+        idx = rank < k1 ? pool + rank : (rank - k1 < nb_cnt ? nb + (rank - k1) : n);
         if (idx < n) {
           ovf = false;
           cnt = TravCounts{0, 0};
@@ -461,9 +483,12 @@ __device__ __forceinline__ void trace_queue(const DevScene &sc, uint32_t n, uint
           if (!active) done(idx, t.best, false, cnt);
         }
       }
-      next = base + nidle;  // >= n: the queue has run dry
-    } else if (nidle == 64) {
-      break;
+      if (nb_cnt > 0) {
+        pool = nb + min(nidle - k1, nb_cnt);
+        pool_end = nb + nb_cnt;
+      } else {
+        pool += k1;
+      }
     }
     if (active) {
       active = trav_step<ANY, COUNT>(sc, t, stk, cnt, ovf);
