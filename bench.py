@@ -268,6 +268,60 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
                                "guided": round(float(lum(guided[0] + guided[1]).mean() / 2), 3)}}
 
 
+def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
+    """Combined stress: 640 noise-displaced icospheres (10,035,200 tris, 70%
+    thin-film materials) at 3840x2160, guided, then an adjoint over every
+    thin-film thickness at 1024x1024 (same geometry, another camera)."""
+    d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c5_seed{seed}")
+    d1 = os.path.join(tempfile.gettempdir(), f"lambda_amd_c5_seed{seed}_1024")
+    kw = dict(seed=seed, n_inst=640, material_p=(0.15, 0.15, 0.7))
+    p4k = os.path.join(d, "icosphere_field.nls")
+    p1k = os.path.join(d1, "icosphere_field.nls")
+    if not os.path.exists(p4k):
+        scenes.icosphere_field(d, width=3840, **kw)
+    if not os.path.exists(p1k):
+        scenes.icosphere_field(d1, width=1024, ar=1.0, **kw)
+    t0 = time.perf_counter()
+    scene = la.load_nls(p4k)
+    t_load = time.perf_counter() - t0
+    ctx = la.Context(0)
+    t0 = time.perf_counter()
+    ctx.upload(scene)
+    t_up = time.perf_counter() - t0
+    st = ctx.stats()
+    log(f"stress: {scene.n_tris} tris, load {t_load:.1f}s, upload (BVH build) {t_up:.1f}s, depth {st.depth}, "
+        f"stack bound {st.stack_bound}")
+    pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=1)
+    gp = la.guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1)
+    ctx.guide_reset(1)
+    t0 = time.perf_counter()
+    img = ctx.render_guided(pr, gp)
+    t_g = time.perf_counter() - t0
+    tm = ctx.timing()
+    s1k = la.load_nls(p1k)
+    ctx.set_camera(s1k)
+    films = [i for i, m in enumerate(s1k.materials()) if m.type == la.MAT_THINFILM]
+    dp = la.DiffParams(s1k, d=films)
+    pa = la.params(spp=adj_spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=3)
+    t0 = time.perf_counter()
+    fwd = ctx.render(pa)
+    t_f = time.perf_counter() - t0
+    # L2 loss against 0.9 x the image: dloss/drgb = 0.2 img
+    t0 = time.perf_counter()
+    grad = ctx.adjoint(pa, dp, (0.2 * fwd).astype(np.float32))
+    t_a = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": f"640 icospheres ({scene.n_tris:,} tris, 70% thin film), {scene.width}x{scene.height}, "
+                        f"{spp} spp guided (trained on the first 32), {bounces} bounces; adjoint over "
+                        f"{dp.n_params} thin-film thicknesses at 1024x1024, {adj_spp} spp",
+            "load_s": round(t_load, 2), "upload_s": round(t_up, 2), "bvh_depth": st.depth,
+            "stack_bound": st.stack_bound,
+            "guided_s": round(t_g, 3), "guided_samples_per_s": round(scene.width * scene.height * spp / t_g, 1),
+            "train_ms": round(tm.guide_train_ms, 1), "image_finite": bool(np.isfinite(img).all()),
+            "forward_s": round(t_f, 3), "adjoint_s": round(t_a, 3), "n_params": dp.n_params,
+            "grad_finite": bool(np.isfinite(grad).all()), "grad_nonzero": int(np.count_nonzero(grad))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,7 +332,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-guiding", action="store_true")
     ap.add_argument("--legs", default="render,roofline,adjoint,guiding,cpu",
-                    help="comma list of legs to run (profiling runs pick one)")
+                    help="comma list of legs to run (profiling runs pick one); 'stress' (not in the default) "
+                         "adds the 10 M-triangle 4K guided + adjoint run")
     ap.add_argument("--guide-spp", type=int, default=1024)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
@@ -300,6 +355,8 @@ def main():
         legs.discard("guiding")
     if "render" not in legs and not args.extend_only:
         out = {}
+        if "stress" in legs:
+            out["stress"] = stress_bench()
         if "adjoint" in legs:
             out["adjoint"] = adjoint_bench()
         if "guiding" in legs:
@@ -359,6 +416,9 @@ def main():
     if "guiding" in legs:
         out["guiding"] = guiding_bench(spp=args.guide_spp)
         log(f"guiding: {out['guiding']}")
+    if "stress" in legs:
+        out["stress"] = stress_bench()
+        log(f"stress: {out['stress']}")
     if "cpu" in legs:
         out["cpu_baseline"] = cpu_baseline(scene, 1024, bounces, p, hero)
         log(f"cpu: {out['cpu_baseline']}")

@@ -325,6 +325,11 @@ int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
  * comparison with the CPU oracle. */
 int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out);
 
+/* Replace the uploaded scene's camera (another view or resolution of the
+ * same geometry without rebuilding the BVH).  No counterpart in the
+ * reference, whose camera is part of the loaded scene (cg.hpp:2039-2063). */
+int lam_set_camera(lam_ctx *ctx, const lam_camera *cam);
+
 int lam_get_timing(const lam_ctx *ctx, lam_timing *out);
 int lam_reset_timing(lam_ctx *ctx);
 

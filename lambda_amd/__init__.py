@@ -198,6 +198,7 @@ def lib():
         "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
+        "lam_set_camera": (C.c_int, [vp, C.POINTER(Camera)]),
         "lam_set_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
         "lamh_load_nls": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
@@ -502,6 +503,7 @@ class Context:
             raise LambdaError(rc, "lam_create failed (no GPU or HIP runtime error)")
         self._h = h
         self.scene = None
+        self._size = (0, 0)  # image size of the current camera
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -528,6 +530,7 @@ class Context:
     def upload(self, scene):
         self._check(self._lib.lam_scene_upload(self._h, C.cast(scene.desc_ptr, C.c_void_p)))
         self.scene = scene
+        self._size = (scene.width, scene.height)
 
     def stats(self):
         s = SceneStats()
@@ -563,8 +566,7 @@ class Context:
         return out
 
     def render(self, p, want_xyz=False):
-        d = self.scene.desc
-        w, h = d.camera.width, d.camera.height
+        w, h = self._size
         rgb = np.zeros((h, w, 3), dtype=np.float32)
         xyz = np.zeros((h, w, 3), dtype=np.float32) if want_xyz else None
         self._check(self._lib.lam_render(self._h, C.byref(p), _fptr(rgb),
@@ -575,9 +577,9 @@ class Context:
         """lam_adjoint: gradient of an image loss with respect to the
         parameters of `dp` (a DiffParams) at the uploaded scene, given
         ∂loss/∂rgb (h x w x 3, as render's output)."""
-        d = self.scene.desc
+        w, h = self._size
         g = np.ascontiguousarray(dloss_drgb, dtype=np.float32)
-        if g.shape != (d.camera.height, d.camera.width, 3):
+        if g.shape != (h, w, 3):
             raise ValueError(f"dloss_drgb has shape {g.shape}")
         grad = np.zeros(dp.n_params, dtype=np.float32)
         cs = dp.c_struct()
@@ -587,8 +589,7 @@ class Context:
     # ---- learned path guiding ----
     def render_guided(self, p, g, want_xyz=False):
         """lam_render_guided: render with the learned proposal (g: GuideParams)."""
-        d = self.scene.desc
-        w, h = d.camera.width, d.camera.height
+        w, h = self._size
         rgb = np.zeros((h, w, 3), dtype=np.float32)
         xyz = np.zeros((h, w, 3), dtype=np.float32) if want_xyz else None
         self._check(self._lib.lam_render_guided(self._h, C.byref(p), C.byref(g), _fptr(rgb),
@@ -640,6 +641,14 @@ class Context:
 
     def reset_timing(self):
         self._check(self._lib.lam_reset_timing(self._h))
+
+    def set_camera(self, scene):
+        """use `scene`'s camera (a view of the uploaded geometry, e.g. the same
+        scene loaded at another resolution) without uploading again"""
+        cam = Camera()
+        C.memmove(C.byref(cam), C.byref(scene.desc.camera), C.sizeof(Camera))
+        self._check(self._lib.lam_set_camera(self._h, C.byref(cam)))
+        self._size = (scene.width, scene.height)
 
     def wave_config(self):
         """the wavefront launch configuration as a dict (lam_get_wave_config)"""

@@ -1210,6 +1210,16 @@ int lam_reset_timing(lam_ctx *ctx) {
   return LAM_OK;
 }
 
+int lam_set_camera(lam_ctx *ctx, const lam_camera *cam) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!cam) return fail(ctx, LAM_E_INVALID, "NULL camera");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (cam->width == 0 || cam->height == 0) return fail(ctx, LAM_E_INVALID, "empty image");
+  if (size_t(cam->width) * cam->height > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
+  ctx->sc.cam = *cam;
+  return LAM_OK;
+}
+
 int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out) {
   if (!ctx || !out) return LAM_E_INVALID;
   out->lanes = ctx->lanes;

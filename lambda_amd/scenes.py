@@ -340,7 +340,11 @@ def geodesic_sphere(freq, seed=2, amp=0.08):
     return V, F, N
 
 
-def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778, hero=4):
+def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778, hero=4, material_p=None):
+    """n_inst noise-displaced icospheres (uniform random rotation, scale
+    U[0.5, 2], position in a 20 m cube) over a ground plane, two area lights.
+    Materials: uniform over Lambertian / Blinn / thin film, or drawn with
+    probabilities material_p = (p_lambertian, p_blinn, p_thinfilm)."""
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     V, F, N = geodesic_sphere(freq, seed)
@@ -356,7 +360,7 @@ def icosphere_field(out_dir, seed=2, n_inst=64, freq=28, width=1920, ar=1.77778,
     objects.append({"name": "ground", "type": "plane",
                     "transform": {"scale": [40, 40, 1], "translate": [0, 0, -0.5]}, "material": "ground"})
     for i in range(n_inst):
-        kind = rng.integers(0, 3)
+        kind = rng.integers(0, 3) if material_p is None else rng.choice(3, p=material_p)
         name = f"m{i}"
         if kind == 0:
             p = os.path.join(out_dir, f"albedo_{i}.csv")
