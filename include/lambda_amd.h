@@ -167,6 +167,7 @@ typedef struct {
 #define LAM_FLAG_PROFILE 0x4u    /* time every kernel with HIP events */
 #define LAM_FLAG_NO_TAIL 0x8u    /* keep short queues in the wavefront (no finish kernel) */
 #define LAM_FLAG_NO_PIPE 0x10u   /* trace passes one after another on one stream (no overlap) */
+#define LAM_FLAG_ACCUMULATE 0x20u /* add the passes to the film of the previous render (progressive) */
 
 typedef struct {
   uint32_t spp;             /* Renderer::SPP            (renderer.hpp:36) */
@@ -324,6 +325,29 @@ int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
  * 1 cosf, 2 powf(a,b), 3 atan2f(a,b), 4 sqrtf(a), 5 a/b), for bitwise
  * comparison with the CPU oracle. */
 int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out);
+
+/* ---- progressive rendering and checkpoints ------------------------------
+ * No counterpart in the reference (progress/ETA output is listed as future
+ * work in its README.md:24).  A render with LAM_FLAG_ACCUMULATE adds passes
+ * spp_offset .. spp_offset+spp-1 to the film of the previous render (same
+ * camera and hero count) and returns the image of everything accumulated;
+ * each pixel still receives its samples in sample order, so rendering in
+ * pieces (or from a restored checkpoint) gives the bits of one render.
+ *
+ * lam_film_get copies the compensated XYZ film (width*height*8 floats per
+ * pixel: X, Y, Z sums, 0, X, Y, Z compensations, 0) and the samples per
+ * pixel and hero count it holds; lam_film_set restores such a checkpoint
+ * (for the current camera). */
+int lam_film_get(lam_ctx *ctx, float *film, uint32_t *spp, uint32_t *hero);
+/* Tone map + 8-bit sRGB of the last rendered image on the device
+ * (Renderer::toneMap, src/renderer.cpp:149-176: log-average exposure
+ * y_mid/Ȳ then the koiFilmic curve, cg.hpp:181-194; rendering::rgb24,
+ * include/renderer.hpp:20-29: sRGB OETF and 8-bit rounding).  rgb8:
+ * width*height*3 bytes, top row first, ready for lamh_write_png8.  The
+ * log-average is summed in double on the device, so Ȳ may differ from the
+ * host's float sum (lamh_tone_map) in its last bits. */
+int lam_tone_map(lam_ctx *ctx, float y_mid, uint8_t *rgb8);
+int lam_film_set(lam_ctx *ctx, const float *film, uint32_t spp, uint32_t hero);
 
 /* Replace the uploaded scene's camera (another view or resolution of the
  * same geometry without rebuilding the BVH).  No counterpart in the

@@ -36,6 +36,8 @@ int lamh_tone_map(const float *rgb_in, uint32_t w, uint32_t h, float y_mid, floa
 /* rendering::rgb24 + lodepng::encode (include/renderer.hpp:20-29,
  * src/renderer.cpp:187-190): sRGB OETF, 8-bit quantisation, RGB8 PNG. */
 int lamh_write_png(const char *path, const float *rgb, uint32_t w, uint32_t h);
+/* The same PNG from 8-bit RGB (w*h*3 bytes, e.g. lam_tone_map's output). */
+int lamh_write_png8(const char *path, const uint8_t *rgb8, uint32_t w, uint32_t h);
 /* Renderer::saveImage file name rule (src/renderer.cpp:182-185). */
 int lamh_image_name(const char *scene_path, uint32_t spp, uint32_t bounces, float p, const char *suffix,
                     char *out, size_t outlen);

@@ -42,6 +42,7 @@ LAM_FLAG_NO_GRAPH = 0x2
 LAM_FLAG_PROFILE = 0x4
 LAM_FLAG_NO_TAIL = 0x8
 LAM_FLAG_NO_PIPE = 0x10
+LAM_FLAG_ACCUMULATE = 0x20
 
 OBJ_SPHERE, OBJ_PLANE, OBJ_TRIMESH = 1, 2, 3
 (MAT_NONE, MAT_LAMBERTIAN, MAT_BLINN, MAT_THINFILM, MAT_MICROFACET, MAT_EMITTER,
@@ -199,6 +200,9 @@ def lib():
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_set_camera": (C.c_int, [vp, C.POINTER(Camera)]),
+        "lam_film_get": (C.c_int, [vp, fp, u32p, u32p]),
+        "lam_film_set": (C.c_int, [vp, fp, C.c_uint32, C.c_uint32]),
+        "lam_tone_map": (C.c_int, [vp, C.c_float, C.POINTER(C.c_uint8)]),
         "lam_set_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
         "lamh_load_nls": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
@@ -210,6 +214,7 @@ def lib():
         "lamh_material_name": (C.c_char_p, [vp, C.c_uint32]),
         "lamh_tone_map": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_float, fp]),
         "lamh_write_png": (C.c_int, [C.c_char_p, fp, C.c_uint32, C.c_uint32]),
+        "lamh_write_png8": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32]),
         "lamh_image_name": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p,
                                       C.c_char_p, sz]),
         "lamh_bvh8_check": (C.c_int, [fp, C.c_uint32, C.POINTER(SceneStats), C.POINTER(C.c_int)]),
@@ -642,6 +647,47 @@ class Context:
     def reset_timing(self):
         self._check(self._lib.lam_reset_timing(self._h))
 
+    # ---- progressive rendering, checkpoints, device tone map ----
+    def film_get(self):
+        """checkpoint of the film: (h x w x 8 float32 film, spp, hero)"""
+        w, h = self._size
+        film = np.zeros((h, w, 8), dtype=np.float32)
+        spp, hero = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.lam_film_get(self._h, _fptr(film), C.byref(spp), C.byref(hero)))
+        return film, spp.value, hero.value
+
+    def film_set(self, film, spp, hero):
+        """restore a checkpoint; continue it with LAM_FLAG_ACCUMULATE"""
+        w, h = self._size
+        film = np.ascontiguousarray(film, dtype=np.float32)
+        if film.shape != (h, w, 8):
+            raise ValueError(f"film has shape {film.shape}, the camera needs {(h, w, 8)}")
+        self._check(self._lib.lam_film_set(self._h, _fptr(film), int(spp), int(hero)))
+
+    def tone_map8(self, y_mid=0.12):
+        """lam_tone_map: tone-mapped 8-bit sRGB (h x w x 3 uint8) of the last render"""
+        w, h = self._size
+        out = np.zeros((h, w, 3), dtype=np.uint8)
+        self._check(self._lib.lam_tone_map(self._h, float(y_mid), out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def render_progressive(self, p, chunk, guide=None):
+        """Render p.spp samples per pixel in pieces of `chunk` passes (from
+        p.spp_offset), yielding (image, samples per pixel so far, estimated
+        seconds left) after each piece; the last image equals a one-shot
+        render of p bit for bit."""
+        import time
+        done, t0 = 0, time.perf_counter()
+        while done < p.spp:
+            n = min(chunk, p.spp - done)
+            q = RenderParams(spp=n, max_scatterings=p.max_scatterings, sample_p=p.sample_p, hero=p.hero,
+                             seed=p.seed, spp_offset=p.spp_offset + done,
+                             flags=p.flags | (LAM_FLAG_ACCUMULATE if done else 0))
+            img = self.render(q) if guide is None else self.render_guided(q, guide)
+            done += n
+            el = time.perf_counter() - t0
+            yield img, done, el / done * (p.spp - done)
+
     def set_camera(self, scene):
         """use `scene`'s camera (a view of the uploaded geometry, e.g. the same
         scene loaded at another resolution) without uploading again"""
@@ -679,6 +725,15 @@ def tone_map(rgb, y_mid=0.12):
     if rc != LAM_OK:
         raise LambdaError(rc, "tone map failed")
     return out
+
+
+def write_png8(path, rgb8):
+    """PNG from 8-bit sRGB (h x w x 3 uint8, e.g. Context.tone_map8's output)."""
+    rgb8 = np.ascontiguousarray(rgb8, dtype=np.uint8)
+    h, w = rgb8.shape[:2]
+    rc = lib().lamh_write_png8(str(path).encode(), rgb8.ctypes.data_as(C.POINTER(C.c_uint8)), w, h)
+    if rc != LAM_OK:
+        raise LambdaError(rc, f"could not write {path}")
 
 
 def write_png(path, rgb):

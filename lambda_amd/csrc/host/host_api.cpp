@@ -133,12 +133,18 @@ int lamh_tone_map(const float *in, uint32_t w, uint32_t h, float y_mid, float *o
 
 int lamh_write_png(const char *path, const float *rgb, uint32_t w, uint32_t h) {
   if (!path || !rgb || w == 0 || h == 0) return LAM_E_INVALID;
+  std::vector<uint8_t> px(size_t(w) * h * 3);
+  for (size_t i = 0; i < px.size(); i++) px[i] = to_byte(rgb[i]);
+  return lamh_write_png8(path, px.data(), w, h);
+}
+
+int lamh_write_png8(const char *path, const uint8_t *rgb8, uint32_t w, uint32_t h) {
+  if (!path || !rgb8 || w == 0 || h == 0) return LAM_E_INVALID;
   std::vector<uint8_t> raw;
   raw.reserve(size_t(h) * (size_t(w) * 3 + 1));
   for (uint32_t y = 0; y < h; y++) {
     raw.push_back(0);  // filter: none
-    for (uint32_t x = 0; x < w; x++)
-      for (int c = 0; c < 3; c++) raw.push_back(to_byte(rgb[(size_t(y) * w + x) * 3 + c]));
+    raw.insert(raw.end(), rgb8 + size_t(y) * w * 3, rgb8 + size_t(y + 1) * w * 3);
   }
   std::vector<uint8_t> z = {0x78, 0x01};
   uint32_t a = 1, b = 0;

@@ -51,6 +51,13 @@ struct lam_ctx {
   } lane[kMaxLanes];
   // block counts are set on the first render (they depend on the CU count);
   // the finish kernel takes over below 131,072 paths or after 12 bounces
+  // the film (scratch[3]) of the last render: pixels, hero count and the
+  // samples per pixel it holds (0: none to accumulate into)
+  size_t film_npix = 0;
+  uint32_t film_hero = 0;
+  uint64_t film_spp = 0;
+  size_t img_npix = 0;       // pixels of the last rendered linear-RGB image (scratch[4])
+  void *tone_buf = nullptr;  // tone map: 1024 per-block log sums (double) + the exposure
   lam::WaveLaunch wl{0, 0, 131072u, 12u, 0, 0};
   int cus = 0;
   uint32_t trace_per_cu = 24;  // extend / shadow blocks per CU
@@ -154,6 +161,32 @@ int check_params(lam_ctx *ctx, const lam_render_params *p) {
   if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
   if (ctx->sc.n_lights == 0) return fail(ctx, LAM_E_INVALID, "scene has no lights");
   return LAM_OK;
+}
+
+// Film for a render of the current camera: zeroed, or with
+// LAM_FLAG_ACCUMULATE the previous render's film (same pixels and hero
+// count).  The film counts as empty until finish_film records the render.
+int begin_film(lam_ctx *ctx, const lam_render_params *p, size_t npix, float **film, uint64_t *base_spp) {
+  if (p->flags & LAM_FLAG_ACCUMULATE) {
+    if (ctx->film_spp == 0 || ctx->film_npix != npix || ctx->film_hero != p->hero)
+      return fail(ctx, LAM_E_INVALID, "no film of this camera and hero count to accumulate into");
+    *base_spp = ctx->film_spp;
+  } else {
+    int rc;
+    if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;  // film: sum + compensation
+    HIPC(lam::launch_zero(ctx->scratch[3], npix * 32, ctx->stream));
+    *base_spp = 0;
+  }
+  ctx->film_spp = 0;
+  *film = static_cast<float *>(ctx->scratch[3]);
+  return LAM_OK;
+}
+
+void finish_film(lam_ctx *ctx, const lam_render_params *p, size_t npix, uint64_t base_spp) {
+  ctx->img_npix = npix;
+  ctx->film_npix = npix;
+  ctx->film_hero = p->hero;
+  ctx->film_spp = base_spp + p->spp;
 }
 
 void free_wave(lam_ctx *ctx, int l) {
@@ -616,6 +649,7 @@ void lam_destroy(lam_ctx *ctx) {
   for (int l = 0; l < kMaxLanes; l++) free_wave(ctx, l);
   free_guide(ctx);
   if (ctx->trace_ctr) (void)hipFree(ctx->trace_ctr);
+  if (ctx->tone_buf) (void)hipFree(ctx->tone_buf);
   for (lam_ctx::Lane &ln : ctx->lane) {
     if (ln.ctrl) (void)hipFree(ln.ctrl);
     if (ln.spill) (void)hipFree(ln.spill);
@@ -767,6 +801,8 @@ int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
   ctx->stats.n_analytic = sc.n_analytic;
   ctx->stats.sah_cost = bvh.sah_cost;
   ctx->has_scene = true;
+  ctx->film_spp = 0;  // a film (and its image) belongs to a scene and camera
+  ctx->img_npix = 0;
   return LAM_OK;
 }
 
@@ -918,11 +954,11 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
   const size_t npix = size_t(w) * h;
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
-  if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;  // film: sum + compensation
   if ((rc = ensure_scratch(ctx, 4, npix * 12))) return rc;  // linRGB
   if ((rc = ensure_scratch(ctx, 5, npix * 12))) return rc;  // XYZ sums
-  float *film = static_cast<float *>(ctx->scratch[3]);
-  HIPC(lam::launch_zero(film, npix * 32, ctx->stream));
+  float *film = nullptr;
+  uint64_t base_spp = 0;
+  if ((rc = begin_film(ctx, p, npix, &film, &base_spp))) return rc;
   if ((rc = clear_err_flag(ctx))) return rc;
   const lam::PathArgs pa = path_args(p);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -935,7 +971,8 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
     return rc;
   }
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
-  const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
+  finish_film(ctx, p, npix, base_spp);
+  const float inv_n = 1.f / float(ctx->film_spp * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
   HIPC(hipMemcpyAsync(linrgb, ctx->scratch[4], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
@@ -1151,19 +1188,20 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   gr.gd_train.rec_count = ctx->g_rec_count;
   gr.gd_train.rec_cap = ctx->g_rec_cap;
   gr.gd_train.rec_of_slot = ctx->g_rec_of_slot;
-  if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;
   if ((rc = ensure_scratch(ctx, 4, npix * 12))) return rc;
   if ((rc = ensure_scratch(ctx, 5, npix * 12))) return rc;
-  float *film = static_cast<float *>(ctx->scratch[3]);
-  HIPC(lam::launch_zero(film, npix * 32, ctx->stream));
+  float *film = nullptr;
+  uint64_t base_spp = 0;
+  if ((rc = begin_film(ctx, p, npix, &film, &base_spp))) return rc;
   if ((rc = clear_err_flag(ctx))) return rc;
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   if ((rc = render_wavefront(ctx, p, film, npix, &gr))) return rc;
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  finish_film(ctx, p, npix, base_spp);
   uint32_t gc[kMaxLanes][2] = {};
   for (int l = 0; l < kMaxLanes; l++)
     if (ctx->lane[l].ctrl) HIPC(hipMemcpyAsync(gc[l], ctx->lane[l].ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
-  const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
+  const float inv_n = 1.f / float(ctx->film_spp * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
   HIPC(hipMemcpyAsync(linrgb, ctx->scratch[4], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
@@ -1210,6 +1248,56 @@ int lam_reset_timing(lam_ctx *ctx) {
   return LAM_OK;
 }
 
+int lam_tone_map(lam_ctx *ctx, float y_mid, uint8_t *rgb8) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!rgb8) return fail(ctx, LAM_E_INVALID, "NULL output buffer");
+  if (!(y_mid > 0.f)) return fail(ctx, LAM_E_INVALID, "y_mid must be positive");
+  if (ctx->img_npix == 0) return fail(ctx, LAM_E_INVALID, "no rendered image");
+  HIPC(hipSetDevice(ctx->device));
+  if (!ctx->tone_buf) HIPC(hipMalloc(&ctx->tone_buf, 1024 * 8 + 16));
+  int rc;
+  if ((rc = ensure_scratch(ctx, 5, ctx->img_npix * 12))) return rc;
+  double *partial = static_cast<double *>(ctx->tone_buf);
+  float *exposure = reinterpret_cast<float *>(partial + 1024);
+  uint8_t *d_out = static_cast<uint8_t *>(ctx->scratch[5]);
+  HIPC(lam::launch_tonemap8(static_cast<const float *>(ctx->scratch[4]), uint32_t(ctx->img_npix), y_mid, partial,
+                            exposure, d_out, ctx->stream));
+  HIPC(hipMemcpyAsync(rgb8, d_out, ctx->img_npix * 3, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return LAM_OK;
+}
+
+int lam_film_get(lam_ctx *ctx, float *film, uint32_t *spp, uint32_t *hero) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!film || !spp || !hero) return fail(ctx, LAM_E_INVALID, "NULL argument");
+  if (ctx->film_spp == 0) return fail(ctx, LAM_E_INVALID, "no film");
+  if (ctx->film_spp > UINT32_MAX) return fail(ctx, LAM_E_LIMIT, "film holds more than 2^32-1 samples per pixel");
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(film, ctx->scratch[3], ctx->film_npix * 32, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  *spp = uint32_t(ctx->film_spp);
+  *hero = ctx->film_hero;
+  return LAM_OK;
+}
+
+int lam_film_set(lam_ctx *ctx, const float *film, uint32_t spp, uint32_t hero) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!film) return fail(ctx, LAM_E_INVALID, "NULL film");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (spp == 0) return fail(ctx, LAM_E_INVALID, "a film holds at least one sample per pixel");
+  if (hero != 4 && hero != 16) return fail(ctx, LAM_E_INVALID, "hero must be 4 or 16");
+  const size_t npix = size_t(ctx->sc.cam.width) * ctx->sc.cam.height;
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_scratch(ctx, 3, npix * 32))) return rc;
+  HIPC(hipMemcpyAsync(ctx->scratch[3], film, npix * 32, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->film_npix = npix;
+  ctx->film_hero = hero;
+  ctx->film_spp = spp;
+  return LAM_OK;
+}
+
 int lam_set_camera(lam_ctx *ctx, const lam_camera *cam) {
   if (!ctx) return LAM_E_INVALID;
   if (!cam) return fail(ctx, LAM_E_INVALID, "NULL camera");
@@ -1217,6 +1305,8 @@ int lam_set_camera(lam_ctx *ctx, const lam_camera *cam) {
   if (cam->width == 0 || cam->height == 0) return fail(ctx, LAM_E_INVALID, "empty image");
   if (size_t(cam->width) * cam->height > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
   ctx->sc.cam = *cam;
+  ctx->film_spp = 0;
+  ctx->img_npix = 0;
   return LAM_OK;
 }
 

@@ -140,6 +140,68 @@ __global__ void k_resolve(DevScene sc, const float *__restrict__ film, uint32_t 
   }
 }
 
+// ---------------------------------------------------------------------------
+// tone map + 8-bit sRGB of a linear-RGB image (Renderer::toneMap,
+// src/renderer.cpp:149-176; rendering::rgb24, include/renderer.hpp:20-29)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kToneBlocks = 1024;
+
+// per-block sums of log(max(Y, FLT_EPSILON)), in double
+__global__ __launch_bounds__(256) void k_logsum(const float *__restrict__ rgb, uint32_t n, double *__restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const float Y = 0.2126f * rgb[3 * size_t(i)] + 0.7152f * rgb[3 * size_t(i) + 1] + 0.0722f * rgb[3 * size_t(i) + 2];
+    acc += double(logf(Y >= 1.1920929e-07f ? Y : 1.1920929e-07f));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// exposure = y_mid / exp(mean log Y), from the per-block sums (one block)
+__global__ __launch_bounds__(256) void k_exposure(const double *__restrict__ partial, uint32_t nb, uint32_t n,
+                                                  float y_mid, float *__restrict__ exposure) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (uint32_t b = threadIdx.x; b < nb; b += 256u) acc += partial[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *exposure = y_mid / float(exp(red[0] / double(n)));
+}
+
+// koiFilmic (filmicToneMap(10 L, .3, .14, .3, 2.8, 10), cg.hpp:181-194),
+// then linRGB2sRGB + sRGB2rgb24 (cg.hpp:149-162, numli.hpp:105-106)
+__global__ __launch_bounds__(256) void k_tonemap8(const float *__restrict__ rgb, uint32_t n,
+                                                  const float *__restrict__ exposure, uint8_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= 3 * n) return;
+  const float x = 10.f * (rgb[i] * *exposure);
+  const float cl = (x * (0.3f * x + 0.14f)) / (x * (0.3f * x + 2.8f) + 10.f);
+  const float s = cl > 0.0031308f ? 1.055f * lam_powf(cl, 1.f / 2.4f) - 0.055f : 12.92f * cl;
+  const float v = 255.f * s + 0.5f;
+  out[i] = !(v > 0.f) ? uint8_t(0) : (v >= 255.f ? uint8_t(255) : uint8_t(int(v)));
+}
+
+hipError_t launch_tonemap8(const float *rgb, uint32_t n, float y_mid, double *partial, float *exposure,
+                           uint8_t *out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const uint32_t nb = std::min(kToneBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(k_logsum, dim3(nb), dim3(256), 0, st, rgb, n, partial);
+  hipLaunchKernelGGL(k_exposure, dim3(1), dim3(256), 0, st, partial, nb, n, y_mid, exposure);
+  hipLaunchKernelGGL(k_tonemap8, dim3(uint32_t((3 * size_t(n) + 255) / 256)), dim3(256), 0, st, rgb, n, exposure,
+                     out);
+  return hipGetLastError();
+}
+
 __global__ void k_math(int fn, const float *__restrict__ a, const float *__restrict__ b, uint32_t n,
                        float *__restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

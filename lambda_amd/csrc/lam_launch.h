@@ -28,6 +28,10 @@ hipError_t launch_trace_paths(const DevScene &sc, const PathArgs &pa, int hero, 
                               uint32_t blocks, hipStream_t st);
 hipError_t launch_render_mega(const DevScene &sc, const PathArgs &pa, int hero, uint32_t sample, uint32_t npix,
                               float *film, uint32_t *err, uint32_t *spill, uint32_t blocks, hipStream_t st);
+// tone map + 8-bit sRGB of n linear-RGB pixels; partial: 1024 doubles,
+// exposure: 1 float of device scratch
+hipError_t launch_tonemap8(const float *rgb, uint32_t n, float y_mid, double *partial, float *exposure,
+                           uint8_t *out, hipStream_t st);
 hipError_t launch_resolve(const DevScene &sc, const float *film, uint32_t w, uint32_t h, float inv_n, float *rgb,
                           float *xyz, hipStream_t st);
 // zeroes bytes / 16 float4 words at p (16-byte aligned; a tail below 16 bytes is left)

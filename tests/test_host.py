@@ -58,3 +58,16 @@ def test_save_image_name_rule():
     # Renderer::saveImage (src/renderer.cpp:182-185)
     assert la.image_name("scenes/box.nls", 16, 8, 0.8) == "render/box-16spp-8b-0.80p.png"
     assert la.image_name("scenes/box.nls", 4, 4, 0.5, "-raw") == "render/box-4spp-4b-0.50p-raw.png"
+
+
+def test_png8_matches_float_png(tmp_path):
+    """lamh_write_png8 of the bytes lamh_write_png derives gives the same file"""
+    rgb = np.random.default_rng(1).uniform(0, 1, (9, 6, 3)).astype(np.float32)
+    a, b = tmp_path / "a.png", tmp_path / "b.png"
+    la.write_png(a, rgb)
+    data = a.read_bytes()
+    idat = data.index(b"IDAT")
+    n = struct.unpack(">I", data[idat - 4:idat])[0]
+    px = np.frombuffer(zlib.decompress(data[idat + 4:idat + 4 + n]), np.uint8).reshape(9, 1 + 18)[:, 1:]
+    la.write_png8(b, px.reshape(9, 6, 3))
+    assert b.read_bytes() == data
