@@ -63,7 +63,8 @@ def test_device_tone_map_matches_host(ctx, box_scene):
     img = ctx.render(la.params(spp=8, max_scatterings=8, sample_p=0.8, hero=4, seed=2))
     dev = ctx.tone_map8(0.18)
     tm = la.tone_map(img, 0.18)
-    s = np.where(tm > 0.0031308, 1.055 * np.power(tm.astype(np.float64), 1 / 2.4) - 0.055, 12.92 * tm)
+    t64 = tm.astype(np.float64)
+    s = np.where(t64 > 0.0031308, 1.055 * np.power(np.maximum(t64, 0.0), 1 / 2.4) - 0.055, 12.92 * t64)
     host = np.clip(np.floor(255 * s + 0.5), 0, 255).astype(np.int32)
     d = np.abs(dev.astype(np.int32) - host)
     assert d.max() <= 1 and np.mean(d > 0) < 1e-3, (d.max(), np.mean(d > 0))
