@@ -445,9 +445,13 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
   const size_t cap = std::min(npix, kMaxWaveCap);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
-  const uint32_t lanes = (p->flags & (LAM_FLAG_NO_PIPE | LAM_FLAG_PROFILE)) ? 1u : std::min(ctx->lanes, p->spp);
+  // every configured lane gets its state on the first render, also when
+  // this one has fewer passes than lanes, so that later renders allocate
+  // nothing
+  const uint32_t lanes_cfg = (p->flags & (LAM_FLAG_NO_PIPE | LAM_FLAG_PROFILE)) ? 1u : ctx->lanes;
+  const uint32_t lanes = std::min(lanes_cfg, p->spp);
   int rc;
-  for (uint32_t l = 0; l < lanes; l++) {
+  for (uint32_t l = 0; l < lanes_cfg; l++) {
     if ((rc = ensure_wave(ctx, cap, int(p->hero), int(l)))) return rc;
     if (gr && (rc = ensure_mix(ctx, int(l)))) return rc;
   }
