@@ -291,6 +291,20 @@ def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
     st = ctx.stats()
     log(f"stress: {scene.n_tris} tris, load {t_load:.1f}s, upload (BVH build) {t_up:.1f}s, depth {st.depth}, "
         f"stack bound {st.stack_bound}")
+    # the same scene with the BVH built on the device, and the plain render on either tree
+    pq = la.params(spp=16, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=1)
+    t0 = time.perf_counter()
+    ref16 = ctx.render(pq)
+    t_host_tree = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx.upload(scene, gpu_bvh=True)
+    t_up_gpu = time.perf_counter() - t0
+    sg = ctx.stats()
+    t0 = time.perf_counter()
+    gpu16 = ctx.render(pq)
+    t_gpu_tree = time.perf_counter() - t0
+    same = bool(np.array_equal(ref16, gpu16))
+    ctx.upload(scene)
     pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=1)
     gp = la.guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1)
     ctx.guide_reset(1)
@@ -316,6 +330,9 @@ def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
                         f"{dp.n_params} thin-film thicknesses at 1024x1024, {adj_spp} spp",
             "load_s": round(t_load, 2), "upload_s": round(t_up, 2), "bvh_depth": st.depth,
             "stack_bound": st.stack_bound,
+            "gpu_bvh": {"upload_s": round(t_up_gpu, 2), "nodes": sg.n_nodes, "depth": sg.depth,
+                        "stack_bound": sg.stack_bound, "render_16spp_s": round(t_gpu_tree, 3),
+                        "host_tree_render_16spp_s": round(t_host_tree, 3), "same_image": same},
             "guided_s": round(t_g, 3), "guided_samples_per_s": round(scene.width * scene.height * spp / t_g, 1),
             "train_ms": round(tm.guide_train_ms, 1), "image_finite": bool(np.isfinite(img).all()),
             "forward_s": round(t_f, 3), "adjoint_s": round(t_a, 3), "n_params": dp.n_params,

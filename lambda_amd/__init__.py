@@ -43,6 +43,7 @@ LAM_FLAG_PROFILE = 0x4
 LAM_FLAG_NO_TAIL = 0x8
 LAM_FLAG_NO_PIPE = 0x10
 LAM_FLAG_ACCUMULATE = 0x20
+LAM_UPLOAD_GPU_BVH = 0x1
 
 OBJ_SPHERE, OBJ_PLANE, OBJ_TRIMESH = 1, 2, 3
 (MAT_NONE, MAT_LAMBERTIAN, MAT_BLINN, MAT_THINFILM, MAT_MICROFACET, MAT_EMITTER,
@@ -200,6 +201,7 @@ def lib():
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_set_camera": (C.c_int, [vp, C.POINTER(Camera)]),
+        "lam_scene_upload_ex": (C.c_int, [vp, vp, C.c_uint32]),
         "lam_film_get": (C.c_int, [vp, fp, u32p, u32p]),
         "lam_film_set": (C.c_int, [vp, fp, C.c_uint32, C.c_uint32]),
         "lam_tone_map": (C.c_int, [vp, C.c_float, C.POINTER(C.c_uint8)]),
@@ -532,8 +534,10 @@ class Context:
         if rc != LAM_OK:
             raise LambdaError(rc, self._lib.lam_last_error(self._h).decode())
 
-    def upload(self, scene):
-        self._check(self._lib.lam_scene_upload(self._h, C.cast(scene.desc_ptr, C.c_void_p)))
+    def upload(self, scene, gpu_bvh=False):
+        """lam_scene_upload(_ex): gpu_bvh builds the BVH8 on the device"""
+        self._check(self._lib.lam_scene_upload_ex(self._h, C.cast(scene.desc_ptr, C.c_void_p),
+                                                  LAM_UPLOAD_GPU_BVH if gpu_bvh else 0))
         self.scene = scene
         self._size = (scene.width, scene.height)
 

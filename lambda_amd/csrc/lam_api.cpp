@@ -163,6 +163,58 @@ int check_params(lam_ctx *ctx, const lam_render_params *p) {
   return LAM_OK;
 }
 
+// BVH8 and leaf-ordered triangle/shading records built on the device from
+// the records in input order; fails with LAM_E_BVH if the tree needs a
+// deeper traversal stack than the kernels hold.
+int build_bvh_on_device(lam_ctx *ctx, const std::vector<float> &tri, const std::vector<float> &shd, uint32_t nt,
+                        lam::GpuBvhStats *st) {
+  const size_t rec = size_t(nt) * 48;
+  void *tin = nullptr, *sin = nullptr, *scratch = nullptr, *nodes = nullptr;
+  auto cleanup = [&]() {
+    for (void *p : {tin, sin, scratch, nodes})
+      if (p) (void)hipFree(p);
+  };
+  auto dev = [&](hipError_t e, const char *what) {
+    if (e == hipSuccess) return LAM_OK;
+    cleanup();
+    return fail(ctx, LAM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  int rc;
+  if ((rc = dev(hipMalloc(&tin, rec), "hipMalloc"))) return rc;
+  if ((rc = dev(hipMalloc(&sin, rec), "hipMalloc"))) return rc;
+  if ((rc = dev(hipMalloc(&scratch, lam::gpu_bvh_scratch_bytes(nt)), "hipMalloc"))) return rc;
+  if ((rc = dev(hipMalloc(&nodes, size_t(nt) * sizeof(lam::Node8d)), "hipMalloc"))) return rc;
+  void *tout = nullptr, *sout = nullptr;
+  if ((rc = dev(hipMalloc(&tout, rec), "hipMalloc"))) return rc;
+  ctx->scene_bufs.push_back(tout);
+  if ((rc = dev(hipMalloc(&sout, rec), "hipMalloc"))) return rc;
+  ctx->scene_bufs.push_back(sout);
+  if ((rc = dev(hipMemcpy(tin, tri.data(), rec, hipMemcpyHostToDevice), "hipMemcpy"))) return rc;
+  if ((rc = dev(hipMemcpy(sin, shd.data(), rec, hipMemcpyHostToDevice), "hipMemcpy"))) return rc;
+  if ((rc = dev(lam::gpu_build_bvh8(static_cast<const float4 *>(tin), static_cast<const float4 *>(sin), nt, scratch,
+                                    static_cast<lam::Node8d *>(nodes), static_cast<float4 *>(tout),
+                                    static_cast<float4 *>(sout), st, ctx->stream),
+                "BVH build")))
+    return rc;
+  if (st->stack_bound > uint32_t(lam::kStack)) {
+    cleanup();
+    return fail(ctx, LAM_E_BVH, "BVH needs a traversal stack of " + std::to_string(st->stack_bound) +
+                                    " entries (limit " + std::to_string(lam::kStack) + ")");
+  }
+  // the nodes into a buffer of their own size
+  void *exact = nullptr;
+  const size_t nbytes = size_t(st->n_nodes) * sizeof(lam::Node8d);
+  if ((rc = dev(hipMalloc(&exact, nbytes), "hipMalloc"))) return rc;
+  ctx->scene_bufs.push_back(exact);
+  if ((rc = dev(hipMemcpy(exact, nodes, nbytes, hipMemcpyDeviceToDevice), "hipMemcpy"))) return rc;
+  cleanup();
+  ctx->sc.nodes = static_cast<const lam::Node8d *>(exact);
+  ctx->sc.tris = static_cast<const float4 *>(tout);
+  ctx->sc.shade = static_cast<const float4 *>(sout);
+  ctx->stats.device_bytes += 2 * rec + nbytes;
+  return LAM_OK;
+}
+
 // Film for a render of the current camera: zeroed, or with
 // LAM_FLAG_ACCUMULATE the previous render's film (same pixels and hero
 // count).  The film counts as empty until finish_film records the render.
@@ -673,7 +725,9 @@ void lam_destroy(lam_ctx *ctx) {
 
 const char *lam_last_error(const lam_ctx *ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
 
-int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
+int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) { return lam_scene_upload_ex(ctx, d, 0); }
+
+int lam_scene_upload_ex(lam_ctx *ctx, const lam_scene_desc *d, uint32_t flags) {
   if (!ctx) return LAM_E_INVALID;
   if (!d) return fail(ctx, LAM_E_INVALID, "scene description is NULL");
   HIPC(hipSetDevice(ctx->device));
@@ -723,20 +777,34 @@ int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
   free_scene(ctx);
   ctx->stats = lam_scene_stats{};
 
-  lamh::Bvh8 bvh;
-  if (!lamh::build_bvh8(d->tri_v, d->n_tris, bvh)) return fail(ctx, LAM_E_LIMIT, "too many triangles");
-  if (bvh.stack_bound > uint32_t(lam::kStack))
-    return fail(ctx, LAM_E_BVH, "BVH needs a traversal stack of " + std::to_string(bvh.stack_bound) +
-                                    " entries (limit " + std::to_string(lam::kStack) + ")");
   const uint32_t nt = d->n_tris;
-  std::vector<float> tri(size_t(nt) * 12), shd(size_t(nt) * 12);
-  for (uint32_t s = 0; s < nt; s++) {
-    const uint32_t o = bvh.order[s];
-    std::memcpy(&tri[size_t(s) * 12], d->tri_v + size_t(o) * 9, 36);
-    uint32_t ids[3] = {d->tri_obj[o], d->tri_prim[o], o};
-    std::memcpy(&tri[size_t(s) * 12 + 9], ids, 12);
-    std::memcpy(&shd[size_t(s) * 12], d->tri_n + size_t(o) * 9, 36);
-    std::memcpy(&shd[size_t(s) * 12 + 9], d->tri_gn + size_t(o) * 3, 12);
+  if (nt > lamh::kLeafFirstMask) return fail(ctx, LAM_E_LIMIT, "too many triangles");
+  // triangle records (v0 v1 v2 | obj prim input-index) and shading records
+  // (n0 n1 n2 | geometric normal), 48 B each; order[s] = input triangle of slot s
+  auto records = [&](const uint32_t *order, std::vector<float> &tri, std::vector<float> &shd) {
+    tri.resize(size_t(nt) * 12);
+    shd.resize(size_t(nt) * 12);
+    for (uint32_t s = 0; s < nt; s++) {
+      const uint32_t o = order ? order[s] : s;
+      std::memcpy(&tri[size_t(s) * 12], d->tri_v + size_t(o) * 9, 36);
+      uint32_t ids[3] = {d->tri_obj[o], d->tri_prim[o], o};
+      std::memcpy(&tri[size_t(s) * 12 + 9], ids, 12);
+      std::memcpy(&shd[size_t(s) * 12], d->tri_n + size_t(o) * 9, 36);
+      std::memcpy(&shd[size_t(s) * 12 + 9], d->tri_gn + size_t(o) * 3, 12);
+    }
+  };
+  std::vector<float> tri, shd;
+  lamh::Bvh8 bvh;
+  lam::GpuBvhStats gst{0, 0, 0, 0};
+  const bool gpu_bvh = (flags & LAM_UPLOAD_GPU_BVH) && nt > 0;
+  if (!gpu_bvh) {
+    if (!lamh::build_bvh8(d->tri_v, nt, bvh)) return fail(ctx, LAM_E_LIMIT, "too many triangles");
+    if (bvh.stack_bound > uint32_t(lam::kStack))
+      return fail(ctx, LAM_E_BVH, "BVH needs a traversal stack of " + std::to_string(bvh.stack_bound) +
+                                      " entries (limit " + std::to_string(lam::kStack) + ")");
+    records(bvh.order.data(), tri, shd);
+  } else {
+    records(nullptr, tri, shd);
   }
   std::vector<uint32_t> analytic;
   for (uint32_t i = 0; i < d->n_objects; i++)
@@ -772,21 +840,25 @@ int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
   lam::DevScene &sc = ctx->sc;
   int rc;
   static_assert(sizeof(lamh::Node8) == sizeof(lam::Node8d), "node layout");
-  const lamh::Node8 *nodes_dev = nullptr;
-  if ((rc = upload(ctx, bvh.nodes.data(), bvh.nodes.size(), &nodes_dev))) return rc;
-  sc.nodes = reinterpret_cast<const lam::Node8d *>(nodes_dev);
-  const float *tri_dev = nullptr, *shd_dev = nullptr;
-  if ((rc = upload(ctx, tri.data(), tri.size(), &tri_dev))) return rc;
-  if ((rc = upload(ctx, shd.data(), shd.size(), &shd_dev))) return rc;
-  sc.tris = reinterpret_cast<const float4 *>(tri_dev);
-  sc.shade = reinterpret_cast<const float4 *>(shd_dev);
+  if (!gpu_bvh) {
+    const lamh::Node8 *nodes_dev = nullptr;
+    if ((rc = upload(ctx, bvh.nodes.data(), bvh.nodes.size(), &nodes_dev))) return rc;
+    sc.nodes = reinterpret_cast<const lam::Node8d *>(nodes_dev);
+    const float *tri_dev = nullptr, *shd_dev = nullptr;
+    if ((rc = upload(ctx, tri.data(), tri.size(), &tri_dev))) return rc;
+    if ((rc = upload(ctx, shd.data(), shd.size(), &shd_dev))) return rc;
+    sc.tris = reinterpret_cast<const float4 *>(tri_dev);
+    sc.shade = reinterpret_cast<const float4 *>(shd_dev);
+  } else if ((rc = build_bvh_on_device(ctx, tri, shd, nt, &gst))) {
+    return rc;
+  }
   if ((rc = upload(ctx, analytic.data(), analytic.size(), &sc.analytic))) return rc;
   if ((rc = upload(ctx, d->objects, d->n_objects, &sc.objects))) return rc;
   if ((rc = upload(ctx, d->materials, d->n_materials, &sc.materials))) return rc;
   if ((rc = upload(ctx, d->lights, d->n_lights, &sc.lights))) return rc;
   if ((rc = upload(ctx, d->spectra, size_t(d->n_spectra) * gn, &sc.spectra))) return rc;
   if ((rc = upload(ctx, d->cmf, size_t(3) * gn, &sc.cmf))) return rc;
-  sc.n_nodes = uint32_t(bvh.nodes.size());
+  sc.n_nodes = gpu_bvh ? gst.n_nodes : uint32_t(bvh.nodes.size());
   sc.n_tris = nt;
   sc.n_analytic = uint32_t(analytic.size());
   sc.n_objects = d->n_objects;
@@ -798,12 +870,12 @@ int lam_scene_upload(lam_ctx *ctx, const lam_scene_desc *d) {
   sc.grid_delta = (d->grid.sup - d->grid.inf) / float(gn - 1);
   sc.cam = d->camera;
   ctx->stats.n_nodes = sc.n_nodes;
-  ctx->stats.n_leaves = bvh.n_leaves;
+  ctx->stats.n_leaves = gpu_bvh ? gst.n_leaves : bvh.n_leaves;
   ctx->stats.n_tris = nt;
-  ctx->stats.depth = bvh.depth;
-  ctx->stats.stack_bound = bvh.stack_bound;
+  ctx->stats.depth = gpu_bvh ? gst.depth : bvh.depth;
+  ctx->stats.stack_bound = gpu_bvh ? gst.stack_bound : bvh.stack_bound;
   ctx->stats.n_analytic = sc.n_analytic;
-  ctx->stats.sah_cost = bvh.sah_cost;
+  ctx->stats.sah_cost = gpu_bvh ? 0.0 : bvh.sah_cost;
   ctx->has_scene = true;
   ctx->film_spp = 0;  // a film (and its image) belongs to a scene and camera
   ctx->img_npix = 0;

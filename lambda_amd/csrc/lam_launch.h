@@ -122,6 +122,18 @@ hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
 hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset,
                         const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st);
 
+// ---- BVH8 build on the GPU (lam_bvh.hip) -----------------------------------------
+struct GpuBvhStats {
+  uint32_t n_nodes, depth, n_leaves, stack_bound;
+};
+// device scratch for gpu_build_bvh8 over n triangles
+size_t gpu_bvh_scratch_bytes(uint32_t n);
+// tri/shd: the triangle and shading records (3 float4 each) in input order;
+// nodes: room for n records; tri_out/shd_out: the records in leaf order.
+// Synchronises the stream (the collapse reads each level's node count).
+hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void *scratch, Node8d *nodes,
+                          float4 *tri_out, float4 *shd_out, GpuBvhStats *st, hipStream_t stream);
+
 // ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------
 constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block
 hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
