@@ -344,6 +344,38 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
       tk[k] = hit ? fminf(tn, kFltMax) : kInf;
       ck[k] = ch[k];
     }
+    if constexpr (ANY) {
+      // any hit: the order of the children does not matter, so no sort:
+      // descend into the first hit child in slot order, push the others
+      int nh = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) nh += tk[k] != kInf ? 1 : 0;
+      if (nh > 0) {
+        const int top = t.sp + nh - 1;
+        if (top > kStack) {
+          overflow = true;
+          return false;
+        }
+        uint32_t first = 0;
+        int j = 0;
+        const bool lds = stk.in_lds(top);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (tk[k] != kInf) {
+            if (j == 0)
+              first = ck[k];
+            else if (lds)
+              stk.put_lds(t.sp + j - 1, ck[k]);
+            else
+              stk.put(t.sp + j - 1, ck[k]);
+            j++;
+          }
+        }
+        t.sp = top;
+        t.cur = first;
+        return true;
+      }
+    } else {
     // sorting network for 8 (19 comparators), nearest first
     cswap(tk[0], ck[0], tk[1], ck[1]); cswap(tk[2], ck[2], tk[3], ck[3]);
     cswap(tk[4], ck[4], tk[5], ck[5]); cswap(tk[6], ck[6], tk[7], ck[7]);
@@ -379,6 +411,7 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
       t.sp = top;
       t.cur = ck[0];
       return true;
+    }
     }
   } else {
     const uint32_t first = cur & 0x07FFFFFFu;
@@ -439,55 +472,76 @@ __device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, const Sta
 constexpr uint32_t kRefill = 16;
 constexpr uint32_t kPool = 256;
 constexpr uint32_t kPoolRounds = 2;
+
+// Wave-uniform pool of queue indices [pool, pool_end) for per-lane refill
+// (see trace_queue).  dry: the queue has no unreserved entries left.
+struct QueuePool {
+  uint32_t pool, pool_end, psize, n;
+  bool dry;
+  __device__ __forceinline__ void init(uint32_t n_) {
+    n = n_;
+    pool = blockIdx.x * 64u;
+    pool_end = pool < n ? min(pool + 64u, n) : pool;
+    dry = pool >= n;
+    const uint32_t first_round = 64u * gridDim.x;
+    psize = n > first_round ? min(kPool, max(32u, (n - first_round) / (gridDim.x * kPoolRounds) + 1u)) : 32u;
+  }
+  __device__ __forceinline__ bool done(uint32_t nidle) const { return nidle == 64 && pool_end == pool && dry; }
+  __device__ __forceinline__ bool can_refill(uint32_t nidle) const {
+    return nidle >= kRefill && (pool_end > pool || !dry);
+  }
+  // Hands idle lanes (ballot `idle`, nidle of them) the next indices: the
+  // rest of the pool, then a new reservation from *fetch.  Returns this
+  // lane's index (n for none; only meaningful where `is_idle`).
+  __device__ __forceinline__ uint32_t take(uint64_t idle, uint32_t nidle, bool is_idle, uint32_t *fetch) {
+    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t k1 = min(nidle, pool_end - pool);
+    uint32_t nb = 0, nb_cnt = 0;
+    if (k1 < nidle && !dry) {
+      uint32_t r = 0;
+      if (lane == 0) r = atomicAdd(fetch, psize);
+      nb = 64u * gridDim.x + __shfl(r, 0, 64);
+      if (nb >= n)
+        dry = true;
+      else
+        nb_cnt = min(psize, n - nb);
+    }
+    uint32_t idx = n;
+    if (is_idle) {
+      const uint32_t rank = __popcll(idle & ((uint64_t(1) << lane) - 1));
+      idx = rank < k1 ? pool + rank : (rank - k1 < nb_cnt ? nb + (rank - k1) : n);
+    }
+    if (nb_cnt > 0) {
+      pool = nb + min(nidle - k1, nb_cnt);
+      pool_end = nb + nb_cnt;
+    } else {
+      pool += k1;
+    }
+    return idx;
+  }
+};
+
 template <bool ANY, bool COUNT, class Stack, class Load, class Done>
 __device__ __forceinline__ void trace_queue(const DevScene &sc, uint32_t n, uint32_t *fetch, const Stack &stk,
                                             Load load, Done done) {
-  const uint32_t lane = threadIdx.x % 64;
-  const uint64_t below = (uint64_t(1) << lane) - 1;
   TravCounts cnt{0, 0};
   Trav t;
   bool active = false, ovf = false;
   uint32_t idx = 0;
-  // wave-uniform pool of reserved indices [pool, pool_end); dry: the queue
-  // has no unreserved rays left
-  uint32_t pool = blockIdx.x * 64u;
-  uint32_t pool_end = pool < n ? min(pool + 64u, n) : pool;
-  bool dry = pool >= n;
-  const uint32_t first_round = 64u * gridDim.x;
-  const uint32_t psize =
-      n > first_round ? min(kPool, max(32u, (n - first_round) / (gridDim.x * kPoolRounds) + 1u)) : 32u;
+  QueuePool qp;
+  qp.init(n);
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    const uint32_t avail = pool_end - pool;
-    if (nidle == 64 && avail == 0 && dry) break;
-    if (nidle >= kRefill && (avail > 0 || !dry)) {
-      const uint32_t k1 = min(nidle, avail);  // from the current pool
-      uint32_t nb = 0, nb_cnt = 0;            // a new reservation
-      if (k1 < nidle && !dry) {
-        uint32_t r = 0;
-        if (lane == 0) r = atomicAdd(fetch, psize);
-        nb = 64u * gridDim.x + __shfl(r, 0, 64);
-        if (nb >= n)
-          dry = true;
-        else
-          nb_cnt = min(psize, n - nb);
-      }
-      if (!active) {
-        const uint32_t rank = __popcll(idle & below);
-        idx = rank < k1 ? pool + rank : (rank - k1 < nb_cnt ? nb + (rank - k1) : n);
-        if (idx < n) {
-          ovf = false;
-          cnt = TravCounts{0, 0};
-          active = trav_begin<ANY>(sc, load(idx), t);
-          if (!active) done(idx, t.best, false, cnt);
-        }
-      }
-      if (nb_cnt > 0) {
-        pool = nb + min(nidle - k1, nb_cnt);
-        pool_end = nb + nb_cnt;
-      } else {
-        pool += k1;
+    if (qp.done(nidle)) break;
+    if (qp.can_refill(nidle)) {
+      const uint32_t i = qp.take(idle, nidle, !active, fetch);
+      if (!active && i < n) {
+        idx = i;
+        ovf = false;
+        cnt = TravCounts{0, 0};
+        active = trav_begin<ANY>(sc, load(idx), t);
+        if (!active) done(idx, t.best, false, cnt);
       }
     }
     if (active) {

@@ -574,73 +574,83 @@ __device__ __forceinline__ float hero_lambdas(const DevScene &sc, float xi, Lam<
 // equal to the reference up to floating-point reassociation.  Starting at
 // vertex s0 with (ray, β, L) lets the wavefront hand its tail of long paths
 // to one thread each.
+// One vertex of the path: intersect, emission, light IS, material IS and
+// MIS (src/renderer.cpp:64-146), updating (ray, β, L).  Returns false when
+// the path ends (L final), true with the continuation ray in `ray`.
+template <int H, class Stack>
+__device__ __forceinline__ bool path_vertex(const DevScene &sc, uint32_t max_scat, float sample_p, uint64_t key,
+                                            const Lam<H> &lam, uint32_t s, Ray &ray, float *beta, float *L,
+                                            const Stack &stk, bool &ovf) {
+  TravCounts tc{0, 0};
+  const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
+  if (hit.obj == LAM_NONE || ovf) return false;
+  const Surf sf = surface(sc, hit, ray.o, ray.d);
+  const f3 o = neg(normalized(ray.d));
+  const f3 n = sf.n();
+  const lam_material &mat = sc.materials[sf.mat];
+  if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
+    float le[H];
+    spec_eval<H>(sc, mat.spec[0], lam, le);
+    float f = 1.f;
+    if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+#pragma unroll
+    for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+    return false;
+  }
+  if (s > max_scat) return false;
+  // light IS (src/renderer.cpp:91-106)
+  const uint32_t nl = sc.n_lights;
+  const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+  const float pl = 1.f / float(nl);
+  const lam_light &lt = sc.lights[li];
+  LightSample<H> ls;
+  sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                  lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+  float coef[H];
+  bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
+  bool vis = true;
+  if (ls.shadow) {
+    const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
+    vis = sh.obj == LAM_NONE;
+  }
+  float lis[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+  // material IS (src/renderer.cpp:116-120)
+  MatSample<H> ms;
+  sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
+                     lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
+                     sample_p, ms);
+  if (!ms.ok) {
+    if (vis) {
+#pragma unroll
+      for (int k = 0; k < H; k++) L[k] += beta[k] * lis[k];
+    }
+    return false;
+  }
+  // MIS (src/renderer.cpp:135-145), power heuristic β=2
+  const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+  const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], sample_p);
+  const float p_L_i = ls.prob * pl;
+  const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+  const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+  if (vis) {
+#pragma unroll
+    for (int k = 0; k < H; k++) L[k] += beta[k] * (lis[k] * w_L);
+  }
+#pragma unroll
+  for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
+  ray.o = sf.p;
+  ray.d = ms.dir;
+  ray.tmax = kFltMax;
+  ray.skip = LAM_NONE;
+  return true;
+}
+
 template <int H, class Stack>
 __device__ void path_loop(const DevScene &sc, uint32_t max_scat, float sample_p, uint64_t key, const Lam<H> &lam,
                           uint32_t s0, Ray ray, float *beta, float *L, const Stack &stk, bool &ovf) {
-  TravCounts tc{0, 0};
-  for (uint32_t s = s0;; s++) {
-    const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
-    if (hit.obj == LAM_NONE || ovf) break;
-    const Surf sf = surface(sc, hit, ray.o, ray.d);
-    const f3 o = neg(normalized(ray.d));
-    const f3 n = sf.n();
-    const lam_material &mat = sc.materials[sf.mat];
-    if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
-      float le[H];
-      spec_eval<H>(sc, mat.spec[0], lam, le);
-      float f = 1.f;
-      if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
-#pragma unroll
-      for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
-      break;
-    }
-    if (s > max_scat) break;
-    // light IS (src/renderer.cpp:91-106)
-    const uint32_t nl = sc.n_lights;
-    const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
-    const float pl = 1.f / float(nl);
-    const lam_light &lt = sc.lights[li];
-    LightSample<H> ls;
-    sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
-                    lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
-    float coef[H];
-    bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
-    bool vis = true;
-    if (ls.shadow) {
-      const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
-      vis = sh.obj == LAM_NONE;
-    }
-    float lis[H];
-#pragma unroll
-    for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
-    // material IS (src/renderer.cpp:116-120)
-    MatSample<H> ms;
-    sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
-                       lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
-                       sample_p, ms);
-    if (!ms.ok) {
-      if (vis) {
-#pragma unroll
-        for (int k = 0; k < H; k++) L[k] += beta[k] * lis[k];
-      }
-      break;
-    }
-    // MIS (src/renderer.cpp:135-145), power heuristic β=2
-    const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
-    const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], sample_p);
-    const float p_L_i = ls.prob * pl;
-    const float w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
-    const float w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
-    if (vis) {
-#pragma unroll
-      for (int k = 0; k < H; k++) L[k] += beta[k] * (lis[k] * w_L);
-    }
-#pragma unroll
-    for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
-    ray.o = sf.p;
-    ray.d = ms.dir;
-    ray.tmax = kFltMax;
-    ray.skip = LAM_NONE;
+  for (uint32_t s = s0; path_vertex<H>(sc, max_scat, sample_p, key, lam, s, ray, beta, L, stk, ovf); s++) {
   }
 }
 

@@ -118,16 +118,6 @@ __global__ void k_pass_next(uint32_t *__restrict__ ctrl, uint32_t stride) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctrl[CTRL_PASS] = ctrl[CTRL_PASS] + stride;
 }
 
-// Work distribution of the persistent trace grids (one 64-lane wave per
-// block): block b first takes chunk b (no atomic), then further chunks
-// gridDim.x + k from one atomic per wave.  A short queue therefore costs no
-// atomics at all; a long one costs one per 64 rays.
-__device__ __forceinline__ uint32_t next_chunk(uint32_t *ctr) {
-  uint32_t k = 0;
-  if (threadIdx.x % 64 == 0) k = atomicAdd(ctr, 1u);
-  return gridDim.x + __shfl(k, 0, 64);
-}
-
 // ---- extend: persistent closest hit ---------------------------------------------------
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
@@ -702,44 +692,63 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
 }
 
 // ---- finish: the tail of the queue, one path per lane to its end ------------------------
+// A lane traces its path one vertex per step and takes the next path of the
+// queue when its own ends (per-lane refill from a wave-local pool, as the
+// trace kernels take rays), so a wave stays busy until the queue is empty
+// instead of idling behind its longest path.
 template <int H>
-__global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
+__global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s0,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
   __shared__ uint32_t stack[kLdsStack * 64];
   if (!ctrl[CTRL_TAIL]) return;
-  const uint32_t lane = threadIdx.x;
   const SplitStack stk = SplitStack::of(stack, ws.spill, blockIdx.x, gridDim.x);
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   const size_t cap = ws.cap;
-  for (uint32_t chunk = blockIdx.x;; chunk = next_chunk(ctrl + CTRL_FETCH_FIN)) {
-    const uint32_t base = chunk * 64u;
-    if (base >= n) break;
-    const uint32_t q = base + lane;
-    if (q < n) {
-      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
-      Ray r;
-      r.o = mk(a.x, a.y, a.z);
-      r.tmax = a.w;
-      r.d = mk(b.x, b.y, b.z);
-      r.skip = LAM_NONE;
-      const uint32_t p = __float_as_uint(b.w);
-      uint32_t pixel, sample;
-      path_id(ids, ctrl, p, pixel, sample);
-      const uint64_t key = lam_path_key(pa.seed, pixel, sample);
-      Lam<H> lam;
-      lam_for<H>(sc, key, lam);
-      float beta[H], L[H];
+  QueuePool qp;
+  qp.init(n);
+  bool active = false, ovf = false;
+  uint32_t p = 0, s = 0;
+  uint64_t key = 0;
+  Lam<H> lam;
+  Ray r;
+  float beta[H], L[H];
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (qp.done(nidle)) break;
+    if (qp.can_refill(nidle)) {
+      const uint32_t q = qp.take(idle, nidle, !active, ctrl + CTRL_FETCH_FIN);
+      if (!active && q < n) {
+        const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
+        r.o = mk(a.x, a.y, a.z);
+        r.tmax = a.w;
+        r.d = mk(b.x, b.y, b.z);
+        r.skip = LAM_NONE;
+        p = __float_as_uint(b.w);
+        uint32_t pixel, sample;
+        path_id(ids, ctrl, p, pixel, sample);
+        key = lam_path_key(pa.seed, pixel, sample);
+        lam_for<H>(sc, key, lam);
 #pragma unroll
-      for (int k = 0; k < H; k++) {
-        beta[k] = ws.beta[size_t(k) * cap + p];
-        L[k] = ws.L[size_t(k) * cap + p];
+        for (int k = 0; k < H; k++) {
+          beta[k] = ws.beta[size_t(k) * cap + p];
+          L[k] = ws.L[size_t(k) * cap + p];
+        }
+        s = s0;
+        ovf = false;
+        active = true;
       }
-      bool ovf = false;
-      path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stk, ovf);
-      if (ovf) atomicOr(ctrl + 15, 1u);
+    }
+    if (active) {
+      if (path_vertex<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stk, ovf)) {
+        s++;
+      } else {
+        if (ovf) atomicOr(ctrl + 15, 1u);
 #pragma unroll
-      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] = L[k];
+        for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] = L[k];
+        active = false;
+      }
     }
   }
 }
