@@ -856,7 +856,12 @@ int lam_scene_upload_ex(lam_ctx *ctx, const lam_scene_desc *d, uint32_t flags) {
   if ((rc = upload(ctx, d->objects, d->n_objects, &sc.objects))) return rc;
   if ((rc = upload(ctx, d->materials, d->n_materials, &sc.materials))) return rc;
   if ((rc = upload(ctx, d->lights, d->n_lights, &sc.lights))) return rc;
-  if ((rc = upload(ctx, d->spectra, size_t(d->n_spectra) * gn, &sc.spectra))) return rc;
+  {
+    // padded to whole float4s: the shade kernel stages the table in LDS 16 B at a time
+    std::vector<float> spec((size_t(d->n_spectra) * gn + 3) / 4 * 4, 0.f);
+    std::copy(d->spectra, d->spectra + size_t(d->n_spectra) * gn, spec.begin());
+    if ((rc = upload(ctx, spec.data(), spec.size(), &sc.spectra))) return rc;
+  }
   if ((rc = upload(ctx, d->cmf, size_t(3) * gn, &sc.cmf))) return rc;
   sc.n_nodes = gpu_bvh ? gst.n_nodes : uint32_t(bvh.nodes.size());
   sc.n_tris = nt;
@@ -864,6 +869,7 @@ int lam_scene_upload_ex(lam_ctx *ctx, const lam_scene_desc *d, uint32_t flags) {
   sc.n_objects = d->n_objects;
   sc.n_materials = d->n_materials;
   sc.n_lights = d->n_lights;
+  sc.n_spectra = d->n_spectra;
   sc.grid_n = gn;
   sc.grid_inf = d->grid.inf;
   sc.grid_sup = d->grid.sup;

@@ -102,6 +102,7 @@ struct DevScene {
   const float *spectra;
   const float *cmf;
   uint32_t n_nodes, n_tris, n_analytic, n_objects, n_materials, n_lights;
+  uint32_t n_spectra;
   uint32_t grid_n;
   float grid_inf, grid_sup, grid_delta;
   lam_camera cam;

@@ -46,6 +46,8 @@ enum : uint32_t {
 
 // threads per shade block (the block appends aggregate this many entries)
 constexpr uint32_t kShadeBlock = 256;
+// largest spectra table the shade kernel stages in LDS (3 blocks per CU)
+constexpr uint32_t kSpecLdsBytes = 48 * 1024;
 
 struct PathIds {
   const uint32_t *pixel;   // nullable: then pixel = slot + pixel_offset
@@ -206,10 +208,21 @@ __device__ __forceinline__ void block_append(BlockQueues<WAVES, NQ> &sm, uint32_
 // 168 VGPRs without spills (2 waves at the default allocation), and the
 // latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
 // whole register file (1 wave).
-template <int H>
+// SPEC_LDS: the coefficient spectra (n_spectra x grid_n floats, at most
+// kSpecLdsBytes) are staged in LDS at the start of the block and every
+// spectral lookup of the block reads them there.
+template <int H, bool SPEC_LDS>
 __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
+  if constexpr (SPEC_LDS) {
+    extern __shared__ float4 spec_lds[];
+    const uint32_t n4 = (sc.n_spectra * sc.grid_n + 3u) / 4u;
+    const float4 *src = reinterpret_cast<const float4 *>(sc.spectra);
+    for (uint32_t i = threadIdx.x; i < n4; i += kShadeBlock) spec_lds[i] = src[i];
+    __syncthreads();
+    sc.spectra = reinterpret_cast<const float *>(spec_lds);
+  }
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
@@ -784,6 +797,8 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
                                  uint32_t n, uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl,
                                  hipStream_t st, WaveProfile *prof, const GuideDev *gd, bool train) {
   const uint32_t b256 = (n + 255) / 256;
+  // (rounded up to whole float4s, as the staging copies them)
+  const uint32_t spec_bytes = (sc.n_spectra * sc.grid_n + 3u) / 4u * 16u;
   auto ev = [&](int slot) {
     if (prof) (void)hipEventRecord(prof->ev[slot], st);
   };
@@ -809,8 +824,12 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     }
     if (gd)
       hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, q, ctrl);
-    if (!gd)
-      hipLaunchKernelGGL((k_shade<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s, q, ctrl);
+    if (!gd && spec_bytes <= kSpecLdsBytes)
+      hipLaunchKernelGGL((k_shade<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
+                         ws, s, q, ctrl);
+    else if (!gd)
+      hipLaunchKernelGGL((k_shade<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s,
+                         q, ctrl);
     else if (train)
       hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gd,
                          s, q, ctrl);
