@@ -225,7 +225,7 @@ struct GuideMix {
   float kappa[kGLobes], w[kGLobes];
 };
 
-__device__ inline void mix_from_raw(const float *raw, GuideMix &m) {
+__device__ __forceinline__ void mix_from_raw(const float *raw, GuideMix &m) {
   float mx = -3.4e38f;
 #pragma unroll
   for (int L = 0; L < kGLobes; L++) mx = fmaxf(mx, raw[5 * L + 4]);
@@ -245,17 +245,17 @@ __device__ inline void mix_from_raw(const float *raw, GuideMix &m) {
 }
 
 // vMF density κ / (2π (1 - e^{-2κ})) · e^{κ(μ·ω - 1)}
-__device__ inline float vmf_pdf(f3 mu, float k, f3 w) {
+__device__ __forceinline__ float vmf_pdf(f3 mu, float k, f3 w) {
   return k / (2.f * kPi * -expm1f(-2.f * k)) * expf(k * (dot(mu, w) - 1.f));
 }
-__device__ inline float mix_pdf(const GuideMix &m, f3 w) {
+__device__ __forceinline__ float mix_pdf(const GuideMix &m, f3 w) {
   float p = 0.f;
 #pragma unroll
   for (int L = 0; L < kGLobes; L++) p += m.w[L] * vmf_pdf(m.mu[L], m.kappa[L], w);
   return p;
 }
 // orthonormal basis around a unit vector (Duff et al. 2017, branchless)
-__device__ inline void basis_of(f3 n, f3 &t, f3 &b) {
+__device__ __forceinline__ void basis_of(f3 n, f3 &t, f3 &b) {
   const float sg = copysignf(1.f, n.z);
   const float a = -1.f / (sg + n.z);
   const float c = n.x * n.y * a;
@@ -263,7 +263,7 @@ __device__ inline void basis_of(f3 n, f3 &t, f3 &b) {
   b = mk(c, sg + n.y * n.y * a, -n.y);
 }
 // vMF sampling, w = 1 + log(ξ + (1-ξ) e^{-2κ}) / κ (Jakob 2012)
-__device__ inline f3 vmf_sample(f3 mu, float k, float u1, float u2) {
+__device__ __forceinline__ f3 vmf_sample(f3 mu, float k, float u1, float u2) {
   const float w = fminf(fmaxf(1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k, -1.f), 1.f);
   const float r = sqrtf(fmaxf(0.f, 1.f - w * w));
   const float ph = 2.f * kPi * u2;
@@ -271,19 +271,27 @@ __device__ inline f3 vmf_sample(f3 mu, float k, float u1, float u2) {
   basis_of(mu, t, b);
   return normalized(add(add(mul(mu, w), mul(t, r * cosf(ph))), mul(b, r * sinf(ph))));
 }
-__device__ inline f3 mix_sample(const GuideMix &m, float ul, float u1, float u2) {
-  // lobe L with cumulative weight above ul (a select chain: the arrays stay
-  // in registers)
-  f3 mu = m.mu[kGLobes - 1];
-  float k = m.kappa[kGLobes - 1], c = 0.f;
+__device__ __forceinline__ f3 mix_sample(const GuideMix &m, float ul, float u1, float u2) {
+  // lobe L with cumulative weight above ul; its parameters are gathered with
+  // 0/1 weights (selects of array elements get folded into an indexed load,
+  // which would move the whole mixture to scratch memory)
+  int sel = kGLobes - 1;
+  float c = 0.f;
   bool found = false;
 #pragma unroll
   for (int L = 0; L < kGLobes - 1; L++) {
     c += m.w[L];
     const bool take = !found && ul < c;
-    mu = take ? m.mu[L] : mu;
-    k = take ? m.kappa[L] : k;
+    sel = take ? L : sel;
     found = found || take;
+  }
+  f3 mu = mk(0.f, 0.f, 0.f);
+  float k = 0.f;
+#pragma unroll
+  for (int L = 0; L < kGLobes; L++) {
+    const float f = sel == L ? 1.f : 0.f;
+    mu = add(mu, mul(m.mu[L], f));
+    k += m.kappa[L] * f;
   }
   return vmf_sample(mu, k, u1, u2);
 }
@@ -293,13 +301,13 @@ __device__ inline f3 mix_sample(const GuideMix &m, float ul, float u1, float u2)
 // all of a Lambertian material (cg.hpp:1755-1778) and the Kd lobe of a Blinn
 // material (cg.hpp:1853-1861).  Glossy and thin-film lobes keep their own
 // samplers.
-__device__ inline bool guide_supported(const lam_material &m) {
+__device__ __forceinline__ bool guide_supported(const lam_material &m) {
   return m.type == LAM_MAT_LAMBERTIAN || (m.type == LAM_MAT_BLINN && m.p_d > 0.f);
 }
 
 // power-heuristic weight a² / (a² + b²) of a strategy with pdf a against one
 // with pdf b; 1 where b = 0 (a pdf that underflowed to 0 leaves 0/0 otherwise)
-__device__ inline float mis_w(float a, float b2) {
+__device__ __forceinline__ float mis_w(float a, float b2) {
   const float a2 = a * a;
   return b2 > 0.f ? a2 / (a2 + b2) : 1.f;
 }

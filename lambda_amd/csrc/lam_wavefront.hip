@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
   }
 }
 
-__device__ inline void load_mix(const uint4 *src, float *raw) {
+__device__ __forceinline__ void load_mix(const uint4 *src, float *raw) {
 #pragma unroll
   for (int c = 0; c < 5; c++) {
     const uint4 v = src[c];
