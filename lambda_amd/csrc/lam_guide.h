@@ -81,21 +81,43 @@ __host__ __device__ inline void g_sincos(float fr, float &s, float &c) {
   c = q == 0 ? cp : (q == 1 ? -sp : (q == 2 ? -cp : sp));
 }
 
+// sin and cos of π 2^j x for j = j0 .. j0 + 3, j0 a multiple of 4: the
+// quarter-turn evaluation above at j0, then double-angle steps
+// (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)), which
+// cost 5 operations instead of a polynomial and double an angle error of a
+// few ulp at most three times
+__host__ __device__ inline void g_sincos_j(float x, int j, float &s, float &c) {
+  const int j0 = j & ~3;
+  const float r = x * float(1 << j0) * .5f;
+  g_sincos(r - floorf(r), s, c);
+  for (int k = j0; k < j; k++) {
+    const float s2 = (s * c) * 2.f, c2 = (c - s) * (c + s);
+    s = s2;
+    c = c2;
+  }
+}
+
 // Feature f of the encoding: f = 24 d + 2 j + c (f < 144) is sin (c = 0) or
-// cos (c = 1) of π 2^j x_d; f = 144 + d is x_d itself; 150..159 are zero.
-// Any 8-aligned block of features holds one input and four frequencies.
+// cos (c = 1) of π 2^j x_d (g_sincos_j); f = 144 + d is x_d itself;
+// 150..159 are zero.  Any 8-aligned block of features holds one input and
+// four frequencies j0 .. j0 + 3 with j0 in {0, 4, 8}.
 __device__ inline float g_feature(int f, const float *in) {
   if (f >= kGFeat) return 0.f;
   if (f >= 144) return in[f - 144];
   const int d = f / 24, j = (f % 24) >> 1;
-  // π 2^j x = 2π (2^(j-1) x)
-  const float x = in[d] * float(1 << j) * .5f;
   float sv, cv;
-  g_sincos(x - floorf(x), sv, cv);
+  g_sincos_j(in[d], j, sv, cv);
   return (f & 1) ? cv : sv;
 }
 
-__device__ inline uint32_t pack_bf2(float a, float b) { return uint32_t(f2bf(a)) | (uint32_t(f2bf(b)) << 16); }
+// two floats as bf16 (round to nearest even, as f2bf) in one word: gfx950's
+// packed conversion instruction
+__device__ inline uint32_t pack_bf2(float a, float b) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  typedef float f32x2_t __attribute__((ext_vector_type(2)));
+  const f32x2_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
 __device__ inline bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 __device__ inline f32x4 mfma16(uint4 a, uint4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
@@ -127,11 +149,16 @@ __device__ inline uint4 feature_chunk(int c, uint32_t g, const GIn &q) {
   float v[8];
   if (f0 < 144) {
     const int d = f0 / 24, j0 = (f0 % 24) >> 1;
-    float x = gin_at(q, d) * float(1 << j0) * .5f;
+    const float r = gin_at(q, d) * float(1 << j0) * .5f;
+    float sv, cv;
+    g_sincos(r - floorf(r), sv, cv);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      g_sincos(x - floorf(x), v[2 * k], v[2 * k + 1]);
-      x *= 2.f;
+      v[2 * k] = sv;
+      v[2 * k + 1] = cv;
+      const float s2 = (sv * cv) * 2.f, c2 = (cv - sv) * (cv + sv);
+      sv = s2;
+      cv = c2;
     }
   } else if (f0 == 144) {
     v[0] = q.x0; v[1] = q.x1; v[2] = q.x2; v[3] = q.x3; v[4] = q.x4; v[5] = q.x5;

@@ -8,7 +8,8 @@ library's own definition:
 
   inputs   6 values in [0,1]
   features f < 144: sin (f even) / cos (f odd) of π 2^j x_d, f = 24 d + 2 j + c,
-           evaluated as the device does (sincos_rev); f = 144 + d: x_d;
+           evaluated as the device does: sincos_rev at j0 = 4 floor(j/4),
+           then j - j0 double-angle steps; f = 144 + d: x_d;
            rounded to bf16
   layers   150 -> 64 -> 64 -> 64 -> 64 (ReLU, activations rounded to bf16)
            -> 40; weights rounded to bf16, biases fp32, sums exact (float64)
@@ -84,10 +85,14 @@ def features(inp):
     """encoded, bf16-rounded features (n x 150)"""
     x = np.asarray(inp, dtype=np.float32).reshape(-1, N_IN)
     cols = []
+    f32 = np.float32
     for f in range(144):
         d, j = f // 24, (f % 24) // 2
-        r = x[:, d] * np.float32(2.0 ** j) * np.float32(0.5)
+        j0 = j & ~3
+        r = x[:, d] * f32(2.0 ** j0) * f32(0.5)
         s, c = sincos_rev(r - np.floor(r))
+        for _ in range(j - j0):  # double-angle steps, float32 as on the device
+            s, c = (s * c) * f32(2), (c - s) * (c + s)
         cols.append((c if f % 2 else s).astype(np.float64))
     for d in range(N_IN):
         cols.append(x[:, d].astype(np.float64))

@@ -105,6 +105,22 @@ def test_features_layout():
     assert abs(F[24 * 1 + 2 * 1 + 1] - np.cos(2 * np.pi * 0.25)) < 1e-2  # cos(2π x_1)
 
 
+def test_features_match_sin_cos():
+    """every encoded feature (quarter-turn polynomial at j = 0, 4, 8, then
+    double-angle steps) is sin/cos(π 2^j x_d) of the float32 input to within
+    half a bf16 step"""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(0, 1, (20000, 6)), [[0, 1, 0.5, 0.25, 1 - 2 ** -24, 2 ** -20]]]).astype(np.float32)
+    F = og.features(x)
+    xd = x.astype(np.float64)
+    for f in range(144):
+        d, j = f // 24, (f % 24) // 2
+        a = np.pi * 2.0 ** j * xd[:, d]
+        want = np.cos(a) if f % 2 else np.sin(a)
+        # half a bf16 step in the binade [2^e, 2^(e+1)) is 2^(e-8) <= 2^-8 |v|
+        assert np.all(np.abs(F[:, f] - want) <= 2.0 ** -8 * np.maximum(np.abs(want), 2.0 ** -6) + 2e-6), f
+
+
 # ---- GPU ---------------------------------------------------------------------------
 
 @pytest.fixture(scope="module")
