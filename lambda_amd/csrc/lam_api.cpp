@@ -158,6 +158,9 @@ lam::PathArgs path_args(const lam_render_params *p) {
 int check_params(lam_ctx *ctx, const lam_render_params *p) {
   if (!p) return fail(ctx, LAM_E_INVALID, "params is NULL");
   if (p->hero != 4 && p->hero != 16) return fail(ctx, LAM_E_INVALID, "hero must be 4 or 16");
+  // Renderer::render divides the film by SPP·HERO_SAMPLES (renderer.cpp:51-52):
+  // no samples would be 0/0 in every pixel
+  if (p->spp == 0) return fail(ctx, LAM_E_INVALID, "spp must be at least 1");
   if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
   if (ctx->sc.n_lights == 0) return fail(ctx, LAM_E_INVALID, "scene has no lights");
   return LAM_OK;
