@@ -339,6 +339,30 @@ def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
             "grad_finite": bool(np.isfinite(grad).all()), "grad_nonzero": int(np.count_nonzero(grad))}
 
 
+def gpu_bvh_bench(bounces=32, p=0.8, spp=4):
+    """The icosphere field's BVH8 built on the device (LAM_UPLOAD_GPU_BVH)
+    next to the host builder: upload time, tree shape, render time and the
+    image (equal bit for bit: the closest hit does not depend on the tree)."""
+    scene = la.load_nls(scene_path())
+    ctx = la.Context(0)
+    pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=4, seed=1)
+    out = {"workload": f"icosphere field ({scene.n_tris:,} tris), {scene.width}x{scene.height}, {spp} spp"}
+    imgs = []
+    for name, gpu in (("host", False), ("gpu", True)):
+        t0 = time.perf_counter()
+        ctx.upload(scene, gpu_bvh=gpu)
+        t_up = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.render(la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=4, seed=9))
+        t0 = time.perf_counter()
+        imgs.append(ctx.render(pr))
+        out[name] = {"upload_s": round(t_up, 3), "nodes": st.n_nodes, "depth": st.depth,
+                     "stack_bound": st.stack_bound, "render_s": round(time.perf_counter() - t0, 4)}
+    out["same_image"] = bool(np.array_equal(*imgs))
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,8 +374,10 @@ def main():
     ap.add_argument("--no-guiding", action="store_true")
     ap.add_argument("--legs", default="render,roofline,adjoint,guiding,cpu",
                     help="comma list of legs to run (profiling runs pick one); 'stress' (not in the default) "
-                         "adds the 10 M-triangle 4K guided + adjoint run")
+                         "adds the 10 M-triangle 4K guided + adjoint run, 'gpu_bvh' (not in the default) the "
+                         "device BVH build of the render scene")
     ap.add_argument("--guide-spp", type=int, default=1024)
+    ap.add_argument("--guide-train-passes", type=int, default=32)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -377,7 +403,9 @@ def main():
         if "adjoint" in legs:
             out["adjoint"] = adjoint_bench()
         if "guiding" in legs:
-            out["guiding"] = guiding_bench(spp=args.guide_spp)
+            out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes)
+        if "gpu_bvh" in legs:
+            out["gpu_bvh"] = gpu_bvh_bench()
         print(json.dumps(out), flush=True)
         return
     bounces, p, hero = 32, 0.8, 4
@@ -431,7 +459,7 @@ def main():
         out["adjoint"] = adjoint_bench()
         log(f"adjoint: {out['adjoint']}")
     if "guiding" in legs:
-        out["guiding"] = guiding_bench(spp=args.guide_spp)
+        out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes)
         log(f"guiding: {out['guiding']}")
     if "stress" in legs:
         out["stress"] = stress_bench()
