@@ -135,86 +135,117 @@ __device__ bool differentiable(const DiffDev &dv, const lam_material &m, uint32_
   return false;
 }
 
-// Replays one path: Ltot is its total radiance, a[k] the adjoint weight of
-// wavelength k (∂loss/∂L_k).
+// One vertex of a path in either pass, with the operations of path_vertex
+// (lam_shade.h).  Forward (replay = false): the collected terms are added
+// to L, which ends as the path's total radiance.  Replay (replay = true): L
+// holds the radiance still to come; each collected term is subtracted and
+// the parameter gradients of the vertex are accumulated into gl, weighted by
+// a[k] = ∂loss/∂L_k.  Sharing the code keeps lanes in different passes on
+// the same instructions.  Returns false at the end of the pass.
 template <int H, class Stack>
-__device__ void adjoint_path(const DevScene &sc, const DiffDev &dv, uint32_t max_scat, float sample_p, uint64_t key,
-                             const Lam<H> &lam, Ray ray, const float *Ltot, const float *a, float *gl,
-                             const Stack &stk, bool &ovf) {
-  float beta[H], Lrem[H];
-#pragma unroll
-  for (int k = 0; k < H; k++) { beta[k] = 1.f; Lrem[k] = Ltot[k]; }
+__device__ __forceinline__ bool adjoint_vertex(const DevScene &sc, const DiffDev &dv, uint32_t max_scat,
+                                               float sample_p, uint64_t key, const Lam<H> &lam, uint32_t s,
+                                               bool replay, Ray &ray, float *beta, float *L, const float *a,
+                                               float *gl, const Stack &stk, bool &ovf) {
   TravCounts tc{0, 0};
-  for (uint32_t s = 0;; s++) {
-    const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
-    if (hit.obj == LAM_NONE || ovf) break;
-    const Surf sf = surface(sc, hit, ray.o, ray.d);
-    const f3 o = neg(normalized(ray.d));
-    const f3 n = sf.n();
-    const lam_material &mat = sc.materials[sf.mat];
-    if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) break;
-    if (s > max_scat) break;
-    const uint32_t nl = sc.n_lights;
-    const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
-    const float pl = 1.f / float(nl);
-    const lam_light &lt = sc.lights[li];
-    LightSample<H> ls;
-    sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
-                    lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
-    float coef[H];
-    bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
-    bool vis = true;
-    if (ls.shadow) {
-      const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
-      vis = sh.obj == LAM_NONE;
-    }
-    float lis[H];
+  const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
+  if (hit.obj == LAM_NONE || ovf) return false;
+  const Surf sf = surface(sc, hit, ray.o, ray.d);
+  const f3 o = neg(normalized(ray.d));
+  const f3 n = sf.n();
+  const lam_material &mat = sc.materials[sf.mat];
+  if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
+    // the last term: in the replay it is what L still holds
+    if (!replay) {
+      float le[H];
+      spec_eval<H>(sc, mat.spec[0], lam, le);
+      float f = 1.f;
+      if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
 #pragma unroll
-    for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
-    MatSample<H> ms;
-    sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
-                       lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
-                       sample_p, ms);
-    const bool diff = differentiable<H>(dv, mat, sf.mat);
-    float w_L = 1.f, w_mat = 0.f;
-    if (ms.ok) {
-      const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
-      const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], sample_p);
-      const float p_L_i = ls.prob * pl;
-      w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
-      w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+      for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
     }
-    if (vis) {
-      float w[H];
-#pragma unroll
-      for (int k = 0; k < H; k++) {
-        const float term = ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
-        Lrem[k] -= term;
-        // ∂term/∂coef = β w_L mult / (p_L p_l)
-        w[k] = a[k] * (beta[k] * w_L * ls.mult[k] / (ls.prob * pl));
-      }
-      if (diff) accumulate<H>(sc, dv, mat, sf.mat, lam, ls.dir, o, n, sf.front, w, false, gl);
-    }
-    if (!ms.ok) break;
-    if (diff) {
-      float w[H];
-#pragma unroll
-      for (int k = 0; k < H; k++) w[k] = a[k] * Lrem[k];
-      accumulate<H>(sc, dv, mat, sf.mat, lam, ms.dir, o, n, sf.front, w, true, gl);
-    }
-#pragma unroll
-    for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
-    ray.o = sf.p;
-    ray.d = ms.dir;
-    ray.tmax = kFltMax;
-    ray.skip = LAM_NONE;
+    return false;
   }
+  if (s > max_scat) return false;
+  const uint32_t nl = sc.n_lights;
+  const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+  const float pl = 1.f / float(nl);
+  const lam_light &lt = sc.lights[li];
+  LightSample<H> ls;
+  sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                  lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+  float coef[H];
+  bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
+  bool vis = true;
+  if (ls.shadow) {
+    const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
+    vis = sh.obj == LAM_NONE;
+  }
+  float lis[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+  MatSample<H> ms;
+  sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
+                     lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
+                     sample_p, ms);
+  const bool diff = replay && differentiable<H>(dv, mat, sf.mat);
+  float w_L = 1.f, w_mat = 0.f;
+  if (ms.ok) {
+    const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+    const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], sample_p);
+    const float p_L_i = ls.prob * pl;
+    w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+    w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+  }
+  if (vis) {
+    float w[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      const float term = ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
+      if (replay) L[k] -= term;
+      else L[k] += term;
+      // ∂term/∂coef = β w_L mult / (p_L p_l)
+      w[k] = a[k] * (beta[k] * w_L * ls.mult[k] / (ls.prob * pl));
+    }
+    if (diff) accumulate<H>(sc, dv, mat, sf.mat, lam, ls.dir, o, n, sf.front, w, false, gl);
+  }
+  if (!ms.ok) return false;
+  if (diff) {
+    float w[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) w[k] = a[k] * L[k];
+    accumulate<H>(sc, dv, mat, sf.mat, lam, ms.dir, o, n, sf.front, w, true, gl);
+  }
+#pragma unroll
+  for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
+  ray.o = sf.p;
+  ray.d = ms.dir;
+  ray.tmax = kFltMax;
+  ray.skip = LAM_NONE;
+  return true;
 }
 
-// One sample per pixel: forward for the path's radiance, adjoint weights
-// from ∂loss/∂rgb of its pixel (the film's linear map, cg.hpp:144-148 and
-// 306-341), then the replay.  Gradients gather in LDS and leave with one
-// atomic per parameter per block.
+// camera ray of a path (renderer.cpp:33-37 with the counter RNG)
+__device__ __forceinline__ Ray adjoint_camera_ray(const DevScene &sc, uint64_t key, uint32_t pixel) {
+  const uint32_t w = sc.cam.width;
+  const uint32_t row = pixel / w, colj = pixel - row * w;
+  const float u = float(colj) + lam_draw_f(key, LAM_DIM_U);
+  const float v = float(row) + lam_draw_f(key, LAM_DIM_V);
+  Ray ray;
+  camera_ray(sc.cam, u, v, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), ray.o, ray.d);
+  ray.tmax = kFltMax;
+  ray.skip = LAM_NONE;
+  return ray;
+}
+
+// One sample per pixel: for each path the forward pass (its total
+// radiance), adjoint weights from ∂loss/∂rgb of its pixel (the film's
+// linear map, cg.hpp:144-148 and 306-341), then the replay.  Each lane runs
+// its own path one vertex per step and takes the next pixel when its replay
+// ends (the queue pool of the traversal kernels: a block starts with pixels
+// 64b..64b+63, then reserves runs from *ctr, which starts at 0), so a wave
+// never waits for its longest path.  Gradients gather in LDS and leave with
+// one atomic per parameter per block.
 template <int H>
 __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDev dv, uint32_t sample, uint32_t npix,
                                                 float inv_n, const float *__restrict__ dloss, float *__restrict__ grad,
@@ -227,49 +258,66 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
   for (uint32_t j = lane; j < dv.n_params; j += 64) gl[j] = 0.f;
   __syncthreads();
   const uint32_t w = sc.cam.width, h = sc.cam.height;
-  for (uint32_t chunk = blockIdx.x;;) {
-    const uint32_t base = chunk * 64u;
-    if (base >= npix) break;
-    const uint32_t pixel = base + lane;
-    if (pixel < npix) {
-      const uint64_t key = lam_path_key(pa.seed, pixel, sample);
-      const uint32_t row = pixel / w, colj = pixel - row * w;
-      const float u = float(colj) + lam_draw_f(key, LAM_DIM_U);
-      const float v = float(row) + lam_draw_f(key, LAM_DIM_V);
-      Ray ray;
-      camera_ray(sc.cam, u, v, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), ray.o, ray.d);
-      ray.tmax = kFltMax;
-      ray.skip = LAM_NONE;
-      Lam<H> lam;
-      const float prob_l = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
-      // adjoint weight of each wavelength: ∂loss/∂L_k
-      const size_t img = (size_t(h - row - 1) * w + colj) * 3;
-      const float g0 = dloss[img], g1 = dloss[img + 1], g2 = dloss[img + 2];
-      const float gx = 3.2404542f * g0 - 0.9692660f * g1 + 0.0556434f * g2;
-      const float gy = -1.5371385f * g0 + 1.8760108f * g1 - 0.2040259f * g2;
-      const float gz = -0.4985314f * g0 + 0.0415560f * g1 + 1.0572252f * g2;
-      float a[H];
-      bool any = false;
+  QueuePool qp;
+  qp.init(npix);
+  bool active = false, replay = false, ovf = false;
+  uint32_t pixel = 0, s = 0;
+  uint64_t key = 0;
+  Lam<H> lam;
+  float a[H], beta[H], L[H];
+  Ray ray;
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (qp.done(nidle)) break;
+    if (qp.can_refill(nidle)) {
+      const uint32_t i = qp.take(idle, nidle, !active, ctr);
+      if (!active && i < npix) {
+        pixel = i;
+        key = lam_path_key(pa.seed, pixel, sample);
+        ray = adjoint_camera_ray(sc, key, pixel);
+        const float prob_l = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
+        // adjoint weight of each wavelength: ∂loss/∂L_k
+        const uint32_t row = pixel / w, colj = pixel - row * w;
+        const size_t img = (size_t(h - row - 1) * w + colj) * 3;
+        const float g0 = dloss[img], g1 = dloss[img + 1], g2 = dloss[img + 2];
+        const float gx = 3.2404542f * g0 - 0.9692660f * g1 + 0.0556434f * g2;
+        const float gy = -1.5371385f * g0 + 1.8760108f * g1 - 0.2040259f * g2;
+        const float gz = -0.4985314f * g0 + 0.0415560f * g1 + 1.0572252f * g2;
+        bool any = false;
 #pragma unroll
-      for (int k = 0; k < H; k++) {
-        float X = 0.f, Y = 0.f, Z = 0.f;
-        splat_xyz(sc, lam.l[k], 1.f, X, Y, Z);
-        a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
-        any = any || a[k] != 0.f;
-      }
-      if (any) {
-        float Lt[H], beta[H];
-#pragma unroll
-        for (int k = 0; k < H; k++) { Lt[k] = 0.f; beta[k] = 1.f; }
-        bool ovf = false;
-        path_loop<H>(sc, pa.max_scat, pa.sample_p, key, lam, 0u, ray, beta, Lt, stk, ovf);
-        adjoint_path<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, ray, Lt, a, gl, stk, ovf);
-        if (ovf) atomicOr(err, 1u);
+        for (int k = 0; k < H; k++) {
+          float X = 0.f, Y = 0.f, Z = 0.f;
+          splat_xyz(sc, lam.l[k], 1.f, X, Y, Z);
+          a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
+          any = any || a[k] != 0.f;
+          beta[k] = 1.f;
+          L[k] = 0.f;
+        }
+        active = any;
+        replay = false;
+        s = 0;
       }
     }
-    uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(ctr, 1u);
-    chunk = gridDim.x + __shfl(k, 0, 64);
+    if (active) {
+      const bool more = adjoint_vertex<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, s, replay, ray, beta, L, a,
+                                          gl, stk, ovf);
+      s++;
+      if (!more) {
+        if (ovf) {
+          atomicOr(err, 1u);
+          active = false;
+        } else if (!replay) {  // L is the path's total: walk it again
+          replay = true;
+          s = 0;
+          ray = adjoint_camera_ray(sc, key, pixel);
+#pragma unroll
+          for (int k = 0; k < H; k++) beta[k] = 1.f;
+        } else {
+          active = false;
+        }
+      }
+    }
   }
   __syncthreads();
   for (uint32_t j = lane; j < dv.n_params; j += 64)
