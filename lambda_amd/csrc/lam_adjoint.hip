@@ -9,8 +9,9 @@
 // term of vertex j).  With the random numbers of the path fixed:
 //   ∂L/∂θ = Σ_j (∂f_j/∂θ / f_j) ⊙ L_after(j) + Σ_j β_j ⊙ w_L,j mult_j ∂coef_j/∂θ / p_L,j
 // where L_after(j) is the radiance collected after vertex j's NEE term.
-// The replay first runs the path forward for its total L, then walks it
-// again subtracting each collected term, so it stores nothing per vertex.
+// The forward pass (the render's wavefront pass) leaves each path's total
+// L; the replay walks the path again subtracting each collected term, so
+// nothing is stored per vertex.
 // None of the differentiated parameters changes a sampled direction, pdf or
 // Russian-roulette decision (Lambertian sampling is cosine, thin-film
 // sampling is the Blinn half-vector of α, cg.hpp:1755-1931).
@@ -135,18 +136,15 @@ __device__ bool differentiable(const DiffDev &dv, const lam_material &m, uint32_
   return false;
 }
 
-// One vertex of a path in either pass, with the operations of path_vertex
-// (lam_shade.h).  Forward (replay = false): the collected terms are added
-// to L, which ends as the path's total radiance.  Replay (replay = true): L
-// holds the radiance still to come; each collected term is subtracted and
-// the parameter gradients of the vertex are accumulated into gl, weighted by
-// a[k] = ∂loss/∂L_k.  Sharing the code keeps lanes in different passes on
-// the same instructions.  Returns false at the end of the pass.
+// One vertex of the replay, with the operations of path_vertex (lam_shade.h):
+// L holds the radiance the path still collects; each collected term is
+// subtracted, and the parameter gradients of the vertex are accumulated
+// into gl, weighted by a[k] = ∂loss/∂L_k.  Returns false at the path's end.
 template <int H, class Stack>
-__device__ __forceinline__ bool adjoint_vertex(const DevScene &sc, const DiffDev &dv, uint32_t max_scat,
-                                               float sample_p, uint64_t key, const Lam<H> &lam, uint32_t s,
-                                               bool replay, Ray &ray, float *beta, float *L, const float *a,
-                                               float *gl, const Stack &stk, bool &ovf) {
+__device__ __forceinline__ bool replay_vertex(const DevScene &sc, const DiffDev &dv, uint32_t max_scat,
+                                              float sample_p, uint64_t key, const Lam<H> &lam, uint32_t s, Ray &ray,
+                                              float *beta, float *L, const float *a, float *gl, const Stack &stk,
+                                              bool &ovf) {
   TravCounts tc{0, 0};
   const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
   if (hit.obj == LAM_NONE || ovf) return false;
@@ -154,18 +152,8 @@ __device__ __forceinline__ bool adjoint_vertex(const DevScene &sc, const DiffDev
   const f3 o = neg(normalized(ray.d));
   const f3 n = sf.n();
   const lam_material &mat = sc.materials[sf.mat];
-  if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) {
-    // the last term: in the replay it is what L still holds
-    if (!replay) {
-      float le[H];
-      spec_eval<H>(sc, mat.spec[0], lam, le);
-      float f = 1.f;
-      if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
-#pragma unroll
-      for (int k = 0; k < H; k++) L[k] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
-    }
-    return false;
-  }
+  // an emitter's term is the path's last: it is what L still holds
+  if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) return false;
   if (s > max_scat) return false;
   const uint32_t nl = sc.n_lights;
   const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
@@ -188,7 +176,7 @@ __device__ __forceinline__ bool adjoint_vertex(const DevScene &sc, const DiffDev
   sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
                      lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
                      sample_p, ms);
-  const bool diff = replay && differentiable<H>(dv, mat, sf.mat);
+  const bool diff = differentiable<H>(dv, mat, sf.mat);
   float w_L = 1.f, w_mat = 0.f;
   if (ms.ok) {
     const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
@@ -201,9 +189,7 @@ __device__ __forceinline__ bool adjoint_vertex(const DevScene &sc, const DiffDev
     float w[H];
 #pragma unroll
     for (int k = 0; k < H; k++) {
-      const float term = ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
-      if (replay) L[k] -= term;
-      else L[k] += term;
+      L[k] -= ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
       // ∂term/∂coef = β w_L mult / (p_L p_l)
       w[k] = a[k] * (beta[k] * w_L * ls.mult[k] / (ls.prob * pl));
     }
@@ -238,16 +224,20 @@ __device__ __forceinline__ Ray adjoint_camera_ray(const DevScene &sc, uint64_t k
   return ray;
 }
 
-// One sample per pixel: for each path the forward pass (its total
-// radiance), adjoint weights from ∂loss/∂rgb of its pixel (the film's
-// linear map, cg.hpp:144-148 and 306-341), then the replay.  Each lane runs
-// its own path one vertex per step and takes the next pixel when its replay
-// ends (the queue pool of the traversal kernels: a block starts with pixels
-// 64b..64b+63, then reserves runs from *ctr, which starts at 0), so a wave
-// never waits for its longest path.  Gradients gather in LDS and leave with
-// one atomic per parameter per block.
+// The replay of one sample per pixel for the n paths of a chunk (pixels
+// pixel_offset .. pixel_offset + n - 1).  The forward pass is the render's
+// wavefront pass, run just before on the same paths: Lfwd[k·lcap + i] is the
+// total radiance of path i at wavelength k (the same value the vertex
+// operations of the replay add up to).  Adjoint weights come from
+// ∂loss/∂rgb of the path's pixel (the film's linear map, cg.hpp:144-148 and
+// 306-341).  Each lane replays its own path one vertex per step and takes
+// the next path when its replay ends (the queue pool of the traversal
+// kernels: a block starts with paths 64b..64b+63, then reserves runs from
+// *ctr, which starts at 0), so a wave never waits for its longest path.
+// Gradients gather in LDS and leave with one atomic per parameter per block.
 template <int H>
-__global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDev dv, uint32_t sample, uint32_t npix,
+__global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDev dv, uint32_t sample, uint32_t n,
+                                                uint32_t pixel_offset, const float *__restrict__ Lfwd, size_t lcap,
                                                 float inv_n, const float *__restrict__ dloss, float *__restrict__ grad,
                                                 uint32_t *__restrict__ ctr, uint32_t *__restrict__ err,
                                                 uint32_t *__restrict__ spill) {
@@ -259,9 +249,9 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
   __syncthreads();
   const uint32_t w = sc.cam.width, h = sc.cam.height;
   QueuePool qp;
-  qp.init(npix);
-  bool active = false, replay = false, ovf = false;
-  uint32_t pixel = 0, s = 0;
+  qp.init(n);
+  bool active = false, ovf = false;
+  uint32_t s = 0;
   uint64_t key = 0;
   Lam<H> lam;
   float a[H], beta[H], L[H];
@@ -272,8 +262,8 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
     if (qp.done(nidle)) break;
     if (qp.can_refill(nidle)) {
       const uint32_t i = qp.take(idle, nidle, !active, ctr);
-      if (!active && i < npix) {
-        pixel = i;
+      if (!active && i < n) {
+        const uint32_t pixel = pixel_offset + i;
         key = lam_path_key(pa.seed, pixel, sample);
         ray = adjoint_camera_ray(sc, key, pixel);
         const float prob_l = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
@@ -292,30 +282,18 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
           a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
           any = any || a[k] != 0.f;
           beta[k] = 1.f;
-          L[k] = 0.f;
+          L[k] = Lfwd[size_t(k) * lcap + i];
         }
         active = any;
-        replay = false;
         s = 0;
       }
     }
     if (active) {
-      const bool more = adjoint_vertex<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, s, replay, ray, beta, L, a,
-                                          gl, stk, ovf);
+      active = replay_vertex<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, s, ray, beta, L, a, gl, stk, ovf);
       s++;
-      if (!more) {
-        if (ovf) {
-          atomicOr(err, 1u);
-          active = false;
-        } else if (!replay) {  // L is the path's total: walk it again
-          replay = true;
-          s = 0;
-          ray = adjoint_camera_ray(sc, key, pixel);
-#pragma unroll
-          for (int k = 0; k < H; k++) beta[k] = 1.f;
-        } else {
-          active = false;
-        }
+      if (ovf) {
+        atomicOr(err, 1u);
+        active = false;
       }
     }
   }
@@ -326,16 +304,17 @@ __global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDe
 
 hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
                           const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
-                          uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
-                          uint32_t *spill, uint32_t blocks, hipStream_t st) {
+                          uint32_t n, uint32_t pixel_offset, const float *Lfwd, size_t lcap, float inv_n,
+                          const float *dloss, float *grad, uint32_t *ctr, uint32_t *err, uint32_t *spill,
+                          uint32_t blocks, hipStream_t st) {
   DiffDev dv{mat_d, mat_ior, mat_albedo, n_params};
   const size_t shmem = size_t(n_params) * 4;
   if (hero == 16)
-    hipLaunchKernelGGL((k_adjoint<16>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, npix, inv_n, dloss,
-                       grad, ctr, err, spill);
+    hipLaunchKernelGGL((k_adjoint<16>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, n, pixel_offset, Lfwd,
+                       lcap, inv_n, dloss, grad, ctr, err, spill);
   else
-    hipLaunchKernelGGL((k_adjoint<4>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, npix, inv_n, dloss,
-                       grad, ctr, err, spill);
+    hipLaunchKernelGGL((k_adjoint<4>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, n, pixel_offset, Lfwd,
+                       lcap, inv_n, dloss, grad, ctr, err, spill);
   return hipGetLastError();
 }
 

@@ -1090,11 +1090,11 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
   const size_t npix = size_t(w) * h;
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
-  // scratch: [0] parameter maps, [1] dloss, [2] gradient + fetch counter
+  // scratch: [0] parameter maps, [1] dloss, [2] gradient + one fetch counter per lane
   const size_t maps = size_t(nm) * 3 * 4;
   if ((rc = ensure_scratch(ctx, 0, maps))) return rc;
   if ((rc = ensure_scratch(ctx, 1, npix * 12))) return rc;
-  if ((rc = ensure_scratch(ctx, 2, size_t(dp->n_params) * 4 + 64))) return rc;
+  if ((rc = ensure_scratch(ctx, 2, size_t(dp->n_params) * 4 + 16 * kMaxLanes + 16))) return rc;
   int32_t *dmaps = static_cast<int32_t *>(ctx->scratch[0]);
   HIPC(hipMemcpyAsync(dmaps, dp->mat_d, nm * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(dmaps + nm, dp->mat_ior, nm * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1102,24 +1102,62 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   HIPC(hipMemcpyAsync(ctx->scratch[1], dloss_drgb, npix * 12, hipMemcpyHostToDevice, ctx->stream));
   float *dgrad = static_cast<float *>(ctx->scratch[2]);
   const size_t gbytes = (size_t(dp->n_params) * 4 + 15) / 16 * 16;
-  uint32_t *ctr = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ctx->scratch[2]) + gbytes);
+  uint32_t *ctr0 = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ctx->scratch[2]) + gbytes);
   HIPC(lam::launch_zero(dgrad, gbytes, ctx->stream));
   if ((rc = clear_err_flag(ctx))) return rc;
-  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
+  // Forward passes: the render's wavefront pass without a film, which
+  // leaves each path's total radiance in its lane's path state; the replay
+  // of the pass follows on the same lane.  Consecutive passes go round-robin
+  // to the pass lanes, so one lane's forward pass runs beside the others'
+  // replays; every replay adds into the one gradient with atomics.
+  const size_t cap = std::min(npix, kMaxWaveCap);
+  const uint32_t lanes = (p->flags & LAM_FLAG_NO_PIPE) ? 1u : std::min(ctx->lanes, p->spp);
+  hipStream_t st[kMaxLanes];
+  for (uint32_t l = 0; l < lanes; l++) {
+    if ((rc = ensure_wave(ctx, cap, int(p->hero), int(l)))) return rc;
+    st[l] = l == 0 ? ctx->stream : ctx->lane[l].stream;
+    HIPC(lam::launch_zero(ctx->lane[l].ctrl, lam::kCtrlWords * 4, ctx->stream));
+  }
+  const lam::PathArgs pa = path_args(p);
+  lam::WaveLaunch wl = launch_for(ctx, p);
   // 8 blocks per CU (within the finish grid the spill columns are sized for)
   const uint32_t blocks = std::min(uint32_t(ctx->cus) * 8u, ctx->wl.finish_blocks);
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
-  for (uint32_t s = 0; s < p->spp; s++) {
-    HIPC(lam::launch_zero(ctr, 16, ctx->stream));
-    HIPC(lam::launch_adjoint(int(p->hero), ctx->sc, path_args(p), dmaps, dmaps + nm, dmaps + 2 * nm, dp->n_params,
-                             p->spp_offset + s, uint32_t(npix), inv_n, static_cast<float *>(ctx->scratch[1]), dgrad,
-                             ctr, ctx->d_err, ctx->lane[0].spill, blocks, ctx->stream));
+  HIPC(hipEventRecord(ctx->lane[0].ev_join, ctx->stream));
+  for (uint32_t l = 1; l < lanes; l++) HIPC(hipStreamWaitEvent(st[l], ctx->lane[0].ev_join, 0));
+  hipError_t e = hipSuccess;
+  for (size_t off = 0; off < npix && e == hipSuccess; off += cap) {
+    const uint32_t n = uint32_t(std::min(cap, npix - off));
+    wl.tail_threshold = std::min(launch_for(ctx, p).tail_threshold, n / ctx->tail_div);
+    for (uint32_t s = 0; s < p->spp && e == hipSuccess; s++) {
+      const uint32_t l = s % lanes;
+      lam_ctx::Lane &ln = ctx->lane[l];
+      uint32_t *ctr = ctr0 + 4 * l;
+      e = lam::enqueue_pass_set(ln.ctrl, p->spp_offset + s, st[l]);
+      if (e == hipSuccess)
+        e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ln.ws, n, ln.ctrl, nullptr,
+                              nullptr, wl, st[l], nullptr, nullptr, false);
+      if (e == hipSuccess) e = lam::launch_zero(ctr, 16, st[l]);
+      if (e == hipSuccess)
+        e = lam::launch_adjoint(int(p->hero), ctx->sc, pa, dmaps, dmaps + nm, dmaps + 2 * nm, dp->n_params,
+                                p->spp_offset + s, n, uint32_t(off), ln.ws.L, ln.ws.cap, inv_n,
+                                static_cast<float *>(ctx->scratch[1]), dgrad, ctr, ctx->d_err, ln.spill, blocks,
+                                st[l]);
+    }
   }
+  // join: stream 0 waits for the other lanes
+  for (uint32_t l = 1; l < lanes; l++) {
+    hipError_t ej = hipEventRecord(ctx->lane[l].ev_join, st[l]);
+    if (ej == hipSuccess) ej = hipStreamWaitEvent(ctx->stream, ctx->lane[l].ev_join, 0);
+    if (e == hipSuccess) e = ej;
+  }
+  HIPC(e);
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipMemcpyAsync(grad, dgrad, size_t(dp->n_params) * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
 

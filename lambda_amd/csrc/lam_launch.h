@@ -138,8 +138,9 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
 constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block
 hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
                           const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
-                          uint32_t npix, float inv_n, const float *dloss, float *grad, uint32_t *ctr, uint32_t *err,
-                          uint32_t *spill, uint32_t blocks, hipStream_t st);
+                          uint32_t n, uint32_t pixel_offset, const float *Lfwd, size_t lcap, float inv_n,
+                          const float *dloss, float *grad, uint32_t *ctr, uint32_t *err, uint32_t *spill,
+                          uint32_t blocks, hipStream_t st);
 hipError_t enqueue_pass_next(uint32_t *ctrl, uint32_t stride, hipStream_t st);
 
 }  // namespace lam

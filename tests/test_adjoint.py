@@ -198,6 +198,23 @@ def test_adjoint_linear_in_dloss(ctx, film_scene):
 
 
 @pytest.mark.gpu
+def test_adjoint_lanes_agree(ctx, film_scene):
+    """forward passes and replays spread over the pass lanes give the
+    one-lane gradient up to the order of the float atomics, also with more
+    samples than lanes and an odd sample count"""
+    dp = _dp(film_scene)
+    rng = np.random.default_rng(4)
+    W = rng.normal(size=(film_scene.height, film_scene.width, 3)).astype(np.float32)
+    ctx.upload(film_scene)
+    for spp in (1, 5):
+        a = ctx.adjoint(la.params(spp=spp, max_scatterings=6, sample_p=0.8, hero=4, seed=2), dp, W)
+        b = ctx.adjoint(la.params(spp=spp, max_scatterings=6, sample_p=0.8, hero=4, seed=2,
+                                  flags=la.LAM_FLAG_NO_PIPE), dp, W)
+        assert np.count_nonzero(a) > 0
+        assert np.allclose(a, b, rtol=1e-4, atol=1e-5 * np.abs(b).max()), spp
+
+
+@pytest.mark.gpu
 def test_adjoint_argument_errors(ctx, film_scene):
     dp = _dp(film_scene)
     p = la.params(spp=1, max_scatterings=2, sample_p=0.8, hero=4)
