@@ -155,12 +155,14 @@ lam::PathArgs path_args(const lam_render_params *p) {
   return pa;
 }
 
-int check_params(lam_ctx *ctx, const lam_render_params *p) {
+// samples: the call renders p->spp samples per pixel (lam_trace_paths
+// traces the (pixel, sample) pairs it is given instead)
+int check_params(lam_ctx *ctx, const lam_render_params *p, bool samples = true) {
   if (!p) return fail(ctx, LAM_E_INVALID, "params is NULL");
   if (p->hero != 4 && p->hero != 16) return fail(ctx, LAM_E_INVALID, "hero must be 4 or 16");
   // Renderer::render divides the film by SPP·HERO_SAMPLES (renderer.cpp:51-52):
   // no samples would be 0/0 in every pixel
-  if (p->spp == 0) return fail(ctx, LAM_E_INVALID, "spp must be at least 1");
+  if (samples && p->spp == 0) return fail(ctx, LAM_E_INVALID, "spp must be at least 1");
   if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
   if (ctx->sc.n_lights == 0) return fail(ctx, LAM_E_INVALID, "scene has no lights");
   return LAM_OK;
@@ -991,7 +993,7 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p, const uint32_t *pi
                     size_t n, float *out) {
   if (!ctx) return LAM_E_INVALID;
   int rc;
-  if ((rc = check_params(ctx, p))) return rc;
+  if ((rc = check_params(ctx, p, false))) return rc;
   if (n == 0) return LAM_OK;
   if (!pixel || !sample || !out) return fail(ctx, LAM_E_INVALID, "NULL buffer");
   if (n > (size_t(1) << 28)) return fail(ctx, LAM_E_LIMIT, "path batch too large");
