@@ -366,7 +366,8 @@ def gpu_bvh_bench(bounces=32, p=0.8, spp=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=64,
+                    help="passes (samples per pixel) in the timed render; 64 is the configuration's spp")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
