@@ -153,26 +153,22 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=15.0):
     h = scene.height
     w = scene.width
     pr = la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=hero, seed=7)
-    # calibrate on a thin stripe, then size the sample (rows, then spp) to
-    # about budget_s seconds of CPU work
-    step = 64
+    # calibrate on one full frame at 1 spp (after a stripe that starts the
+    # threads), then size the sample's spp to about budget_s seconds
+    orc.render(pr, threads=threads, row_step=64, row_phase=32)
     t0 = time.perf_counter()
-    orc.render(pr, threads=threads, row_step=step, row_phase=0)
+    orc.render(pr, threads=threads)
     dt = time.perf_counter() - t0
-    rows = len(range(0, h, step))
-    rate = rows * w / max(dt, 1e-6)  # samples/s
-    want = rate * budget_s
-    step = int(max(1, min(h, h * w / max(want, 1.0))))
-    spp_cpu = int(max(1, min(spp, want / (h * w)))) if step == 1 else 1
-    pr = la.params(spp=spp_cpu, max_scatterings=bounces, sample_p=p, hero=hero, seed=7)
-    t0 = time.perf_counter()
-    orc.render(pr, threads=threads, row_step=step, row_phase=1 % step)
-    dt = time.perf_counter() - t0
-    rows = len(range(1 % step, h, step))
-    samples = rows * w * spp_cpu
+    spp_cpu = 1
+    if dt < budget_s / 2:
+        spp_cpu = int(max(1, min(spp, round(budget_s / max(dt, 1e-6)))))
+        pr = la.params(spp=spp_cpu, max_scatterings=bounces, sample_p=p, hero=hero, seed=8)
+        t0 = time.perf_counter()
+        orc.render(pr, threads=threads)
+        dt = time.perf_counter() - t0
+    samples = h * w * spp_cpu
     return {"value": round(samples / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"every {step}th row ({rows} rows x {w} px), {spp_cpu} spp, {bounces} bounces, "
-                      f"p={p}, hero {hero}; {dt:.1f} s"}
+            "sample": f"full frame {w}x{h}, {spp_cpu} spp, {bounces} bounces, p={p}, hero {hero}; {dt:.1f} s"}
 
 
 def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
