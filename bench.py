@@ -146,7 +146,7 @@ def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     }
 
 
-def cpu_baseline(scene, spp, bounces, p, hero, budget_s=15.0):
+def cpu_baseline(scene, spp, bounces, p, hero, budget_s=20.0):
     from oracle.binding import Oracle
     orc = Oracle(scene)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
