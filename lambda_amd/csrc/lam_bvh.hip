@@ -174,27 +174,71 @@ __device__ __forceinline__ void ref_box(uint32_t ref, const float4 *__restrict__
   }
 }
 
-// boxes bottom-up: the second child to arrive at a node writes its box
-// (device-scope fences around the arrival counter make the sibling's box
-// visible across the XCDs' L2 caches)
-__global__ __launch_bounds__(256) void k_bottom_up(const uint2 *__restrict__ child, const uint32_t *__restrict__ parent_in,
-                                                   const uint32_t *__restrict__ parent_leaf,
-                                                   const uint32_t *__restrict__ perm, const float4 *__restrict__ pbox,
-                                                   uint32_t n, float4 *ibox, uint32_t *arrive) {
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+// Boxes bottom-up: the second child to arrive at a node writes its box.
+// The climb is split so that nearly all of it needs only workgroup-scope
+// ordering (a device-scope fence writes back and invalidates the L2 of the
+// wave's XCD; one per climb step cost milliseconds).  Phase 1: block b owns
+// the kBuBlock consecutive sorted leaves [kBuBlock b, kBuBlock (b+1)) and
+// climbs only through nodes whose key range lies inside them, with block-
+// scope atomics and fences; a climber whose parent's range crosses a block
+// boundary appends that parent to `top` and stops.  Phase 2 (one block,
+// after the kernel boundary has made phase 1's boxes visible) climbs from
+// the appended nodes through the crossing nodes, which form the top of the
+// tree.  Each crossing node gets exactly two arrivals: one per child, from
+// the phase-1 completer of a non-crossing child (or the leaf's own thread)
+// and from the phase-2 completer of a crossing child.
+constexpr uint32_t kBuBlock = 1024;
+
+__device__ __forceinline__ void node_box(uint32_t node, const uint2 *__restrict__ child,
+                                         const uint32_t *__restrict__ perm, const float4 *__restrict__ pbox,
+                                         float4 *ibox) {
+  const uint2 c = child[node];
+  float4 la, ha, lb, hb;
+  ref_box(c.x, pbox, perm, ibox, la, ha);
+  ref_box(c.y, pbox, perm, ibox, lb, hb);
+  ibox[2 * size_t(node)] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.f);
+  ibox[2 * size_t(node) + 1] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.f);
+}
+
+__global__ __launch_bounds__(kBuBlock) void k_bottom_up_local(
+    const uint2 *__restrict__ child, const uint2 *__restrict__ range, const uint32_t *__restrict__ parent_in,
+    const uint32_t *__restrict__ parent_leaf, const uint32_t *__restrict__ perm, const float4 *__restrict__ pbox,
+    uint32_t n, float4 *ibox, uint32_t *arrive, uint32_t *top, uint32_t *top_count, uint32_t top_cap) {
+  const uint32_t t = blockIdx.x * kBuBlock + threadIdx.x;
   if (t >= n) return;
   uint32_t node = parent_leaf[t];
   while (node != kNone) {
-    __threadfence();
-    if (atomicAdd(arrive + node, 1u) == 0u) return;
-    __threadfence();
-    const uint2 c = child[node];
-    float4 la, ha, lb, hb;
-    ref_box(c.x, pbox, perm, ibox, la, ha);
-    ref_box(c.y, pbox, perm, ibox, lb, hb);
-    ibox[2 * size_t(node)] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.f);
-    ibox[2 * size_t(node) + 1] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.f);
+    const uint2 r = range[node];
+    if (r.x / kBuBlock != r.y / kBuBlock) {
+      const uint32_t k = atomicAdd(top_count, 1u);
+      if (k < top_cap) top[k] = node;  // at most two per crossing node: k < 2(n-1)
+      return;
+    }
+    __threadfence_block();
+    if (__hip_atomic_fetch_add(arrive + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+    __threadfence_block();
+    node_box(node, child, perm, pbox, ibox);
     node = parent_in[node];
+  }
+}
+
+__global__ __launch_bounds__(kBuBlock) void k_bottom_up_top(const uint2 *__restrict__ child,
+                                                            const uint32_t *__restrict__ parent_in,
+                                                            const uint32_t *__restrict__ perm,
+                                                            const float4 *__restrict__ pbox, float4 *ibox,
+                                                            uint32_t *arrive, const uint32_t *__restrict__ top,
+                                                            const uint32_t *__restrict__ top_count,
+                                                            uint32_t top_cap) {
+  const uint32_t cnt = min(*top_count, top_cap);
+  for (uint32_t e = threadIdx.x; e < cnt; e += kBuBlock) {
+    uint32_t node = top[e];
+    while (node != kNone) {
+      __threadfence_block();
+      if (__hip_atomic_fetch_add(arrive + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
+      __threadfence_block();
+      node_box(node, child, perm, pbox, ibox);
+      node = parent_in[node];
+    }
   }
 }
 
@@ -407,6 +451,7 @@ size_t gpu_bvh_scratch_bytes(uint32_t n) {
          + m * 4 + size_t(n) * 4    // parents
          + m * 32                   // internal boxes
          + m * 4                    // arrival counters
+         + (2 * m + 1) * 4          // crossing nodes for the top of the bottom-up pass
          + size_t(n) * 8 + 8        // node_ref, node_acc (one per BVH8 node, at most n)
          + kRedBlocks * 24 + 64     // reduction partials, bounds, stats
          + sort_bytes + 32 * 256;   // sort scratch, alignment of each piece
@@ -433,6 +478,9 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
   uint32_t *parent_leaf = static_cast<uint32_t *>(take(size_t(n) * 4));
   float4 *ibox = static_cast<float4 *>(take(m * 32));
   uint32_t *arrive = static_cast<uint32_t *>(take(m * 4));
+  // at most two entries per crossing node (one per child), then the count
+  uint32_t *top = static_cast<uint32_t *>(take((2 * m + 1) * 4));
+  uint32_t *top_count = top + 2 * m;
   uint32_t *node_ref = static_cast<uint32_t *>(take(size_t(n) * 4 + 4));
   uint32_t *node_acc = static_cast<uint32_t *>(take(size_t(n) * 4 + 4));
   uint32_t *partial = static_cast<uint32_t *>(take(kRedBlocks * 24));
@@ -453,13 +501,16 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
   if (e != hipSuccess) return e;
   if (n >= 2) {
     if ((e = launch_zero(arrive, m * 4, stream)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(top_count, 0, 4, stream)) != hipSuccess) return e;
     // a parent the radix tree does not set ends the upward walk
     if ((e = hipMemsetAsync(parent_in, 0xFF, m * 4, stream)) != hipSuccess) return e;
     if ((e = hipMemsetAsync(parent_leaf, 0xFF, size_t(n) * 4, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_karras, dim3(uint32_t((m + 255) / 256)), dim3(256), 0, stream, skeys, int(n), child, range,
                        parent_in, parent_leaf);
-    hipLaunchKernelGGL(k_bottom_up, dim3(b256), dim3(256), 0, stream, child, parent_in, parent_leaf, perm, pbox, n,
-                       ibox, arrive);
+    hipLaunchKernelGGL(k_bottom_up_local, dim3((n + kBuBlock - 1) / kBuBlock), dim3(kBuBlock), 0, stream, child,
+                       range, parent_in, parent_leaf, perm, pbox, n, ibox, arrive, top, top_count, uint32_t(2 * m));
+    hipLaunchKernelGGL(k_bottom_up_top, dim3(1), dim3(kBuBlock), 0, stream, child, parent_in, perm, pbox, ibox, arrive,
+                       top, top_count, uint32_t(2 * m));
   }
   hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, stream, n, node_ref, node_acc, stats);
   if ((e = hipGetLastError()) != hipSuccess) return e;
