@@ -1,6 +1,6 @@
 """Edge cases of the render path: scenes with no triangles and with no
-objects, meshes of 1, 2 and 9 triangles (the host and the device BVH
-builds), 1x1 and 3x5 images, ray batches of 0, 1, 63, 65 and 6,145 rays,
+objects, meshes of 1, 2, 9, 1,023, 1,024, 1,025 and 3,000 triangles (the
+host and the device BVH builds), 1x1 and 3x5 images, ray batches of 0, 1, 63, 65 and 6,145 rays,
 and a render of zero samples.
 
 CPU: the loader's image sizes for the tiny cameras.
@@ -57,7 +57,9 @@ def _make(d, kind):
     return la.Scene.from_text(json.dumps(j))
 
 
-KINDS = ["analytic", "no_objects", "tri1", "tri2", "tri9"]
+# 1,023 .. 3,000 triangles straddle the 1,024-leaf blocks of the device
+# builder's bottom-up pass
+KINDS = ["analytic", "no_objects", "tri1", "tri2", "tri9", "tri1023", "tri1024", "tri1025", "tri3000"]
 
 
 @pytest.fixture(scope="module")
@@ -70,7 +72,7 @@ def test_tiny_camera_sizes(edge_scenes):
     assert (edge_scenes["1x1"].width, edge_scenes["1x1"].height) == (1, 1)
     assert (edge_scenes["3x5"].width, edge_scenes["3x5"].height) == (3, 5)
     assert edge_scenes["analytic"].n_tris == 0 and edge_scenes["no_objects"].n_tris == 0
-    assert [edge_scenes[k].n_tris for k in ("tri1", "tri2", "tri9")] == [1, 2, 9]
+    assert [edge_scenes[k].n_tris for k in ("tri1", "tri2", "tri9", "tri1025")] == [1, 2, 9, 1025]
 
 
 @pytest.fixture(scope="module")
