@@ -381,8 +381,8 @@ def main():
                          "(the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
-                                   "(the sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this renderer runs on one GPU; --gpus > 1 is not supported "
+                                   "(samples would shard by pixel tiles with one film all-reduce)"}))
         sys.exit(2)
     legs = set(args.legs.split(","))
     if args.no_cpu:

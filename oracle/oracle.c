@@ -7,7 +7,7 @@
  *   liboracle32.so  FP32, -ffp-contract=off: bit-level parity target of the
  *                   GPU extend/shadow kernels and per-path radiance;
  *   liboracle64.so  FP64 "truth mode": same estimator in double with libm,
- *                   the judge of image-level error.
+ *                   the reference for image-level error.
  *
  * What it restates (reference file:line):
  *   render loop + film              src/renderer.cpp:14-61, cg.hpp:306-341

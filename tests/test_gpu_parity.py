@@ -1,6 +1,6 @@
 """GPU parity: the HIP kernels against the CPU oracle on identical inputs.
 
-Bars (BASELINE north star):
+Bars:
   (a) closest-hit t / triangle / object / prim bit-exact, ties to lowest id;
   (b) per-path spectral radiance within 1e-4 relative;
   (c) images: same-sample FP32 renders agree to film reassociation; against
