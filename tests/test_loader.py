@@ -103,6 +103,27 @@ def test_flattened_triangles_are_world_space(mixed_scene):
     assert np.array_equal(s.tri_prim(), np.arange(s.n_tris, dtype=np.uint32))
 
 
+def test_meshes_shared_by_path(tmp_path):
+    """objects naming the same OBJ path share one mesh (loaded once,
+    cg.hpp:2295-2303); each instance is flattened with its own transform"""
+    V, F, N = scenes.geodesic_sphere(1, seed=1)
+    scenes.write_obj(str(tmp_path / "a.obj"), V, F, N)
+    scenes.write_obj(str(tmp_path / "b.obj"), V[:3] * 2.0, np.array([[0, 1, 2]]), N[:3])
+    objs = ",".join([
+        f'{{"name":"a{i}","type":"trimesh","source":"{tmp_path / "a.obj"}","material":"m",'
+        f'"transform":{{"translate":[{3 * i},0,0]}}}}' for i in range(3)] +
+        [f'{{"name":"b","type":"trimesh","source":"{tmp_path / "b.obj"}","material":"m","transform":{{}}}}'])
+    s = la.Scene.from_text(_minimal(objects=f"[{objs}]"))
+    meshes = [o.mesh for o in s.objects()]
+    assert meshes[0] == meshes[1] == meshes[2] != meshes[3]
+    nf = len(F)
+    assert s.n_tris == 3 * nf + 1
+    tv = s.tri_v().reshape(-1, 3, 3)
+    obj = s.tri_obj()
+    for i in (1, 2):  # instance i is instance 0 moved by (3i, 0, 0), vertex for vertex
+        assert np.allclose(tv[obj == i] - tv[obj == 0], [3.0 * i, 0.0, 0.0], atol=1e-5)
+
+
 def _minimal(lights='[{"name":"l","type":"point","pos":[0,0,5],"radiant_intensity":{"val":1}}]',
              objects='[{"name":"p","type":"plane","transform":{},"material":"m"}]',
              materials='[{"name":"m","type":"lambertian","albedo":{"val":0.5}}]'):
