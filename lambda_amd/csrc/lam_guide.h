@@ -332,6 +332,24 @@ __device__ __forceinline__ bool guide_supported(const lam_material &m) {
   return m.type == LAM_MAT_LAMBERTIAN || (m.type == LAM_MAT_BLINN && m.p_d > 0.f);
 }
 
+// Does the guided shade draw this vertex's continuation from a diffuse lobe
+// the proposal applies to, i.e. read the network's mixture?  The
+// reference's lobe choice from the LOBE dimension xl (cg.hpp:1768, 1853)
+// with roulette probability P; pp_d is the chosen lobe's selection
+// probability.  The network is evaluated for exactly these vertices.
+__device__ __forceinline__ bool guide_diffuse_draw(const GuideDev &gd, const lam_material &m, float xl, float P,
+                                                   float &pp_d) {
+  pp_d = 0.f;
+  if (!(gd.alpha > 0.f) || !((gd.mat_mask >> m.type) & 1u) || !guide_supported(m)) return false;
+  if (m.type == LAM_MAT_LAMBERTIAN) {
+    pp_d = P;
+    return !(xl > P);
+  }
+  const float pp_spec = P * m.p_spec;
+  pp_d = P * m.p_d;
+  return xl < P && !(xl < pp_spec) && xl < pp_spec + pp_d;
+}
+
 // power-heuristic weight a² / (a² + b²) of a strategy with pdf a against one
 // with pdf b; 1 where b = 0 (a pdf that underflowed to 0 leaves 0/0 otherwise)
 __device__ __forceinline__ float mis_w(float a, float b2) {

@@ -342,13 +342,33 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
 // TRAIN: each guided continuation appends a training record (network inputs,
 // ω, Σβ, ΣL before the vertex, ΣNEE term, slot; k_shadow marks a visible
 // NEE term).
-// The network for every entry of the extend queue (after k_extend): inputs
-// from the hit's surface (position, shading normal) and the path's hero
-// wavelength; 40 outputs stored as fp16.  One wave evaluates 64 entries with MFMA, the
-// packed parameters staged in LDS once per block.
+// k-th set bit (from 0) of m, for k < popcount(m): binary search over halves
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
+  uint32_t base = 0;
+#pragma unroll
+  for (uint32_t w = 32; w >= 1; w >>= 1) {
+    const uint64_t lo = m & ((uint64_t(1) << w) - 1);
+    const uint32_t c = __popcll(lo);
+    const bool hi = k >= c;
+    k = hi ? k - c : k;
+    m = hi ? m >> w : lo;
+    base += hi ? w : 0u;
+  }
+  return base;
+}
+
+// The network for the entries of the extend queue (after k_extend) whose
+// continuation k_shade_guided will draw from the mixture (guide_diffuse_draw
+// of bounce s): inputs from the hit's surface (position, shading normal) and
+// the path's hero wavelength; 40 outputs stored as fp16 at the entry.  A
+// wave scans 64 entries at a time and packs the ones that need the network
+// into full waves of 64 queries (ballot, k-th set bit, lane shuffles; one
+// wave of overflow), so that each MFMA evaluation serves 64 queries; the
+// packed parameters are staged in LDS once per block.
 template <int H>
 __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
-                                                     GuideDev gd, uint32_t qsel, const uint32_t *__restrict__ ctrl) {
+                                                     GuideDev gd, uint32_t s, uint32_t qsel,
+                                                     const uint32_t *__restrict__ ctrl) {
   __shared__ uint4 sfrag[kGFrags * 64];
   __shared__ float sbias[kGBiasWords];
   if (ctrl[CTRL_TAIL]) return;
@@ -361,36 +381,21 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t wbase = first - (threadIdx.x % 64); wbase < n; wbase += stride) {
-    const uint32_t q = wbase + (threadIdx.x % 64);
-    float gin[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    bool valid = false;
-    if (q < n) {
-      const Hit hit = ws.hits[q];
-      if (hit.obj != LAM_NONE) {
-        const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
-        const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
-        const Surf sf = surface(sc, hit, ro, rd);
-        uint32_t pixel, sample;
-        path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
-        Lam<H> lam;
-        lam_for<H>(sc, lam_path_key(pa.seed, pixel, sample), lam);
-        guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
-        valid = true;
-      }
-    }
-    const uint64_t vmask = __ballot(valid);
-    if (!vmask) continue;
+  const uint32_t lane = threadIdx.x % 64, ln = lane % 16, lg = lane / 16;
+  uint2 *mix = reinterpret_cast<uint2 *>(gd.mix);
+  // queries waiting for the network: lanes [0, P) of slot A; slot B holds
+  // what a scan adds beyond 64
+  float ain[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, bin[kGIn];
+  uint32_t aq = 0, bq = 0, P = 0;
+  auto evaluate = [&](uint32_t nvalid) {
     f32x4 o[4][3];
-    mlp_wave(sfrag, sbias, gin, o);
+    mlp_wave(sfrag, sbias, ain, o);
     // lane (n, g) holds rows 16t + 4g .. +3 of the query in lane 16G + n
-    const uint32_t lane = threadIdx.x % 64, ln = lane % 16, lg = lane / 16;
-    uint2 *mix = reinterpret_cast<uint2 *>(gd.mix);
 #pragma unroll
     for (int G = 0; G < 4; G++) {
-      const uint32_t src = 16 * G + ln;
-      if (!((vmask >> src) & 1u)) continue;
+      const uint32_t src = 16u * G + ln;
+      const uint32_t q = __shfl(aq, int(src), 64);
+      if (src >= nvalid) continue;
 #pragma unroll
       for (int t = 0; t < 3; t++) {
         const uint32_t row = 16 * t + 4 * lg;
@@ -400,11 +405,64 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
                               (uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[1]))) << 16);
           const uint32_t w1 = uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[2]))) |
                               (uint32_t(__builtin_bit_cast(uint16_t, _Float16(v[3]))) << 16);
-          mix[size_t(wbase + src) * 10 + row / 4] = make_uint2(w0, w1);
+          mix[size_t(q) * 10 + row / 4] = make_uint2(w0, w1);
         }
       }
     }
+  };
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t wbase = first - lane; wbase < n; wbase += stride) {
+    const uint32_t q = wbase + lane;
+    float gin[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool need = false;
+    if (q < n && s <= pa.max_scat) {
+      const Hit hit = ws.hits[q];
+      if (hit.obj != LAM_NONE) {
+        const lam_material &mat = sc.materials[sc.objects[hit.obj].material];
+        if (mat.type != LAM_MAT_EMITTER && mat.type != LAM_MAT_DIREMITTER) {
+          const float4 b = rin[2 * size_t(q) + 1];
+          uint32_t pixel, sample;
+          path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
+          const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+          float pp_d;
+          if (guide_diffuse_draw(gd, mat, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)), pa.sample_p, pp_d)) {
+            const float4 a = rin[2 * size_t(q)];
+            const Surf sf = surface(sc, hit, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+            Lam<H> lam;
+            lam_for<H>(sc, key, lam);
+            guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
+            need = true;
+          }
+        }
+      }
+    }
+    const uint64_t m = __ballot(need);
+    const uint32_t c = __popcll(m);
+    if (c == 0) continue;
+    // lane d of slot A takes new query d - P, lane d of slot B new query d + 64 - P
+    const bool take_a = lane >= P && lane - P < c;
+    const bool take_b = lane + 64u - P < c;
+    const int sa = int(take_a ? nth_set_bit(m, lane - P) : lane);
+    const int sb = int(take_b ? nth_set_bit(m, lane + 64u - P) : lane);
+#pragma unroll
+    for (int i = 0; i < kGIn; i++) {
+      const float va = __shfl(gin[i], sa, 64), vb = __shfl(gin[i], sb, 64);
+      if (take_a) ain[i] = va;
+      bin[i] = vb;
+    }
+    const uint32_t qa = __shfl(q, sa, 64), qb = __shfl(q, sb, 64);
+    if (take_a) aq = qa;
+    bq = qb;
+    P += c;
+    if (P >= 64) {
+      evaluate(64);
+#pragma unroll
+      for (int i = 0; i < kGIn; i++) ain[i] = bin[i];
+      aq = bq;
+      P -= 64;
+    }
   }
+  if (P > 0) evaluate(P);
 }
 
 __device__ __forceinline__ void load_mix(const uint4 *src, float *raw) {
@@ -495,18 +553,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       const float P = pa.sample_p;
       const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
       // does the reference's sampler take the diffuse lobe (cg.hpp:1768, 1853)?
-      bool diffuse = false;
       float pp_d = 0.f;
-      if (alpha > 0.f) {
-        if (mat.type == LAM_MAT_LAMBERTIAN) {
-          diffuse = !(xl > P);
-          pp_d = P;
-        } else {
-          const float pp_spec = P * mat.p_spec;
-          pp_d = P * mat.p_d;
-          diffuse = xl < P && !(xl < pp_spec) && xl < pp_spec + pp_d;
-        }
-      }
+      const bool diffuse = guide_diffuse_draw(gd, mat, xl, P, pp_d);
       bool ok = false;
       f3 wi = mk(0.f, 0.f, 0.f);
       float fac[H];
@@ -823,7 +871,8 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
                          reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
     }
     if (gd)
-      hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, q, ctrl);
+      hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, s, q,
+                         ctrl);
     if (!gd && spec_bytes <= kSpecLdsBytes)
       hipLaunchKernelGGL((k_shade<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
                          ws, s, q, ctrl);
