@@ -197,10 +197,24 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
     adj = time.perf_counter() - t0
     adj_stream = ctx.timing().render_ms
     ctx.close()
+    # albedo gradients on each material's CSV sample grid (transpose of the
+    # loader's box filter), beside the 64 thickness partials
+    t0 = time.perf_counter()
+    gn = scene.desc.grid.n
+    csv_grads = []
+    for m in range(16):
+        name = scene.material_name(int(m))
+        csv = os.path.join(d, f"albedo_{name[len('mat_'):]}.csv")
+        lam_csv = np.loadtxt(csv, delimiter=",", dtype=np.float32)[:, 0]
+        o = dp.mat_albedo[m]
+        csv_grads.append(la.spectrum_grad_to_samples(scene.desc.grid, lam_csv, grad[o:o + gn]))
+    map_s = time.perf_counter() - t0
     paths = scene.width * scene.height * spp
     return {"workload": "box room, 16 Lambertian albedos + 64 thin-film spheres (11,564 tris), 512x512, "
                         "L2 loss to a +-15% perturbed target, 16 bounces, p=0.8, hero 4",
-            "n_params": int(dp.n_params), "spp": spp, "forward_s": round(fwd, 4), "adjoint_s": round(adj, 4),
+            "n_params": int(dp.n_params),
+            "n_params_csv_grid": int((dp.mat_d >= 0).sum() + sum(g.size for g in csv_grads)),
+            "csv_map_s": round(map_s, 4), "spp": spp, "forward_s": round(fwd, 4), "adjoint_s": round(adj, 4),
             "adjoint_stream_ms": round(adj_stream, 2), "adjoint_paths_per_s": round(paths / adj, 1),
             "grads_per_s": round(1.0 / adj, 3), "loss": float(((rgb - target) ** 2).sum()),
             "grad_finite": bool(np.isfinite(grad).all())}

@@ -30,6 +30,17 @@ uint32_t lamh_scene_hero(const lamh_scene *s);
 const char *lamh_object_name(const lamh_scene *s, uint32_t i);
 const char *lamh_material_name(const lamh_scene *s, uint32_t i);
 
+/* Transpose of the loader's box filter coefficientλ(sampledλ)
+ * (lib/numli/cg.hpp:252-272), which is linear in the sample values: for a
+ * spectrum whose grid coefficients are mult * box_filter(samples), maps a
+ * gradient with respect to the grid coefficients (dcoef, grid->n values,
+ * e.g. a lam_adjoint albedo or film-IOR slot range) to the gradient with
+ * respect to the CSV's value column (dsamples, n_samples values) at the
+ * strictly increasing wavelengths sample_l.  LAM_E_INVALID on NULL
+ * pointers, n_samples == 0 or unsorted wavelengths. */
+int lamh_spectrum_grad_to_samples(const lam_grid *grid, const float *sample_l, size_t n_samples, float mult,
+                                  const float *dcoef, float *dsamples);
+
 /* Renderer::toneMap (src/renderer.cpp:149-176): log-average exposure
  * Y_MID/Ȳ then the koiFilmic curve per channel.  rgb_in/out: w*h*3. */
 int lamh_tone_map(const float *rgb_in, uint32_t w, uint32_t h, float y_mid, float *rgb_out);

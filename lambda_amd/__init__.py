@@ -220,6 +220,7 @@ def lib():
         "lamh_image_name": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_float, C.c_char_p,
                                       C.c_char_p, sz]),
         "lamh_bvh8_check": (C.c_int, [fp, C.c_uint32, C.POINTER(SceneStats), C.POINTER(C.c_int)]),
+        "lamh_spectrum_grad_to_samples": (C.c_int, [C.POINTER(Grid), fp, sz, C.c_float, fp, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -729,6 +730,26 @@ def tone_map(rgb, y_mid=0.12):
     if rc != LAM_OK:
         raise LambdaError(rc, "tone map failed")
     return out
+
+
+def spectrum_grad_to_samples(grid, sample_l, dcoef, mult=1.0):
+    """Gradient with respect to a spectrum CSV's value column, given the
+    gradient with respect to its grid coefficients (the transpose of the
+    loader's box filter, lib/numli/cg.hpp:252-272; `mult` is the spectrum's
+    "mult").  `grid` is a Grid (Scene.desc.grid); `dcoef` a slot range of
+    Context.adjoint's gradient, or rows of them (..., grid.n)."""
+    sample_l = np.ascontiguousarray(sample_l, dtype=np.float32)
+    dcoef = np.ascontiguousarray(dcoef, dtype=np.float32)
+    if dcoef.shape[-1] != grid.n:
+        raise ValueError(f"dcoef has {dcoef.shape[-1]} values per row, the grid {grid.n}")
+    rows = dcoef.reshape(-1, grid.n)
+    out = np.zeros((rows.shape[0], len(sample_l)), dtype=np.float32)
+    for r in range(rows.shape[0]):
+        rc = lib().lamh_spectrum_grad_to_samples(C.byref(grid), _fptr(sample_l), len(sample_l), float(mult),
+                                                 _fptr(rows[r]), _fptr(out[r]))
+        if rc != LAM_OK:
+            raise LambdaError(rc, "spectrum gradient mapping failed")
+    return out.reshape(dcoef.shape[:-1] + (len(sample_l),))
 
 
 def write_png8(path, rgb8):

@@ -109,6 +109,21 @@ const char *lamh_material_name(const lamh_scene *s, uint32_t i) {
   return s->scene.materials[i].name.c_str();
 }
 
+int lamh_spectrum_grad_to_samples(const lam_grid *grid, const float *sample_l, size_t n_samples, float mult,
+                                   const float *dcoef, float *dsamples) {
+  if (!grid || !sample_l || !dcoef || !dsamples || n_samples == 0 || grid->n < 2) return LAM_E_INVALID;
+  for (size_t k = 1; k < n_samples; k++)
+    if (!(sample_l[k] > sample_l[k - 1])) return LAM_E_INVALID;
+  lamh::Grid g;
+  g.n = grid->n;
+  g.inf = grid->inf;
+  g.sup = grid->sup;
+  std::vector<float> l(sample_l, sample_l + n_samples);
+  lamh::box_filter_transpose(g, l, dcoef, dsamples);
+  for (size_t k = 0; k < n_samples; k++) dsamples[k] *= mult;
+  return LAM_OK;
+}
+
 int lamh_tone_map(const float *in, uint32_t w, uint32_t h, float y_mid, float *out) {
   if (!in || !out) return LAM_E_INVALID;
   const size_t N = size_t(w) * h;

@@ -128,6 +128,31 @@ Spectrum box_filter(const Grid &g, const Sampled &s) {
   return out;
 }
 
+// Transpose of box_filter with respect to the sample values: the same
+// segment walk, each trapezoid .5*(v[k]+v[k+1])*seg/D sending seg/(2D) of
+// the coefficient's gradient to both of its end samples.
+void box_filter_transpose(const Grid &g, const std::vector<float> &l, const float *dcoef, float *dv) {
+  const float D = g.delta();
+  const size_t L = l.size();
+  for (size_t k = 0; k < L; k++) dv[k] = 0.f;
+  size_t j = 0;
+  for (uint32_t i = 0; i < g.n; i++) {
+    const float l0 = g.inf + float(i) * D;
+    const float l1 = l0 + D;
+    while (j + 1 < L && l[j + 1] <= l0) j++;
+    size_t k = j;
+    while (k + 1 < L && l[k] < l1) {
+      float seg = fmin_ref(l1, l[k + 1]) - fmax_ref(l0, l[k]);
+      if (seg > 0) {
+        float w = .5f * seg / D * dcoef[i];
+        dv[k] += w;
+        dv[k + 1] += w;
+      }
+      k++;
+    }
+  }
+}
+
 // operator| (cg.hpp:283-288)
 float spectrum_dot(const Grid &g, const Spectrum &a, const Spectrum &b) {
   float sum = 0.f;

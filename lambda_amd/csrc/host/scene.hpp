@@ -98,6 +98,8 @@ bool read_sampled_csv(const std::string &path, Sampled &out, std::string &err);
 bool read_sampled_csv_columns(const std::string &path, float minl, float stepl,
                               int column, Sampled &out, std::string &err);
 Spectrum box_filter(const Grid &g, const Sampled &s);          // cg.hpp:252-272
+// dv[k] = Σ_i dcoef[i] ∂box_filter(g, {l, v})[i]/∂v[k] (box_filter is linear in v)
+void box_filter_transpose(const Grid &g, const std::vector<float> &l, const float *dcoef, float *dv);
 float spectrum_dot(const Grid &g, const Spectrum &a, const Spectrum &b);  // cg.hpp:283-288
 float spectrum_eval(const Grid &g, const Spectrum &s, float l);  // cg.hpp:289-297
 std::string spectrum_to_string(const Spectrum &s);               // cg.hpp:321-326

@@ -203,6 +203,35 @@ def test_spectrum_box_filter_exact_on_linear(tmp_path):
     assert np.all(s2.spectra()[s2.materials()[0].spec[0]] == np.float32(0.5))
 
 
+@pytest.mark.parametrize("grid", ['', ',"spectral_grid":{"n":95,"inf":360,"sup":830}'])
+def test_spectrum_grad_to_samples_is_the_filter_transpose(tmp_path, grid):
+    """The loader's box filter is linear in the CSV values, so its Jacobian
+    column k is the filtered unit vector e_k.  The transposed map applied to
+    a random coefficient gradient must equal Jᵀ·g built from 81 loads, on the
+    95-bin 5 nm grid (bins outside the samples stay 0) and the reference's
+    unaligned 80-bin grid; mult scales it."""
+    lam = np.arange(380.0, 781.0, 5.0)
+    cols = []
+    for k in range(len(lam)):
+        p = tmp_path / f"e{k}.csv"
+        e = np.zeros(len(lam))
+        e[k] = 1.0
+        scenes.write_csv(str(p), lam, e)
+        text = _minimal(materials='[{"name":"m","type":"lambertian","albedo":{"val":"%s"}}]' % p)
+        if grid:
+            text = text[:-1] + grid + "}"
+        s = la.Scene.from_text(text)
+        cols.append(s.spectra()[s.materials()[0].spec[0]].astype(np.float64))
+    J = np.stack(cols, axis=1)  # (grid.n, 81)
+    g = np.random.default_rng(3).normal(size=(2, s.desc.grid.n)).astype(np.float32)
+    got = la.spectrum_grad_to_samples(s.desc.grid, lam, g)
+    np.testing.assert_allclose(got, g.astype(np.float64) @ J, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+    got2 = la.spectrum_grad_to_samples(s.desc.grid, lam, g[0], mult=2.0)
+    np.testing.assert_array_equal(got2, 2 * got[0])
+    with pytest.raises(la.LambdaError):
+        la.spectrum_grad_to_samples(s.desc.grid, lam[::-1], g[0])
+
+
 def test_rgb_spectrum_is_bt709_basis_sum():
     s = la.Scene.from_text(_minimal(materials='[{"name":"m","type":"lambertian","albedo":{"val":[1,0,0]}}]'))
     s2 = la.Scene.from_text(_minimal(materials='[{"name":"m","type":"lambertian","albedo":{"val":[0,1,0]}}]'))
