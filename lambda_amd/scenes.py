@@ -451,7 +451,8 @@ def rounded_box(k, p=8.0):
     return V, F, N
 
 
-def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14, plate=1.0, plate_gap=0.35):
+def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14, plate=1.0, plate_gap=0.35,
+                    box_ks=0.35):
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     lam_ref = np.arange(355.0, 836.0, 5.0)
@@ -512,7 +513,7 @@ def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv
         kd = os.path.join(out_dir, f"kd_{placed}.csv")
         write_csv(kd, lam_ref, 0.6 * smooth_reflectance(rng, lam_ref))
         alpha = 2 * int(round(rng.uniform(50, 500) / 2))
-        materials.append({"name": f"box{placed}", "type": "blinn", "Kd": {"val": kd}, "Ks": {"val": 0.35},
+        materials.append({"name": f"box{placed}", "type": "blinn", "Kd": {"val": kd}, "Ks": {"val": box_ks},
                           "ior": {"val": 1.5}, "glossiness": alpha})
         objects.append({"name": f"box{placed}", "type": "trimesh", "material": f"box{placed}",
                         "source": os.path.join(out_dir, "rbox.obj"),

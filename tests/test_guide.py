@@ -272,3 +272,37 @@ def test_guided_render_unbiased(ctx, glossy):
         sd = np.sqrt(vp.sum() + vg.sum())
         assert abs(tot_g - tot_p) < 4 * sd, (trained, tot_p, tot_g, sd)
         assert np.median(np.abs(z)) < 1.2, (trained, np.median(np.abs(z)))
+
+
+@pytest.mark.gpu
+def test_trained_network_lowers_variance(ctx, tmp_path):
+    """What the learned proposal gains: on the hidden-light room (Lambertian
+    boxes), per-pixel variance over 8 independent renders with the network
+    trained on 64 passes is below the plain sampler's (measured ratio
+    0.83) and well below guiding with the untrained network (2.6x plain),
+    at equal samples per pixel."""
+    from lambda_amd import scenes
+    sc = la.load_nls(scenes.glossy_interior(str(tmp_path), seed=4, width=64, n_boxes=24, box_subdiv=3,
+                                            box_ks=0.0))
+    ctx.upload(sc)
+    w = np.array([0.2126, 0.7152, 0.0722])
+
+    def rel_var(runs):
+        y = np.stack([r.astype(np.float64) @ w for r in runs])
+        mu = y.mean(0)
+        return float(np.mean(y.var(0, ddof=1) / (mu ** 2 + 0.01 * mu.mean() ** 2)))
+
+    mk = lambda s: la.params(spp=64, max_scatterings=16, sample_p=0.8, hero=4, seed=s)
+    K = 8
+    v_plain = rel_var([ctx.render(mk(10 + k)) for k in range(K)])
+    frozen = la.guide_params(alpha=0.5, train_passes=0, seed=3)
+    ctx.guide_reset(3)
+    v_untrained = rel_var([ctx.render_guided(mk(50 + k), frozen) for k in range(K)])
+    ctx.guide_reset(3)
+    ctx.render_guided(la.params(spp=64, max_scatterings=16, sample_p=0.8, hero=4, seed=1),
+                      la.guide_params(alpha=0.5, train_passes=64, steps_per_pass=8, batch=4096, lr=5e-3, seed=3))
+    assert ctx.timing().guide_records > 100000
+    v_trained = rel_var([ctx.render_guided(mk(50 + k), frozen) for k in range(K)])
+    print(f"relative variance: plain {v_plain:.5f}, untrained {v_untrained:.5f}, trained {v_trained:.5f}")
+    assert v_trained < 0.93 * v_plain, (v_plain, v_trained)
+    assert v_trained < 0.5 * v_untrained, (v_untrained, v_trained)
