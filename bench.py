@@ -220,7 +220,8 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
             "grad_finite": bool(np.isfinite(grad).all())}
 
 
-def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32):
+def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32, max_vertex=0,
+                  steps_per_pass=8, batch=16384):
     """Learned proposal on the glossy interior: two independent plain runs
     and two guided runs of `spp` samples per pixel (the guided runs train
     the network online on their first `train_passes` passes, inside the timed
@@ -252,8 +253,8 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     guided, tg, tel = [], [], []
     for s in (1, 2):
         ctx.guide_reset(s)
-        gp = la.guide_params(alpha=alpha, train_passes=train_passes, steps_per_pass=8, batch=16384, lr=5e-3,
-                             seed=s)
+        gp = la.guide_params(alpha=alpha, train_passes=train_passes, steps_per_pass=steps_per_pass, batch=batch,
+                             lr=5e-3, seed=s, max_vertex=max_vertex)
         t0 = time.perf_counter()
         guided.append(ctx.render_guided(pr(100 + s), gp))
         tg.append(time.perf_counter() - t0)
@@ -264,7 +265,8 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     samples = scene.width * scene.height * spp
     return {"workload": f"glossy interior ({scene.n_tris:,} tris: room shell + 200 Blinn rounded boxes, "
                         f"hidden light), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
-            "spp_per_run": spp, "train_passes": train_passes, "alpha": alpha,
+            "spp_per_run": spp, "train_passes": train_passes, "alpha": alpha, "max_vertex": max_vertex,
+            "steps_per_pass": steps_per_pass, "batch": batch,
             "plain": {"run_s": round(t_p, 3), "samples_per_s": round(samples / t_p, 1), "relmse": round(rp, 5),
                       "time_to_relmse_1e-3_s": round(t_p * rp / 1e-3, 2)},
             "guided": {"run_s": round(t_g, 3), "samples_per_s": round(samples / t_g, 1), "relmse": round(rg, 5),
@@ -389,6 +391,9 @@ def main():
                          "device BVH build of the render scene")
     ap.add_argument("--guide-spp", type=int, default=1024)
     ap.add_argument("--guide-train-passes", type=int, default=32)
+    ap.add_argument("--guide-steps-per-pass", type=int, default=8)
+    ap.add_argument("--guide-max-vertex", type=int, default=4,
+                    help="guide path vertices of bounce index below this (0 = every vertex)")
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -414,7 +419,9 @@ def main():
         if "adjoint" in legs:
             out["adjoint"] = adjoint_bench()
         if "guiding" in legs:
-            out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes)
+            out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
+                                           max_vertex=args.guide_max_vertex,
+                                           steps_per_pass=args.guide_steps_per_pass)
         if "gpu_bvh" in legs:
             out["gpu_bvh"] = gpu_bvh_bench()
         print(json.dumps(out), flush=True)
@@ -470,7 +477,9 @@ def main():
         out["adjoint"] = adjoint_bench()
         log(f"adjoint: {out['adjoint']}")
     if "guiding" in legs:
-        out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes)
+        out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
+                                           max_vertex=args.guide_max_vertex,
+                                           steps_per_pass=args.guide_steps_per_pass)
         log(f"guiding: {out['guiding']}")
     if "stress" in legs:
         out["stress"] = stress_bench()

@@ -289,7 +289,7 @@ typedef struct {
   float lr;                /* Adam step size */
   uint64_t seed;           /* network initialisation (if none yet) and minibatches */
   uint32_t materials;      /* material types guided: bit (1 << LAM_MAT_*); 0 = all supported */
-  uint32_t pad;
+  uint32_t max_vertex;     /* guide path vertices of bounce index < max_vertex (0 = every vertex) */
 } lam_guide_params;
 
 /* lam_render with the learned proposal.  The network persists in the

@@ -337,10 +337,13 @@ __device__ __forceinline__ bool guide_supported(const lam_material &m) {
 // reference's lobe choice from the LOBE dimension xl (cg.hpp:1768, 1853)
 // with roulette probability P; pp_d is the chosen lobe's selection
 // probability.  The network is evaluated for exactly these vertices.
-__device__ __forceinline__ bool guide_diffuse_draw(const GuideDev &gd, const lam_material &m, float xl, float P,
-                                                   float &pp_d) {
+__device__ __forceinline__ bool guide_vertex(const GuideDev &gd, const lam_material &m, uint32_t s) {
+  return gd.alpha > 0.f && ((gd.mat_mask >> m.type) & 1u) && guide_supported(m) && s < gd.max_vertex;
+}
+__device__ __forceinline__ bool guide_diffuse_draw(const GuideDev &gd, const lam_material &m, uint32_t s, float xl,
+                                                   float P, float &pp_d) {
   pp_d = 0.f;
-  if (!(gd.alpha > 0.f) || !((gd.mat_mask >> m.type) & 1u) || !guide_supported(m)) return false;
+  if (!guide_vertex(gd, m, s)) return false;
   if (m.type == LAM_MAT_LAMBERTIAN) {
     pp_d = P;
     return !(xl > P);

@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
           path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
           const uint64_t key = lam_path_key(pa.seed, pixel, sample);
           float pp_d;
-          if (guide_diffuse_draw(gd, mat, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)), pa.sample_p, pp_d)) {
+          if (guide_diffuse_draw(gd, mat, s, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)), pa.sample_p, pp_d)) {
             const float4 a = rin[2 * size_t(q)];
             const Surf sf = surface(sc, hit, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
             Lam<H> lam;
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
             ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
         } else if (s <= pa.max_scat) {
           live = true;
-          guided = gd.alpha > 0.f && ((gd.mat_mask >> mat.type) & 1u) && guide_supported(mat);
+          guided = guide_vertex(gd, mat, s);
           if (TRAIN && guided) guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
         }
       }
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
       // does the reference's sampler take the diffuse lobe (cg.hpp:1768, 1853)?
       float pp_d = 0.f;
-      const bool diffuse = guide_diffuse_draw(gd, mat, xl, P, pp_d);
+      const bool diffuse = guide_diffuse_draw(gd, mat, s, xl, P, pp_d);
       bool ok = false;
       f3 wi = mk(0.f, 0.f, 0.f);
       float fac[H];
@@ -870,25 +870,28 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
                          reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
     }
-    if (gd)
-      hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gd, s, q,
+    // vertices past the guided ones are shaded by the plain kernels (the guided
+    // kernels with no guided vertex compute the same values)
+    const GuideDev *gv = (gd && s < gd->max_vertex) ? gd : nullptr;
+    if (gv)
+      hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gv, s, q,
                          ctrl);
-    if (!gd && spec_bytes <= kSpecLdsBytes)
+    if (!gv && spec_bytes <= kSpecLdsBytes)
       hipLaunchKernelGGL((k_shade<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
                          ws, s, q, ctrl);
-    else if (!gd)
+    else if (!gv)
       hipLaunchKernelGGL((k_shade<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s,
                          q, ctrl);
     else if (train)
-      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gd,
+      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gv,
                          s, q, ctrl);
     else
       hipLaunchKernelGGL((k_shade_guided<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws,
-                         *gd, s, q, ctrl);
+                         *gv, s, q, ctrl);
     if (prof) (void)hipEventRecord(prof->shd0[s], st);
-    if (gd && train)
+    if (gv && train)
       hipLaunchKernelGGL((k_shadow<H, false, true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
-                         gd->rec, gd->rec_of_slot, gd->rec_cap);
+                         gv->rec, gv->rec_of_slot, gv->rec_cap);
     else
       hipLaunchKernelGGL((k_shadow<H, false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
                          nullptr, nullptr, 0u);

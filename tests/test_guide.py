@@ -235,13 +235,19 @@ def test_guided_alpha0_is_plain(ctx, glossy, flags):
     # one-lane image
     p1 = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=4, seed=3, flags=flags | la.LAM_FLAG_NO_PIPE)
     assert np.array_equal(g5, ctx.render_guided(p1, la.guide_params(alpha=0.5, train_passes=0)))
+    # a vertex limit past every bounce changes nothing; guiding only the
+    # first vertex gives a third image (deeper vertices use the plain kernels)
+    assert np.array_equal(g5, ctx.render_guided(p, la.guide_params(alpha=0.5, train_passes=0, max_vertex=64)))
+    g1 = ctx.render_guided(p, la.guide_params(alpha=0.5, train_passes=0, max_vertex=1))
+    assert not np.array_equal(g1, g5) and not np.array_equal(g1, plain)
 
 
 @pytest.mark.gpu
 def test_guided_render_unbiased(ctx, glossy):
     """guided and plain renders are independent estimates of one image: the
     per-pixel difference of their means is noise (z-test), with a trained
-    network and with an untrained one"""
+    network and with an untrained one, and with only the first two vertices
+    guided"""
     ctx.upload(glossy)
     spp = 256
     p = la.params(spp=spp, max_scatterings=8, sample_p=0.8, hero=4, seed=5)
@@ -249,17 +255,18 @@ def test_guided_render_unbiased(ctx, glossy):
     for half in range(2):  # two halves for a per-pixel variance estimate
         plain.append(ctx.render(la.params(spp=spp // 2, max_scatterings=8, sample_p=0.8, hero=4, seed=5,
                                           spp_offset=half * (spp // 2))).astype(np.float64))
-    for trained in (False, True):
+    for trained, mv in ((False, 0), (True, 0), (True, 2)):
         ctx.guide_reset(2)
         gp = la.guide_params(alpha=0.5, train_passes=8 if trained else 0, steps_per_pass=8, batch=4096,
-                             lr=5e-3, seed=7)
+                             lr=5e-3, seed=7, max_vertex=mv)
         guided = []
         for half in range(2):
             # pure wavefront: every vertex is shaded by the guided kernel
             pg = la.params(spp=spp // 2, max_scatterings=8, sample_p=0.8, hero=4, seed=99 + half,
                            spp_offset=0, flags=la.LAM_FLAG_NO_TAIL)
             guided.append(ctx.render_guided(pg, gp if half == 0 else
-                                            la.guide_params(alpha=0.5, train_passes=0, seed=7)).astype(np.float64))
+                                            la.guide_params(alpha=0.5, train_passes=0, seed=7,
+                                                            max_vertex=mv)).astype(np.float64))
             if trained and half == 0:
                 assert ctx.timing().guide_records > 1000
         mp, mg = (plain[0] + plain[1]) / 2, (guided[0] + guided[1]) / 2
