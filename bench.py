@@ -318,7 +318,7 @@ def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
     same = bool(np.array_equal(ref16, gpu16))
     ctx.upload(scene)
     pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=1)
-    gp = la.guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1)
+    gp = la.guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1, max_vertex=4)
     ctx.guide_reset(1)
     t0 = time.perf_counter()
     img = ctx.render_guided(pr, gp)
