@@ -29,7 +29,8 @@ Also reported:
                 hidden light; 470,424 tris), 1024x1024, 32 bounces, p=0.8:
                 two independent plain renders and two independent guided
                 renders (each with a fresh network trained online on its
-                first 32 passes, training time included) of the same spp.
+                first 32 passes, training time included; the proposal is
+                used at the first 4 path vertices) of the same spp.
                 relMSE = mean over pixels of Var(Y) / (Ȳ² + 0.01 mean(Ȳ)²),
                 Y = luminance, Var estimated from the two runs (both
                 estimators are unbiased); time to relMSE 1e-3 =
