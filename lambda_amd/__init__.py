@@ -143,7 +143,7 @@ class SceneStats(C.Structure):
 class WaveConfig(C.Structure):
     """lam_wave_config"""
     _fields_ = [("lanes", C.c_uint32), ("tail_threshold", C.c_uint32), ("tail_div", C.c_uint32),
-                ("max_iters", C.c_uint32), ("trace_per_cu", C.c_uint32)]
+                ("max_iters", C.c_uint32), ("trace_per_cu", C.c_uint32), ("batch_per_cu", C.c_uint32)]
 
 
 class Timing(C.Structure):
@@ -711,7 +711,8 @@ class Context:
 
     def set_wave_config(self, **kw):
         """change fields of the launch configuration (lanes, tail_threshold,
-        tail_div, max_iters, trace_per_cu); images do not depend on them"""
+        tail_div, max_iters, trace_per_cu, batch_per_cu); images and hits do
+        not depend on them"""
         c = WaveConfig(**{**self.wave_config(), **kw})
         self._check(self._lib.lam_set_wave_config(self._h, C.byref(c)))
 

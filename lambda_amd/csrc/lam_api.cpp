@@ -60,7 +60,9 @@ struct lam_ctx {
   void *tone_buf = nullptr;  // tone map: 1024 per-block log sums (double) + the exposure
   lam::WaveLaunch wl{0, 0, 131072u, 12u, 0, 0};
   int cus = 0;
-  uint32_t trace_per_cu = 24;  // extend / shadow blocks per CU
+  uint32_t trace_per_cu = 16;  // extend / shadow blocks per CU in render passes
+  uint32_t batch_per_cu = 24;  // ... for ray batches outside a render
+  uint32_t batch_blocks = 0;
   uint32_t *trace_ctr = nullptr;  // fetch counters of the batch trace kernel (one per launch)
   uint32_t lanes = 3;     // passes in flight
   uint32_t tail_div = 16;  // the finish kernel's threshold is at most 1/tail_div of a chunk
@@ -265,11 +267,15 @@ int ensure_launch(lam_ctx *ctx) {
     ctx->cus = cus > 0 ? cus : 256;
     const uint32_t c = uint32_t(ctx->cus);
     // extend/shadow: 6 KB of LDS stack per 64-lane block (the rest of the
-    // stack spills to HBM), trace_per_cu blocks per CU (24: the VGPR limit);
-    // finish: 16 KB of LDS stack per block, 10 per CU;
+    // stack spills to HBM); trace_per_cu blocks per CU in render passes (16:
+    // the passes of the other lanes run beside them; a full 24 made the 1080p
+    // render 6% slower), batch_per_cu for a lone ray batch (24, the most the
+    // LDS stacks leave room for beside other work);
+    // finish: 6 KB of LDS stack per 64-lane block, 10 per CU;
     // shade: 256-lane blocks, grid-stride, one occupancy round (its launch
     // bounds give 3 waves per SIMD = 3 blocks per CU)
     ctx->wl.trace_blocks = c * ctx->trace_per_cu;
+    ctx->batch_blocks = c * ctx->batch_per_cu;
     ctx->wl.finish_blocks = c * 10u;
     ctx->wl.shade_blocks = c * 3u;
     // guiding network: 51 KB of LDS per 256-lane block, 3 blocks per CU
@@ -279,11 +285,13 @@ int ensure_launch(lam_ctx *ctx) {
 }
 
 // Traversal-stack spill columns of lane l for its largest grid: the trace
-// grid (extend, shadow, batch traces) or the finish grid (finish, and on
-// lane 0 the adjoint, megakernel and path-trace hooks, capped at it).
+// grid (extend, shadow), the batch grid (lane 0: lone ray batches) or the
+// finish grid (finish, and on lane 0 the adjoint, megakernel and path-trace
+// hooks, capped at it).
 int ensure_spill(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
-  const uint32_t glanes = std::max(ctx->wl.trace_blocks, ctx->wl.finish_blocks) * 64u;
+  const uint32_t glanes =
+      std::max(std::max(ctx->wl.trace_blocks, ctx->wl.finish_blocks), l == 0 ? ctx->batch_blocks : 0u) * 64u;
   if (ln.spill_lanes < glanes) {
     if (ln.spill) HIPC(hipFree(ln.spill));
     ln.spill = nullptr;
@@ -918,7 +926,7 @@ static int trace_rays_host(lam_ctx *ctx, const lam_ray *rays, size_t n, lam_hit 
   HIPC(lam::launch_trace_rays(ctx->sc, static_cast<lam_ray *>(ctx->scratch[0]), n,
                               occl ? nullptr : static_cast<lam_hit *>(ctx->scratch[1]),
                               occl ? static_cast<uint32_t *>(ctx->scratch[1]) : nullptr, nullptr, ctx->d_err,
-                              ctx->trace_ctr, ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
+                              ctx->trace_ctr, ctx->lane[0].spill, ctx->batch_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   if (occl)
     HIPC(hipMemcpyAsync(occl, ctx->scratch[1], n * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -953,7 +961,7 @@ int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hit
   HIPC(lam::launch_zero(ctx->trace_ctr, 16, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, d_counts, ctx->d_err, ctx->trace_ctr,
-                              ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
+                              ctx->lane[0].spill, ctx->batch_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.extend_ms += event_ms(ctx);
@@ -978,7 +986,7 @@ int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit 
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   for (uint32_t r = 0; r < reps; r++)
     HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, nullptr, ctx->d_err, ctx->trace_ctr + r,
-                                ctx->lane[0].spill, ctx->wl.trace_blocks, ctx->stream));
+                                ctx->lane[0].spill, ctx->batch_blocks, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   const double ms = event_ms(ctx);
@@ -1443,6 +1451,7 @@ int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out) {
   out->tail_div = ctx->tail_div;
   out->max_iters = ctx->wl.max_iters;
   out->trace_per_cu = ctx->trace_per_cu;
+  out->batch_per_cu = ctx->batch_per_cu;
   return LAM_OK;
 }
 
@@ -1455,14 +1464,19 @@ int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
   // gives the pure wavefront)
   if (cfg->max_iters < 1) return fail(ctx, LAM_E_INVALID, "max_iters must be >= 1");
   if (cfg->trace_per_cu < 1 || cfg->trace_per_cu > 32) return fail(ctx, LAM_E_INVALID, "trace_per_cu must be 1..32");
+  if (cfg->batch_per_cu < 1 || cfg->batch_per_cu > 32) return fail(ctx, LAM_E_INVALID, "batch_per_cu must be 1..32");
   ctx->lanes = cfg->lanes;
   ctx->wl.tail_threshold = cfg->tail_threshold;
   ctx->tail_div = cfg->tail_div;
   ctx->wl.max_iters = cfg->max_iters;
   ctx->trace_per_cu = cfg->trace_per_cu;
-  // before the first render the CU count is not known yet: ensure_wave
-  // sets the grid from trace_per_cu then
-  if (ctx->cus > 0) ctx->wl.trace_blocks = uint32_t(ctx->cus) * cfg->trace_per_cu;
+  ctx->batch_per_cu = cfg->batch_per_cu;
+  // before the first launch the CU count is not known yet: ensure_launch
+  // sets the grids from trace_per_cu and batch_per_cu then
+  if (ctx->cus > 0) {
+    ctx->wl.trace_blocks = uint32_t(ctx->cus) * cfg->trace_per_cu;
+    ctx->batch_blocks = uint32_t(ctx->cus) * cfg->batch_per_cu;
+  }
   return LAM_OK;
 }
 

@@ -278,20 +278,25 @@ def test_gpu_bvh_same_hits_and_images(ctx, request, scene_name):
 
 
 def test_wave_config_keeps_the_image(ctx, box_scene):
-    """lanes, tail threshold and bounce cap change the schedule, not the image"""
+    """lanes, tail threshold, bounce cap and trace grids change the schedule,
+    not the image or the hits of a ray batch"""
     ctx.upload(box_scene)
     pr = la.params(spp=7, max_scatterings=8, sample_p=0.8, hero=4, seed=11, spp_offset=2)
     base = ctx.render(pr)
+    rays = random_rays(np.random.default_rng(31), 50000, [-0.9, -0.9, 0.1], [0.9, 0.9, 1.9])
+    hits = ctx.extend(rays)
     default = ctx.wave_config()
-    assert default == {"lanes": 3, "tail_threshold": 131072, "tail_div": 16, "max_iters": 12, "trace_per_cu": 24}
+    assert default == {"lanes": 3, "tail_threshold": 131072, "tail_div": 16, "max_iters": 12, "trace_per_cu": 16,
+                       "batch_per_cu": 24}
     try:
-        for cfg in [dict(lanes=1), dict(lanes=2, tail_div=2, max_iters=3, trace_per_cu=1),
-                    dict(lanes=4, tail_threshold=0, trace_per_cu=32),
+        for cfg in [dict(lanes=1), dict(lanes=2, tail_div=2, max_iters=3, trace_per_cu=1, batch_per_cu=1),
+                    dict(lanes=4, tail_threshold=0, trace_per_cu=32, batch_per_cu=32),
                     dict(lanes=4, tail_threshold=1 << 30, tail_div=1, max_iters=1)]:
             ctx.set_wave_config(**{**default, **cfg})
             assert np.array_equal(ctx.render(pr), base), cfg
+            assert np.array_equal(_bits(ctx.extend(rays)), _bits(hits)), cfg
         for bad in [dict(lanes=0), dict(lanes=5), dict(tail_div=0), dict(max_iters=0), dict(trace_per_cu=0),
-                    dict(trace_per_cu=33)]:
+                    dict(trace_per_cu=33), dict(batch_per_cu=0), dict(batch_per_cu=33)]:
             with pytest.raises(la.LambdaError) as e:
                 ctx.set_wave_config(**{**default, **bad})
             assert e.value.code == la.LAM_E_INVALID
