@@ -379,6 +379,8 @@ typedef struct {
   uint32_t max_iters;      /* synchronised bounces before the finish kernel takes the rest, >= 1 (default 12) */
   uint32_t trace_per_cu;   /* extend / shadow blocks (64 lanes) per CU in render passes: 1..32 (default 16) */
   uint32_t batch_per_cu;   /* ... for ray batches (lam_extend, lam_occluded, lam_extend_dev): 1..32 (default 24) */
+  uint32_t finish_per_cu;  /* finish-kernel blocks (64 lanes) per CU: 1..32 (default 10) */
+  uint32_t shade_per_cu;   /* shade-kernel blocks (256 lanes) per CU: 1..8 (default 3) */
 } lam_wave_config;
 int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out);
 int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg);

@@ -63,6 +63,8 @@ struct lam_ctx {
   uint32_t trace_per_cu = 16;  // extend / shadow blocks per CU in render passes
   uint32_t batch_per_cu = 24;  // ... for ray batches outside a render
   uint32_t batch_blocks = 0;
+  uint32_t finish_per_cu = 10;  // finish-kernel blocks (64 lanes) per CU
+  uint32_t shade_per_cu = 3;    // shade-kernel blocks (256 lanes) per CU
   uint32_t *trace_ctr = nullptr;  // fetch counters of the batch trace kernel (one per launch)
   uint32_t lanes = 3;     // passes in flight
   uint32_t tail_div = 16;  // the finish kernel's threshold is at most 1/tail_div of a chunk
@@ -276,8 +278,8 @@ int ensure_launch(lam_ctx *ctx) {
     // bounds give 3 waves per SIMD = 3 blocks per CU)
     ctx->wl.trace_blocks = c * ctx->trace_per_cu;
     ctx->batch_blocks = c * ctx->batch_per_cu;
-    ctx->wl.finish_blocks = c * 10u;
-    ctx->wl.shade_blocks = c * 3u;
+    ctx->wl.finish_blocks = c * ctx->finish_per_cu;
+    ctx->wl.shade_blocks = c * ctx->shade_per_cu;
     // guiding network: 51 KB of LDS per 256-lane block, 3 blocks per CU
     ctx->wl.guide_blocks = c * 3u;
   }
@@ -1452,6 +1454,8 @@ int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out) {
   out->max_iters = ctx->wl.max_iters;
   out->trace_per_cu = ctx->trace_per_cu;
   out->batch_per_cu = ctx->batch_per_cu;
+  out->finish_per_cu = ctx->finish_per_cu;
+  out->shade_per_cu = ctx->shade_per_cu;
   return LAM_OK;
 }
 
@@ -1465,17 +1469,24 @@ int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
   if (cfg->max_iters < 1) return fail(ctx, LAM_E_INVALID, "max_iters must be >= 1");
   if (cfg->trace_per_cu < 1 || cfg->trace_per_cu > 32) return fail(ctx, LAM_E_INVALID, "trace_per_cu must be 1..32");
   if (cfg->batch_per_cu < 1 || cfg->batch_per_cu > 32) return fail(ctx, LAM_E_INVALID, "batch_per_cu must be 1..32");
+  if (cfg->finish_per_cu < 1 || cfg->finish_per_cu > 32)
+    return fail(ctx, LAM_E_INVALID, "finish_per_cu must be 1..32");
+  if (cfg->shade_per_cu < 1 || cfg->shade_per_cu > 8) return fail(ctx, LAM_E_INVALID, "shade_per_cu must be 1..8");
   ctx->lanes = cfg->lanes;
   ctx->wl.tail_threshold = cfg->tail_threshold;
   ctx->tail_div = cfg->tail_div;
   ctx->wl.max_iters = cfg->max_iters;
   ctx->trace_per_cu = cfg->trace_per_cu;
   ctx->batch_per_cu = cfg->batch_per_cu;
+  ctx->finish_per_cu = cfg->finish_per_cu;
+  ctx->shade_per_cu = cfg->shade_per_cu;
   // before the first launch the CU count is not known yet: ensure_launch
-  // sets the grids from trace_per_cu and batch_per_cu then
+  // sets the grids from the per-CU counts then
   if (ctx->cus > 0) {
     ctx->wl.trace_blocks = uint32_t(ctx->cus) * cfg->trace_per_cu;
     ctx->batch_blocks = uint32_t(ctx->cus) * cfg->batch_per_cu;
+    ctx->wl.finish_blocks = uint32_t(ctx->cus) * cfg->finish_per_cu;
+    ctx->wl.shade_blocks = uint32_t(ctx->cus) * cfg->shade_per_cu;
   }
   return LAM_OK;
 }

@@ -59,7 +59,7 @@ struct WaveLaunch {
   uint32_t tail_threshold;  // queue length at or below which k_finish takes over
   uint32_t max_iters;       // 0: pure wavefront; else cap on bounce iterations per pass
   uint32_t guide_blocks;    // guiding-network grid (256-lane blocks, 51 KB LDS each)
-  uint32_t finish_blocks;   // persistent finish grid (64-lane blocks, 16 KB LDS stacks)
+  uint32_t finish_blocks;   // persistent finish grid (64-lane blocks, split stacks)
 };
 // bounce iterations enqueued per pass
 inline uint32_t wave_iterations(uint32_t max_scat, const WaveLaunch &wl) {
