@@ -9,6 +9,8 @@ whole frame (2,073,600 camera paths).  value = samples/s over the timed
 steps, scene resident in HBM.
 
 Also reported:
+  render_rays   rays the timed render traced (extend + shadow, counted on the
+                device) per second and per sample.
   roofline      extend (closest-hit) kernel on a 2^22-ray batch (camera rays
                 + first-bounce rays) of the same scene: algorithmic bytes per
                 ray = 32 (ray) + 16 (hit) + 128 per BVH8 node fetched + 48
@@ -453,6 +455,7 @@ def main():
         ctx.render(la.params(spp=args.warmup, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
                              spp_offset=1 << 20, flags=args.flags))
     pr = la.params(spp=args.steps, max_scatterings=bounces, sample_p=p, hero=hero, seed=1, flags=args.flags)
+    ctx.reset_timing()
     t0 = time.perf_counter()
     img = ctx.render(pr)
     wall = time.perf_counter() - t0
@@ -469,6 +472,9 @@ def main():
                                "2 area lights, 1920x1080, 32 bounces, p=0.8, hero 4 on 360-830 nm",
                    "width": w, "height": h, "spp_per_step": 1, "bounces": bounces, "bounce_prob": p,
                    "triangles": scene.n_tris},
+        "render_rays": {"rays_per_s": round((tm.extend_rays + tm.shadow_rays) / wall, 1),
+                        "extend_rays_per_sample": round(tm.extend_rays / samples, 4),
+                        "shadow_rays_per_sample": round(tm.shadow_rays / samples, 4)},
     }
     if "roofline" in legs:
         out["roofline"] = extend_roofline(ctx, scene.desc)

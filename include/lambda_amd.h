@@ -191,7 +191,9 @@ typedef struct {
 } lam_scene_stats;
 
 /* Kernel timing accumulated since the last reset (LAM_FLAG_PROFILE, or the
- * extend/occlusion hooks which are always timed). */
+ * extend/occlusion hooks which are always timed).  extend_rays and
+ * shadow_rays also count the rays the passes of lam_render and
+ * lam_render_guided trace (not those of LAM_FLAG_MEGAKERNEL). */
 typedef struct {
   double extend_ms, shadow_ms, shade_ms, raygen_ms, film_ms, other_ms;
   uint64_t extend_launches, shadow_launches, shade_launches;

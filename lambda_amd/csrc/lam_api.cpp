@@ -48,6 +48,7 @@ struct lam_ctx {
     size_t mix_cap = 0;
     uint32_t *spill = nullptr;  // traversal-stack spill columns (WaveState::spill)
     uint32_t spill_lanes = 0;
+    unsigned long long *rays = nullptr;  // ray counters of the lane's passes (WaveState::rays)
   } lane[kMaxLanes];
   // block counts are set on the first render (they depend on the CU count);
   // the finish kernel takes over below 131,072 paths or after 12 bounces
@@ -250,6 +251,25 @@ void finish_film(lam_ctx *ctx, const lam_render_params *p, size_t npix, uint64_t
   ctx->film_spp = base_spp + p->spp;
 }
 
+constexpr size_t kRayBytes = size_t(lam::kRaySlots) * lam::kRaySlotWords * 8;
+
+// the rays counted by the lanes' passes: copied on the library stream, added
+// to the timing after it synchronises
+int read_rays_async(lam_ctx *ctx, std::vector<unsigned long long> &rays) {
+  rays.assign(size_t(kMaxLanes) * lam::kRaySlots * lam::kRaySlotWords, 0ull);
+  for (int l = 0; l < kMaxLanes; l++)
+    if (ctx->lane[l].rays)
+      HIPC(hipMemcpyAsync(rays.data() + size_t(l) * lam::kRaySlots * lam::kRaySlotWords, ctx->lane[l].rays,
+                          kRayBytes, hipMemcpyDeviceToHost, ctx->stream));
+  return LAM_OK;
+}
+void add_rays(lam_ctx *ctx, const std::vector<unsigned long long> &rays) {
+  for (size_t i = 0; i + 1 < rays.size(); i += lam::kRaySlotWords) {
+    ctx->timing.extend_rays += rays[i];
+    ctx->timing.shadow_rays += rays[i + 1];
+  }
+}
+
 void free_wave(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
   for (void *p : ln.bufs) (void)hipFree(p);
@@ -321,8 +341,15 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
     HIPC(hipEventCreateWithFlags(&ln.ev_join, hipEventDisableTiming));
     HIPC(hipEventCreateWithFlags(&ln.ev_film, hipEventDisableTiming));
   }
+  if (!ln.rays) {
+    HIPC(hipMalloc(reinterpret_cast<void **>(&ln.rays), kRayBytes));
+    HIPC(lam::launch_zero(ln.rays, kRayBytes, ctx->stream));
+  }
   if ((rc = ensure_spill(ctx, l))) return rc;
-  if (ln.ws.cap >= cap && ln.hero >= hero) return LAM_OK;
+  if (ln.ws.cap >= cap && ln.hero >= hero) {
+    ln.ws.rays = ln.rays;
+    return LAM_OK;
+  }
   free_wave(ctx, l);
   cap = std::max(cap, size_t(64));
   auto alloc = [&](size_t bytes, void **p) -> int {
@@ -348,6 +375,7 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
   ln.ws.cap = cap;
   ln.ws.spill = ln.spill;
   ln.ws.spill_lanes = ln.spill_lanes;
+  ln.ws.rays = ln.rays;
   ln.hero = hero;
   return LAM_OK;
 }
@@ -534,6 +562,7 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
     // every control block (also of lanes this render leaves idle) starts at
     // zero, so that the counters read after the render are this render's
     if (ctrl[l]) HIPC(lam::launch_zero(ctrl[l], lam::kCtrlWords * 4, st[0]));
+    if (ln.rays) HIPC(lam::launch_zero(ln.rays, kRayBytes, st[0]));
     if (gr) {
       gdl[l] = gr->gd_render;
       gdl[l].mix = static_cast<uint4 *>(ln.mix);
@@ -723,6 +752,7 @@ void lam_destroy(lam_ctx *ctx) {
   if (ctx->tone_buf) (void)hipFree(ctx->tone_buf);
   for (lam_ctx::Lane &ln : ctx->lane) {
     if (ln.ctrl) (void)hipFree(ln.ctrl);
+    if (ln.rays) (void)hipFree(ln.rays);
     if (ln.spill) (void)hipFree(ln.spill);
     for (hipEvent_t e : {ln.ev_film, ln.ev_join})
       if (e) (void)hipEventDestroy(e);
@@ -1069,6 +1099,8 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   }
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   finish_film(ctx, p, npix, base_spp);
+  std::vector<unsigned long long> rays;  // the megakernel does not count its rays
+  if (!(p->flags & LAM_FLAG_MEGAKERNEL) && (rc = read_rays_async(ctx, rays))) return rc;
   const float inv_n = 1.f / float(ctx->film_spp * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
@@ -1076,6 +1108,7 @@ int lam_render(lam_ctx *ctx, const lam_render_params *p, float *linrgb, float *x
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  add_rays(ctx, rays);
   if ((rc = check_ctrl_overflow(ctx))) return rc;
   return check_err_flag(ctx);
 }
@@ -1337,6 +1370,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   uint32_t gc[kMaxLanes][2] = {};
   for (int l = 0; l < kMaxLanes; l++)
     if (ctx->lane[l].ctrl) HIPC(hipMemcpyAsync(gc[l], ctx->lane[l].ctrl + 12, 8, hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<unsigned long long> rays;
+  if ((rc = read_rays_async(ctx, rays))) return rc;
   const float inv_n = 1.f / float(ctx->film_spp * p->hero);
   HIPC(lam::launch_resolve(ctx->sc, film, w, h, inv_n, static_cast<float *>(ctx->scratch[4]),
                            xyz ? static_cast<float *>(ctx->scratch[5]) : nullptr, ctx->stream));
@@ -1344,6 +1379,7 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   if (xyz) HIPC(hipMemcpyAsync(xyz, ctx->scratch[5], npix * 12, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->timing.render_ms = event_ms(ctx);
+  add_rays(ctx, rays);
   ctx->timing.guide_dropped = ctx->timing.guide_zero_pdf = 0;
   for (int l = 0; l < kMaxLanes; l++) {
     ctx->timing.guide_dropped += gc[l][0];

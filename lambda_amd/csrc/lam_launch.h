@@ -52,7 +52,15 @@ struct WaveState {
   // kLdsStack of lane g at spill[(k - kLdsStack) * spill_lanes + g]
   uint32_t *spill;
   uint32_t spill_lanes;
+  // rays traced by the lane's passes (nullable): kRaySlots slots, 128 B
+  // apart, each [0] extend, [1] shadow; their sum is the count
+  unsigned long long *rays;
 };
+// counter slots of WaveState::rays: the finish kernel's waves add to slot
+// (block % kRaySlots), so that its end-of-kernel atomics spread over many
+// cache lines instead of serialising on one address
+constexpr uint32_t kRaySlots = 64;
+constexpr uint32_t kRaySlotWords = 16;
 struct WaveLaunch {
   uint32_t trace_blocks;    // persistent extend / shadow grid (64-lane blocks, split stacks)
   uint32_t shade_blocks;    // shade grid (256-lane blocks)

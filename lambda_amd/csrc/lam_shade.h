@@ -577,10 +577,11 @@ __device__ __forceinline__ float hero_lambdas(const DevScene &sc, float xi, Lam<
 // One vertex of the path: intersect, emission, light IS, material IS and
 // MIS (src/renderer.cpp:64-146), updating (ray, β, L).  Returns false when
 // the path ends (L final), true with the continuation ray in `ray`.
+// n_shadow (nullable) counts the shadow rays traced.
 template <int H, class Stack>
 __device__ __forceinline__ bool path_vertex(const DevScene &sc, uint32_t max_scat, float sample_p, uint64_t key,
                                             const Lam<H> &lam, uint32_t s, Ray &ray, float *beta, float *L,
-                                            const Stack &stk, bool &ovf) {
+                                            const Stack &stk, bool &ovf, uint32_t *n_shadow = nullptr) {
   TravCounts tc{0, 0};
   const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
   if (hit.obj == LAM_NONE || ovf) return false;
@@ -612,6 +613,7 @@ __device__ __forceinline__ bool path_vertex(const DevScene &sc, uint32_t max_sca
   if (ls.shadow) {
     const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
     vis = sh.obj == LAM_NONE;
+    if (n_shadow) (*n_shadow)++;
   }
   float lis[H];
 #pragma unroll

@@ -101,8 +101,12 @@ __global__ __launch_bounds__(256) void k_raygen(DevScene sc, PathArgs pa, PathId
 // queue is short enough that a synchronised bounce costs about as much as
 // its slowest ray, or at the last scheduled iteration (force).
 __global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q, uint32_t tail_threshold,
-                             uint32_t force) {
+                             uint32_t force, unsigned long long *__restrict__ rays) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    // the shadow rays of the previous iteration were traced; the queue of
+    // this one is traced by k_extend unless the finish kernel takes it (it
+    // counts its own rays)
+    if (rays) rays[1] += ctrl[CTRL_SHADOW];
     ctrl[CTRL_COUNT0 + next_q] = 0;
     ctrl[CTRL_SHADOW] = 0;
     ctrl[CTRL_FETCH_EXT] = 0;
@@ -110,6 +114,7 @@ __global__ void k_iter_begin(uint32_t *__restrict__ ctrl, uint32_t next_q, uint3
     ctrl[CTRL_FETCH_FIN] = 0;
     const uint32_t c = ctrl[CTRL_COUNT0 + (next_q ^ 1u)];
     if (c > 0 && (c <= tail_threshold || force)) ctrl[CTRL_TAIL] = 1;
+    if (rays && !ctrl[CTRL_TAIL]) rays[0] += c;
   }
 }
 
@@ -769,7 +774,7 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
   QueuePool qp;
   qp.init(n);
   bool active = false, ovf = false;
-  uint32_t p = 0, s = 0;
+  uint32_t p = 0, s = 0, n_ext = 0, n_shd = 0;
   uint64_t key = 0;
   Lam<H> lam;
   Ray r;
@@ -802,7 +807,8 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
       }
     }
     if (active) {
-      if (path_vertex<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stk, ovf)) {
+      n_ext++;
+      if (path_vertex<H>(sc, pa.max_scat, pa.sample_p, key, lam, s, r, beta, L, stk, ovf, &n_shd)) {
         s++;
       } else {
         if (ovf) atomicOr(ctrl + 15, 1u);
@@ -810,6 +816,17 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
         for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] = L[k];
         active = false;
       }
+    }
+  }
+  if (ws.rays) {  // one pair of atomics per wave, on the block's slot
+    for (int o = 32; o > 0; o >>= 1) {
+      n_ext += __shfl_xor(n_ext, o, 64);
+      n_shd += __shfl_xor(n_shd, o, 64);
+    }
+    if (threadIdx.x % 64 == 0) {
+      unsigned long long *slot = ws.rays + size_t(blockIdx.x % kRaySlots) * kRaySlotWords;
+      atomicAdd(slot, (unsigned long long)n_ext);
+      atomicAdd(slot + 1, (unsigned long long)n_shd);
     }
   }
 }
@@ -858,7 +875,8 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
   for (uint32_t s = 0; s < iters; s++) {
     const bool last = wl.max_iters && s + 1 == iters;
     if (prof) (void)hipEventRecord(prof->fin0[s], st);
-    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u, wl.tail_threshold, last ? 1u : 0u);
+    hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u, wl.tail_threshold, last ? 1u : 0u,
+                       ws.rays);
     if (wl.max_iters)
       hipLaunchKernelGGL((k_finish<H>), dim3(wl.finish_blocks), dim3(64), 0, st, sc, pa, ids, ws, s, q, ctrl);
     if (prof) {

@@ -33,6 +33,8 @@ def _load(f64):
     L.orc_occluded.argtypes = [vp, vp, sz, u32p, C.c_int]
     L.orc_trace_paths.restype = C.c_int
     L.orc_trace_paths.argtypes = [vp, vp, u32p, u32p, sz, fp]
+    L.orc_count_rays.restype = C.c_int
+    L.orc_count_rays.argtypes = [vp, vp, u32p, u32p, sz, C.POINTER(C.c_uint64)]
     L.orc_render.restype = C.c_int
     L.orc_render.argtypes = [vp, vp, fp, fp, C.c_int, C.c_int, C.c_int]
     i32p, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
@@ -100,6 +102,17 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle trace_paths failed")
         return out
+
+    def count_rays(self, p, pixel, sample):
+        """(extend, shadow) rays tracePath traces for these paths"""
+        pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
+        sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        out = np.zeros(2, dtype=np.uint64)
+        rc = self.L.orc_count_rays(self._h, C.cast(C.pointer(p), C.c_void_p), _u(pixel), _u(sample), len(pixel),
+                                   out.ctypes.data_as(C.POINTER(C.c_uint64)))
+        if rc != 0:
+            raise RuntimeError("oracle count_rays failed")
+        return int(out[0]), int(out[1])
 
     def render(self, p, threads=0, row_step=1, row_phase=0, want_xyz=False):
         d = self.scene.desc

@@ -826,10 +826,15 @@ typedef struct {
   real mult[HMAX]; /* includes shadowing, as the reference */
 } olsample;
 
+/* rays traced by this thread's paths (extend queries of trace_path, shadow
+ * queries of the light samples), summed by orc_count_rays */
+static __thread uint64_t n_ext_rays, n_shd_rays;
+
 static int shadowed(const orc_scene *s, v3 p, v3 dir, real tmax, uint32_t skip) {
   oray r;
   ohit h;
   r.o = p; r.d = dir; r.tmax = tmax; r.skip = skip;
+  n_shd_rays++;
   trace(s, &r, 1, 1, &h);
   return h.obj != NONE;
 }
@@ -1023,6 +1028,7 @@ static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro
   for (j = 0; j < K; j++)
     for (k = 0; k < H; k++) dLi[j * HMAX + k] = 0;
   r.o = ro; r.d = rd; r.tmax = UBF; r.skip = NONE;
+  n_ext_rays++;
   trace(s, &r, 0, 1, &h);
   if (h.obj == NONE) { for (k = 0; k < H; k++) Li[k] = 0; return; }
   surface(s, &h, ro, rd, &sf);
@@ -1114,6 +1120,39 @@ int orc_trace_paths(const orc_scene *s, const lam_render_params *p, const uint32
     trace_path(s, &pa, &L, ro, rd, 0, Li, NULL, NULL);
     for (k = 0; k < H; k++) out[i * H + k] = (float)Li[k];
   }
+  return 0;
+}
+
+/* The extend and shadow rays tracePath traces for these (pixel, sample)
+ * paths (src/renderer.cpp:64-146: one closest-hit query per vertex, one
+ * shadow query per light sample): counts[0] extend, counts[1] shadow. */
+int orc_count_rays(const orc_scene *s, const lam_render_params *p, const uint32_t *pixel, const uint32_t *sample,
+                   size_t n, uint64_t *counts) {
+  long i;
+  int H = (int)p->hero;
+  uint64_t ext = 0, shd = 0;
+  if (H != 4 && H != 16) return -1;
+#pragma omp parallel reduction(+ : ext, shd)
+  {
+    n_ext_rays = 0;
+    n_shd_rays = 0;
+#pragma omp for schedule(dynamic, 16)
+    for (i = 0; i < (long)n; i++) {
+      opath pa;
+      olam L;
+      v3 ro, rd;
+      real Li[HMAX];
+      pa.key = om_path_key(p->seed, pixel[i], sample[i]);
+      pa.max_scat = p->max_scatterings;
+      pa.p = (real)p->sample_p;
+      start_path(s, pa.key, pixel[i], H, &L, &ro, &rd);
+      trace_path(s, &pa, &L, ro, rd, 0, Li, NULL, NULL);
+    }
+    ext += n_ext_rays;
+    shd += n_shd_rays;
+  }
+  counts[0] = ext;
+  counts[1] = shd;
   return 0;
 }
 

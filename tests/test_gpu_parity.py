@@ -344,3 +344,37 @@ def test_render_against_fp64_truth(ctx, box_scene):
     relmse = np.mean((g.mean(0) - t.mean(0)) ** 2) / np.mean(t.mean(0) ** 2)
     noise = (np.mean(g.var(0, ddof=1)) / 8 + np.mean(t.var(0, ddof=1)) / 8) / np.mean(t.mean(0) ** 2)
     assert relmse < 3 * noise, (relmse, noise)
+
+
+@pytest.mark.parametrize("scene_name", ["box_scene", "field_scene"])
+def test_render_counts_rays(ctx, request, scene_name):
+    """the rays a render traces (extend queues, shadow queues and the
+    finish kernel's own queries) equal the oracle's count of tracePath's
+    queries for the same (pixel, sample) paths; the pass schedule does not
+    change them"""
+    scene = request.getfixturevalue(scene_name)
+    ctx.upload(scene)
+    orc = ob.Oracle(scene)
+    npix = scene.width * scene.height
+    counts = {}
+    for flags in (0, la.LAM_FLAG_NO_TAIL, la.LAM_FLAG_NO_PIPE, la.LAM_FLAG_NO_GRAPH):
+        pr = la.params(spp=3, max_scatterings=8, sample_p=0.8, hero=4, seed=3, flags=flags)
+        ctx.reset_timing()
+        ctx.render(pr)
+        t = ctx.timing()
+        counts[flags] = (t.extend_rays, t.shadow_rays)
+    assert len(set(counts.values())) == 1, counts
+    pix = np.tile(np.arange(npix, dtype=np.uint32), 3)
+    smp = np.repeat(np.arange(3, dtype=np.uint32), npix)
+    e, s = orc.count_rays(la.params(spp=3, max_scatterings=8, sample_p=0.8, hero=4, seed=3), pix, smp)
+    ge, gs = counts[0]
+    # a sampling decision on a float boundary may differ on a rare path
+    assert abs(ge - e) <= max(2, 1e-4 * e) and abs(gs - s) <= max(2, 1e-4 * s), (ge, e, gs, s)
+    assert e >= 3 * npix
+    # closed form in the closed room: no roulette and no bounce after the
+    # first vertex -> 2 extend rays and 1 shadow ray per path
+    if scene_name == "box_scene":
+        ctx.reset_timing()
+        ctx.render(la.params(spp=2, max_scatterings=0, sample_p=1.0, hero=4, seed=3))
+        t = ctx.timing()
+        assert (t.extend_rays, t.shadow_rays) == (4 * npix, 2 * npix)
