@@ -170,8 +170,16 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=20.0):
         orc.render(pr, threads=threads)
         dt = time.perf_counter() - t0
     samples = h * w * spp_cpu
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(samples / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"full frame {w}x{h}, {spp_cpu} spp, {bounces} bounces, p={p}, hero {hero}; {dt:.1f} s"}
+            "sample": f"full frame {w}x{h}, {spp_cpu} spp, {bounces} bounces, p={p}, hero {hero}; {dt:.1f} s",
+            "cpu": model, "host_logical_cpus": os.cpu_count(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_SCHEDULE", "OMP_NUM_THREADS")}}
 
 
 def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
