@@ -232,7 +232,7 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
 
 
 def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32, max_vertex=0,
-                  steps_per_pass=8, batch=16384):
+                  steps_per_pass=8, batch=16384, materials=0):
     """Learned proposal on the glossy interior: two independent plain runs
     and two guided runs of `spp` samples per pixel (the guided runs train
     the network online on their first `train_passes` passes, inside the timed
@@ -265,7 +265,7 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     for s in (1, 2):
         ctx.guide_reset(s)
         gp = la.guide_params(alpha=alpha, train_passes=train_passes, steps_per_pass=steps_per_pass, batch=batch,
-                             lr=5e-3, seed=s, max_vertex=max_vertex)
+                             lr=5e-3, seed=s, max_vertex=max_vertex, materials=materials)
         t0 = time.perf_counter()
         guided.append(ctx.render_guided(pr(100 + s), gp))
         tg.append(time.perf_counter() - t0)
