@@ -10,7 +10,9 @@ Checks:
        every thin-film thickness partial;
   GPU  lam_adjoint against central finite differences of lam_render at
        h = 1e-3 relative;
-  GPU  the torch.autograd binding against lam_adjoint.
+  GPU  the torch.autograd binding against lam_adjoint;
+  GPU  the albedo gradient on a CSV file's samples against central
+       differences of lam_render over edits of that file.
 """
 import numpy as np
 import pytest
@@ -258,3 +260,41 @@ def test_autograd_binding(ctx, film_scene):
         step = 1e-3 * torch.abs(theta) * torch.sign(g)
         l2 = ((lag.render(ctx, film_scene, dp, p, theta - step) - target) ** 2).sum()
     assert float(l2) < loss.item()
+
+
+@pytest.mark.gpu
+def test_adjoint_on_csv_samples_matches_fd(ctx, tmp_path):
+    """the albedo gradient mapped onto the samples of the material's CSV file
+    (lamh_spectrum_grad_to_samples: the loader's box filter transposed)
+    against central differences of lam_render after rewriting those samples
+    in the file and loading the scene again, h = 1e-3 relative"""
+    from lambda_amd import scenes
+    path = scenes.box_room(str(tmp_path), seed=1, width=32)
+    scene = la.load_nls(path)
+    m = 0
+    csv = tmp_path / f"albedo_{scene.material_name(m)[len('mat_'):]}.csv"
+    data = np.loadtxt(csv, delimiter=",")
+    lam, val = data[:, 0].astype(np.float32), data[:, 1]
+    dp = la.DiffParams(scene, albedo=[m])
+    pr = la.params(spp=8, max_scatterings=6, sample_p=0.8, hero=4, seed=5)
+    rng = np.random.default_rng(12)
+    W = rng.uniform(0.5, 1.5, size=(scene.height, scene.width, 3)).astype(np.float32)
+    ctx.upload(scene)
+    g = ctx.adjoint(pr, dp, W)
+    o = int(dp.mat_albedo[m])
+    gs = la.spectrum_grad_to_samples(scene.desc.grid, lam, g[o:o + scene.desc.grid.n]).astype(np.float64)
+    v = rng.normal(size=len(val)) * val
+
+    def loss(vals):
+        scenes.write_csv(str(csv), lam, vals)
+        ctx.upload(la.load_nls(path))
+        return float((ctx.render(pr).astype(np.float64) * W).sum())
+
+    h = 1e-3
+    vp, vm = val + h * v, val - h * v
+    # the direction the file's float32 values actually moved
+    ve = (np.float32(vp).astype(np.float64) - np.float32(vm).astype(np.float64)) / (2 * h)
+    fd = (loss(vp) - loss(vm)) / (2 * h)
+    ad = float(gs @ ve)
+    assert abs(ad) > 0
+    assert abs(fd - ad) <= 1e-2 * abs(ad), (fd, ad)
