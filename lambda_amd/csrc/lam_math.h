@@ -8,7 +8,7 @@
 // double precision and rounded once to float, so with -ffp-contract=off
 // they give the same bits on x86 and gfx950.  Accuracy: below 1 ulp of
 // float over the argument ranges the renderer uses (checked against libm
-// in tests/test_math.py).
+// in tests/test_oracle.py::test_math_*, and bitwise against the oracle in tests/test_gpu_parity.py).
 //
 // Methods (textbook, no vendor code):
 //   sin/cos  Cody-Waite reduction by pi/2 (two-part constant) + Taylor

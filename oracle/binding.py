@@ -44,6 +44,22 @@ def _load(f64):
     L.orc_math.argtypes = [C.c_int, fp, fp, sz, fp]
     L.orc_rng.restype = None
     L.orc_rng.argtypes = [C.c_uint64, u32p, u32p, u32p, sz, u32p]
+    i32p = C.POINTER(C.c_int32)
+    L.orc_probe_bxdf.restype = C.c_int
+    L.orc_probe_bxdf.argtypes = [vp, C.c_uint32, C.c_int, fp, fp, fp, fp, i32p, sz, fp]
+    L.orc_probe_sample_material.restype = C.c_int
+    L.orc_probe_sample_material.argtypes = [vp, C.c_uint32, C.c_int, fp, fp, i32p, fp, fp, C.c_float, sz, i32p, fp,
+                                            fp, fp]
+    L.orc_probe_prob_material.restype = C.c_int
+    L.orc_probe_prob_material.argtypes = [vp, C.c_uint32, fp, i32p, fp, fp, fp, C.c_float, sz, fp]
+    L.orc_probe_sample_light.restype = C.c_int
+    L.orc_probe_sample_light.argtypes = [vp, C.c_uint32, C.c_int, fp, fp, fp, i32p, fp, sz, fp, fp, fp]
+    L.orc_probe_prob_light.restype = C.c_int
+    L.orc_probe_prob_light.argtypes = [vp, C.c_uint32, fp, fp, sz, fp]
+    L.orc_probe_splat.restype = C.c_int
+    L.orc_probe_splat.argtypes = [vp, fp, fp, sz, fp]
+    L.orc_probe_camera.restype = C.c_int
+    L.orc_probe_camera.argtypes = [vp, C.c_int, fp, sz, fp, fp, fp]
     _LIBS[key] = L
     return L
 
@@ -144,6 +160,109 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle jvp failed")
         return out
+
+
+    # ---- unit probes of single oracle functions (tests pin them against
+    # closed forms and histograms) ----------------------------------------
+    @staticmethod
+    def _f32(a, shape):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), shape), dtype=np.float32)
+
+    @staticmethod
+    def _i32(a, n):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.int32), (n,)), dtype=np.int32)
+
+    def bxdf(self, mat, lam, i, o, n, front=1):
+        """BxDFcosθ of material `mat`: i, o, n (N x 3), lam (N x H) -> N x H"""
+        i = np.atleast_2d(i)
+        N = len(i)
+        lam = np.atleast_2d(lam)
+        H = lam.shape[1]
+        lam, i, o, n = self._f32(lam, (N, H)), self._f32(i, (N, 3)), self._f32(o, (N, 3)), self._f32(n, (N, 3))
+        fr = self._i32(front, N)
+        out = np.zeros((N, H), np.float32)
+        if self.L.orc_probe_bxdf(self._h, int(mat), H, _f(lam), _f(i), _f(o), _f(n),
+                                 fr.ctypes.data_as(C.POINTER(C.c_int32)), N, _f(out)) != 0:
+            raise ValueError("bad probe arguments")
+        return out
+
+    def sample_material(self, mat, lam, frame, o, xi, p, front=1):
+        """sample::materiali: frame (N x 3 x 3 rows x, y, z), o (N x 3), xi
+        (N x 3: xl, xa, xb) -> ok, dir, prob, mult"""
+        xi = np.atleast_2d(xi)
+        N = len(xi)
+        lam = np.atleast_2d(lam)
+        H = lam.shape[1]
+        lam, fr_, o, xi = (self._f32(lam, (N, H)), self._f32(frame, (N, 3, 3)), self._f32(o, (N, 3)),
+                           self._f32(xi, (N, 3)))
+        fr = self._i32(front, N)
+        ok = np.zeros(N, np.int32)
+        d = np.zeros((N, 3), np.float32)
+        pr = np.zeros(N, np.float32)
+        mult = np.zeros((N, H), np.float32)
+        i32 = C.POINTER(C.c_int32)
+        if self.L.orc_probe_sample_material(self._h, int(mat), H, _f(lam), _f(fr_), fr.ctypes.data_as(i32), _f(o),
+                                            _f(xi), float(p), N, ok.ctypes.data_as(i32), _f(d), _f(pr),
+                                            _f(mult)) != 0:
+            raise ValueError("bad probe arguments")
+        return ok.astype(bool), d, pr, mult
+
+    def prob_material(self, mat, frame, wl, o, l0, p, front=1):
+        """probForMateriali as tracePath calls it (wl: light direction)"""
+        N = max(len(np.atleast_2d(wl)), len(np.atleast_2d(o)), np.asarray(l0).size)
+        fr_, wl, o, l0 = self._f32(frame, (N, 3, 3)), self._f32(wl, (N, 3)), self._f32(o, (N, 3)), self._f32(l0, (N,))
+        fr = self._i32(front, N)
+        out = np.zeros(N, np.float32)
+        if self.L.orc_probe_prob_material(self._h, int(mat), _f(fr_), fr.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          _f(wl), _f(o), _f(l0), float(p), N, _f(out)) != 0:
+            raise ValueError("bad probe arguments")
+        return out
+
+    def sample_light(self, light, lam, pos, frame, xi, front=1):
+        """sample::light at surface points pos (N x 3) -> dir, prob, mult"""
+        xi = np.atleast_2d(xi)
+        N = len(xi)
+        lam = np.atleast_2d(lam)
+        H = lam.shape[1]
+        lam, pos, fr_, xi = (self._f32(lam, (N, H)), self._f32(pos, (N, 3)), self._f32(frame, (N, 3, 3)),
+                             self._f32(xi, (N, 2)))
+        fr = self._i32(front, N)
+        d = np.zeros((N, 3), np.float32)
+        pr = np.zeros(N, np.float32)
+        mult = np.zeros((N, H), np.float32)
+        if self.L.orc_probe_sample_light(self._h, int(light), H, _f(lam), _f(pos), _f(fr_),
+                                         fr.ctypes.data_as(C.POINTER(C.c_int32)), _f(xi), N, _f(d), _f(pr),
+                                         _f(mult)) != 0:
+            raise ValueError("bad probe arguments")
+        return d, pr, mult
+
+    def prob_light(self, light, pos, dir):
+        dir = np.atleast_2d(dir)
+        N = len(dir)
+        pos, dir = self._f32(pos, (N, 3)), self._f32(dir, (N, 3))
+        out = np.zeros(N, np.float32)
+        if self.L.orc_probe_prob_light(self._h, int(light), _f(pos), _f(dir), N, _f(out)) != 0:
+            raise ValueError("bad probe arguments")
+        return out
+
+    def splat(self, lam, val):
+        """addHeroλ of (lam[q], val[q]) pairs into an empty grid.n spectrum"""
+        lam = np.atleast_1d(np.asarray(lam, np.float32))
+        val = self._f32(val, lam.shape)
+        bins = np.zeros(self.scene.desc.grid.n, np.float32)
+        self.L.orc_probe_splat(self._h, _f(np.ascontiguousarray(lam)), _f(val), len(lam), _f(bins))
+        return bins
+
+    def camera(self, u, H):
+        """camera ray and hero wavelengths for u = (u, v, lens x, lens y, hero ξ) rows"""
+        u = self._f32(np.atleast_2d(u), (len(np.atleast_2d(u)), 5))
+        N = len(u)
+        ro = np.zeros((N, 3), np.float32)
+        rd = np.zeros((N, 3), np.float32)
+        lam = np.zeros((N, H), np.float32)
+        if self.L.orc_probe_camera(self._h, int(H), _f(u), N, _f(ro), _f(rd), _f(lam)) != 0:
+            raise ValueError("bad probe arguments")
+        return ro, rd, lam
 
 
 def math(fn, a, b=None):

@@ -10,7 +10,7 @@
  *   log2:    x = m 2^e, ln m = 2 atanh((m-1)/(m+1)) (odd series to f^23);
  *   exp2:    2^n * exp(r ln 2), Taylor to r^15;
  *   atan:    two half-angle reductions then the odd series to t^23.
- * They are pinned against glibc in tests/test_oracle_math.py.
+ * They are pinned against glibc in tests/test_oracle.py (test_math_*).
  * RNG: splitmix64 finalizer (Steele, Lea, Flood 2014), keyed by
  * (seed, pixel, sample) plus a per-draw dimension.
  */

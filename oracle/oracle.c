@@ -1074,13 +1074,12 @@ static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro
     }
 }
 
-/* camera ray + hero wavelengths for (pixel, sample) (src/renderer.cpp:35-40,
- * cg.hpp:1517-1542); returns prob_λ = 1/Ω */
-static real start_path(const orc_scene *s, uint64_t key, uint32_t pixel, int H, olam *L, v3 *ro, v3 *rd) {
+/* camera ray + hero wavelengths (src/renderer.cpp:35-40, cg.hpp:1517-1542)
+ * for film position (u, v) in pixels, lens numbers (lx, ly) and hero ξ;
+ * returns prob_λ = 1/Ω */
+static real camera_ray(const orc_scene *s, real u, real v, real lx, real ly, real xh, int H, olam *L, v3 *ro,
+                       v3 *rd) {
   const lam_camera *c = &s->d->camera;
-  uint32_t w = c->width, row = pixel / w, col = pixel - row * w;
-  real u = (real)((float)col + om_flt(key, OM_DIM_U)), v = (real)((float)row + om_flt(key, OM_DIM_V));
-  real lx = (real)om_flt(key, OM_DIM_LX), ly = (real)om_flt(key, OM_DIM_LY);
   v3 pos = fv(c->pos), X = fv(c->x), Y = fv(c->y), Z = fv(c->z), wij, wkl;
   real om, hero, step;
   int k;
@@ -1090,7 +1089,7 @@ static real start_path(const orc_scene *s, uint64_t key, uint32_t pixel, int H, 
   *ro = wkl;
   *rd = vnorm(vsub(wij, wkl));
   om = (real)s->d->grid.sup - (real)s->d->grid.inf;
-  hero = (real)s->d->grid.inf + om * (real)om_flt(key, OM_DIM_HERO);
+  hero = (real)s->d->grid.inf + om * xh;
   step = om / (real)H;
   L->H = H;
   for (k = 0; k < H; k++) {
@@ -1099,6 +1098,14 @@ static real start_path(const orc_scene *s, uint64_t key, uint32_t pixel, int H, 
     L->l[k] = l;
   }
   return (real)1 / om;
+}
+
+/* the camera ray of (pixel, sample) from its counter-RNG key */
+static real start_path(const orc_scene *s, uint64_t key, uint32_t pixel, int H, olam *L, v3 *ro, v3 *rd) {
+  uint32_t w = s->d->camera.width, row = pixel / w, col = pixel - row * w;
+  real u = (real)((float)col + om_flt(key, OM_DIM_U)), v = (real)((float)row + om_flt(key, OM_DIM_V));
+  return camera_ray(s, u, v, (real)om_flt(key, OM_DIM_LX), (real)om_flt(key, OM_DIM_LY),
+                    (real)om_flt(key, OM_DIM_HERO), H, L, ro, rd);
 }
 
 int orc_trace_paths(const orc_scene *s, const lam_render_params *p, const uint32_t *pixel, const uint32_t *sample,
@@ -1326,6 +1333,158 @@ int orc_jvp(const orc_scene *s, const lam_render_params *p, const int32_t *mat_d
   for (px = 0; px < w * h; px++)
     for (j = 0; j < K; j++) out[j] += part[(size_t)px * K + j];
   free(part);
+  return 0;
+}
+
+/* ---- unit probes: one material, light, film or camera function per call ------
+ * Each probe runs the oracle's own function (the one trace_path and
+ * orc_render use) on caller-given geometry and random numbers, so tests can
+ * pin it against closed forms and histograms of the mathematics.  Arrays
+ * are float regardless of the build's `real`.
+ *   frame: 9 floats per query (x, y, z of the shading frame, z = normal)
+ *   lam:   H floats per query (hero wavelengths) */
+static void probe_frame(const float *f, int front, osurf *sf) {
+  sf->F.x = fv(f);
+  sf->F.y = fv(f + 3);
+  sf->F.z = fv(f + 6);
+  sf->front = front;
+  sf->p = V(0, 0, 0);
+  sf->gn = sf->F.z;
+  sf->mat = sf->obj = NONE;
+}
+static void probe_lam(int H, const float *lam, olam *L) {
+  int k;
+  L->H = H;
+  for (k = 0; k < H; k++) L->l[k] = (real)lam[k];
+}
+
+/* BxDFcosθ of material `mat` (bxdf): out = cnt*H */
+int orc_probe_bxdf(const orc_scene *s, uint32_t mat, int H, const float *lam, const float *i, const float *o,
+                   const float *n, const int32_t *front, size_t cnt, float *out) {
+  size_t q;
+  int k;
+  if (mat >= s->d->n_materials || H < 1 || H > HMAX) return -1;
+  for (q = 0; q < cnt; q++) {
+    olam L;
+    real v[HMAX];
+    probe_lam(H, lam + q * H, &L);
+    bxdf(s, &s->d->materials[mat], &L, fv(i + 3 * q), fv(o + 3 * q), fv(n + 3 * q), front[q], v);
+    for (k = 0; k < H; k++) out[q * H + k] = (float)v[k];
+  }
+  return 0;
+}
+
+/* sample::materiali (sample_material) with random numbers xi = (xl, xa, xb)
+ * per query: ok, dir (3), prob, mult (H) */
+int orc_probe_sample_material(const orc_scene *s, uint32_t mat, int H, const float *lam, const float *frame,
+                              const int32_t *front, const float *o, const float *xi, float p, size_t cnt,
+                              int32_t *ok, float *dir, float *prob, float *mult) {
+  size_t q;
+  int k;
+  if (mat >= s->d->n_materials || H < 1 || H > HMAX) return -1;
+  for (q = 0; q < cnt; q++) {
+    olam L;
+    osurf sf;
+    omsample ms;
+    probe_lam(H, lam + q * H, &L);
+    probe_frame(frame + 9 * q, front[q], &sf);
+    memset(&ms, 0, sizeof ms);
+    sample_material(s, &s->d->materials[mat], &L, &sf, fv(o + 3 * q), (real)xi[3 * q], (real)xi[3 * q + 1],
+                    (real)xi[3 * q + 2], (real)p, &ms);
+    ok[q] = ms.ok;
+    dir[3 * q] = (float)ms.dir.x;
+    dir[3 * q + 1] = (float)ms.dir.y;
+    dir[3 * q + 2] = (float)ms.dir.z;
+    prob[q] = (float)ms.prob;
+    for (k = 0; k < H; k++) mult[q * H + k] = ms.ok ? (float)ms.mult[k] : 0.f;
+  }
+  return 0;
+}
+
+/* probForMateriali (prob_material), called as trace_path calls it:
+ * wl = the light sample's direction, o = the outgoing direction */
+int orc_probe_prob_material(const orc_scene *s, uint32_t mat, const float *frame, const int32_t *front,
+                            const float *wl, const float *o, const float *l0, float p, size_t cnt, float *out) {
+  size_t q;
+  if (mat >= s->d->n_materials) return -1;
+  for (q = 0; q < cnt; q++) {
+    osurf sf;
+    probe_frame(frame + 9 * q, front[q], &sf);
+    out[q] = (float)prob_material(s, &s->d->materials[mat], &sf, fv(wl + 3 * q), fv(o + 3 * q), (real)l0[q],
+                                  (real)p);
+  }
+  return 0;
+}
+
+/* sample::light (sample_light, including its shadow query) at surface point
+ * pos with shading frame `frame`: dir (3), prob, mult (H) */
+int orc_probe_sample_light(const orc_scene *s, uint32_t light, int H, const float *lam, const float *pos,
+                           const float *frame, const int32_t *front, const float *xi, size_t cnt, float *dir,
+                           float *prob, float *mult) {
+  size_t q;
+  int k;
+  if (light >= s->d->n_lights || H < 1 || H > HMAX) return -1;
+  for (q = 0; q < cnt; q++) {
+    olam L;
+    osurf sf;
+    olsample ls;
+    probe_lam(H, lam + q * H, &L);
+    probe_frame(frame + 9 * q, front[q], &sf);
+    sf.p = fv(pos + 3 * q);
+    sample_light(s, &s->d->lights[light], &L, &sf, (real)xi[2 * q], (real)xi[2 * q + 1], &ls);
+    dir[3 * q] = (float)ls.dir.x;
+    dir[3 * q + 1] = (float)ls.dir.y;
+    dir[3 * q + 2] = (float)ls.dir.z;
+    prob[q] = (float)ls.prob;
+    for (k = 0; k < H; k++) mult[q * H + k] = (float)ls.mult[k];
+  }
+  return 0;
+}
+
+/* probForLight (prob_light) */
+int orc_probe_prob_light(const orc_scene *s, uint32_t light, const float *pos, const float *dir, size_t cnt,
+                         float *out) {
+  size_t q;
+  if (light >= s->d->n_lights) return -1;
+  for (q = 0; q < cnt; q++) out[q] = (float)prob_light(s, &s->d->lights[light], fv(pos + 3 * q), fv(dir + 3 * q));
+  return 0;
+}
+
+/* addHeroλ (add_hero): splats the (lam[q], val[q]) pairs one at a time into
+ * the grid.n-bin spectrum `bins` (accumulated, not cleared) */
+int orc_probe_splat(const orc_scene *s, const float *lam, const float *val, size_t cnt, float *bins) {
+  uint32_t n = s->d->grid.n, b;
+  size_t q;
+  real acc[512];
+  if (n > 512) return -1;
+  for (b = 0; b < n; b++) acc[b] = (real)bins[b];
+  for (q = 0; q < cnt; q++) {
+    olam L;
+    real v = (real)val[q];
+    L.H = 1;
+    L.l[0] = (real)lam[q];
+    add_hero(s, acc, &L, &v);
+  }
+  for (b = 0; b < n; b++) bins[b] = (float)acc[b];
+  return 0;
+}
+
+/* camera_ray for explicit sample numbers: u = (u, v, lens x, lens y,
+ * hero ξ) per query with u, v film positions in pixels (column + fraction,
+ * row from the bottom + fraction) -> ro (3), rd (3), lam (H) */
+int orc_probe_camera(const orc_scene *s, int H, const float *u, size_t cnt, float *ro, float *rd, float *lam) {
+  size_t q;
+  int k;
+  if (H < 1 || H > HMAX) return -1;
+  for (q = 0; q < cnt; q++) {
+    const float *x = u + 5 * q;
+    olam L;
+    v3 o, d;
+    camera_ray(s, (real)x[0], (real)x[1], (real)x[2], (real)x[3], (real)x[4], H, &L, &o, &d);
+    ro[3 * q] = (float)o.x; ro[3 * q + 1] = (float)o.y; ro[3 * q + 2] = (float)o.z;
+    rd[3 * q] = (float)d.x; rd[3 * q + 1] = (float)d.y; rd[3 * q + 2] = (float)d.z;
+    for (k = 0; k < H; k++) lam[q * H + k] = (float)L.l[k];
+  }
   return 0;
 }
 
