@@ -39,7 +39,6 @@ Also reported:
                 run time x relMSE / 1e-3 (variance falls as 1/spp).
 """
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -87,22 +86,6 @@ def camera_rays(desc):
     return r
 
 
-def hip_lib():
-    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
-        try:
-            h = C.CDLL(name)
-        except OSError:
-            continue
-        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        h.hipMalloc.restype = C.c_int
-        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        h.hipMemcpy.restype = C.c_int
-        h.hipFree.argtypes = [C.c_void_p]
-        h.hipFree.restype = C.c_int
-        return h
-    raise RuntimeError("libamdhip64.so not found")
-
-
 def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     cam = camera_rays(desc)
     hits = ctx.extend(cam)
@@ -119,20 +102,16 @@ def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     b["skip"] = la.LAM_NONE
     rays = np.concatenate([cam, b])
     n = len(rays)
-    hip = hip_lib()
-    d_rays, d_hits, d_cnt = C.c_void_p(), C.c_void_p(), C.c_void_p()
-    for p, sz in ((d_rays, n * 32), (d_hits, n * 16), (d_cnt, n * 8)):
-        assert hip.hipMalloc(C.byref(p), C.c_size_t(sz)) == 0
-    assert hip.hipMemcpy(d_rays, rays.ctypes.data, C.c_size_t(n * 32), 1) == 0  # H2D
+    b_rays, b_hits, b_cnt = la.DeviceBuffer.from_array(rays), la.DeviceBuffer(n * 16), la.DeviceBuffer(n * 8)
+    d_rays, d_hits, d_cnt = b_rays.ptr, b_hits.ptr, b_cnt.ptr
     ctx.extend_dev(d_rays, n, d_hits, d_cnt)
-    cnt = np.zeros(2 * n, np.uint32)
-    assert hip.hipMemcpy(cnt.ctypes.data, d_cnt, C.c_size_t(n * 8), 2) == 0  # D2H
+    cnt = b_cnt.download(np.uint32, 2 * n)
     nodes, tris = cnt[0::2].astype(np.float64), cnt[1::2].astype(np.float64)
     bytes_per_launch = float(np.sum(32 + 16 + 128 * nodes + 48 * tris))
     ctx.extend_dev(d_rays, n, d_hits)  # warm
     avg_ms = ctx.extend_dev_timed(d_rays, n, d_hits, reps)
-    for p in (d_rays, d_hits, d_cnt):
-        hip.hipFree(p)
+    for b in (b_rays, b_hits, b_cnt):
+        b.free()
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
     rec_path = os.path.join(ROOT, "profiles", "extend_icosphere_field_4Mrays.traffic.json")
@@ -448,14 +427,12 @@ def main():
     w, h = scene.width, scene.height
     if args.extend_only:
         rl = extend_roofline(ctx, scene.desc, reps=3)
-        hip = hip_lib()
-        a, b = C.c_void_p(), C.c_void_p()
         nbytes = 1 << 30
-        assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
-        for _ in range(3):
-            assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # device to device
-        hip.hipFree(a)
-        hip.hipFree(b)
+        a, b = la.DeviceBuffer(nbytes), la.DeviceBuffer(nbytes)
+        for _ in range(3):  # device to device: the counters' scale reference
+            assert la._hip().hipMemcpy(b.ptr, a.ptr, nbytes, 3) == 0
+        a.free()
+        b.free()
         print(json.dumps({"extend": rl, "copy_bytes": nbytes}), flush=True)
         ctx.close()
         return

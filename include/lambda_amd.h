@@ -327,6 +327,18 @@ int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n,
  * triangles tested per ray. */
 int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
                    lam_hit *d_hits, uint32_t *d_counts);
+/* lam_extend_dev for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The caller passes the
+ * device rays and hits; the ray-queue counter, the overflow flag and the
+ * traversal-stack spill columns live in a static device array of
+ * 54,526,208 bytes that every GPU the library runs on holds until the
+ * process ends (a grid of at most 2,048 64-lane blocks).  Results are those
+ * of lam_extend_dev bit for bit.  n <= 2^31 (LAM_E_LIMIT beyond).  Calls
+ * that overlap on one GPU (from several contexts or threads) race on that
+ * array.  Synchronous on return, like lam_extend_dev. */
+int lam_extend_dev_noalloc(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
+                           lam_hit *d_hits);
 /* Measurement hook: `reps` (<= 4096) back-to-back closest-hit launches
  * over device rays between one pair of HIP events on the context's stream;
  * *avg_ms receives the mean duration of one launch. */

@@ -200,6 +200,7 @@ def lib():
         "lam_occluded": (C.c_int, [vp, vp, sz, u32p]),
         "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
         "lam_extend_dev_timed": (C.c_int, [vp, vp, sz, vp, C.c_uint32, C.POINTER(C.c_double)]),
+        "lam_extend_dev_noalloc": (C.c_int, [vp, vp, sz, vp]),
         "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
@@ -257,6 +258,79 @@ def _carr(ptr, n, dtype):
     if n == 0 or not ptr:
         return np.zeros(0, dtype=dtype)
     return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+
+_HIP = None
+
+
+def _hip():
+    """libamdhip64 for the device buffers of the `_dev` entry points"""
+    global _HIP
+    if _HIP is None:
+        lib()  # its NEEDED libamdhip64.so.7 is the runtime instance to share
+        # (another name could load a second copy of the runtime, e.g. beside
+        # torch's own libamdhip64, which then reports no device)
+        for name in ("libamdhip64.so.7", "libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                h = C.CDLL(name)
+            except OSError:
+                continue
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMalloc.restype = C.c_int
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemcpy.restype = C.c_int
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipFree.restype = C.c_int
+            _HIP = h
+            break
+        else:
+            raise RuntimeError("libamdhip64.so not found")
+    return _HIP
+
+
+class DeviceBuffer:
+    """A block of device memory (hipMalloc) for the `_dev` entry points, on
+    the current HIP device."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        rc = _hip().hipMalloc(C.byref(self.ptr), C.c_size_t(max(self.nbytes, 16)))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({self.nbytes}) failed: {rc}")
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if _hip().hipMemcpy(self.ptr, a.ctypes.data, C.c_size_t(a.nbytes), 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    def download(self, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("read past the buffer")
+        if _hip().hipMemcpy(out.ctypes.data, self.ptr, C.c_size_t(out.nbytes), 2) != 0:  # device to host
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self.ptr.value:
+            _hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Scene:
@@ -564,6 +638,10 @@ class Context:
 
     def extend_dev(self, rays_ptr, n, hits_ptr, counts_ptr=None):
         self._check(self._lib.lam_extend_dev(self._h, rays_ptr, n, hits_ptr, counts_ptr))
+
+    def extend_dev_noalloc(self, rays_ptr, n, hits_ptr):
+        """lam_extend_dev without device allocation (static device scratch)"""
+        self._check(self._lib.lam_extend_dev_noalloc(self._h, rays_ptr, n, hits_ptr))
 
     def extend_dev_timed(self, rays_ptr, n, hits_ptr, reps):
         ms = C.c_double()

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,46 @@
 static constexpr int kMaxLanes = 4;
 // launches of the batch trace kernel that one timed call may queue
 static constexpr uint32_t kTraceCounters = 4096;
+
+// ---- device scratch of lam_extend_dev_noalloc ------------------------------
+// A 256-byte header (word 0: the ray-queue counter, word 4: the traversal-
+// stack overflow flag) followed by the traversal-stack spill columns of a
+// grid of up to kNoAllocBlocks 64-lane blocks, (kStack - kLdsStack) words per
+// lane: 64 + 104 * 2048 * 64 words = 54,526,208 bytes.  Every GPU the library
+// runs on holds one copy until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host it names a stand-in, not device
+// memory): it uses the per-device address from hipGetSymbolAddress.
+static constexpr uint32_t kNoAllocBlocks = 2048;
+static constexpr size_t kNoAllocHeaderWords = 64;
+static constexpr size_t kNoAllocWords =
+    kNoAllocHeaderWords + size_t(lam::kStack - lam::kLdsStack) * kNoAllocBlocks * 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+// per-device address of `scratch`, resolved once per device by lam_create
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// with `device` current; returns the device address or nullptr (*rc set)
+uint32_t *scratch_address(int device, hipError_t *rc) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    *rc = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  if (!g_scratch_done[device]) {
+    void *p = nullptr;
+    g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_rc[device] == hipSuccess ? static_cast<uint32_t *>(p) : nullptr;
+    g_scratch_done[device] = true;
+  }
+  *rc = g_scratch_rc[device];
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 struct lam_ctx {
   int device = 0;
@@ -61,6 +102,9 @@ struct lam_ctx {
   void *tone_buf = nullptr;  // tone map: 1024 per-block log sums (double) + the exposure
   lam::WaveLaunch wl{0, 0, 131072u, 12u, 0, 0};
   int cus = 0;
+  // this device's copy of `scratch` (lam_extend_dev_noalloc), set by lam_create
+  uint32_t *noalloc = nullptr;
+  hipError_t noalloc_rc = hipSuccess;
   uint32_t trace_per_cu = 16;  // extend / shadow blocks per CU in render passes
   uint32_t batch_per_cu = 24;  // ... for ray batches outside a render
   uint32_t batch_blocks = 0;
@@ -730,6 +774,9 @@ int lam_create(lam_ctx **out, int device) {
       rc = LAM_E_DEVICE;
       break;
     }
+    // the device address of the no-alloc scratch; a failure is reported by
+    // lam_extend_dev_noalloc, the other entry points do not use it
+    ctx->noalloc = scratch_address(device, &ctx->noalloc_rc);
   } while (false);
   if (rc != LAM_OK) {
     lam_destroy(ctx);
@@ -1000,6 +1047,33 @@ int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hit
   ctx->timing.extend_launches++;
   ctx->timing.extend_rays += n;
   return check_err_flag(ctx);
+}
+
+int lam_extend_dev_noalloc(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hits) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large (at most 2^31 rays)");
+  if (n && (!d_rays || !d_hits)) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (!ctx->noalloc)
+    return fail(ctx, LAM_E_DEVICE, std::string("no-alloc scratch unavailable: hipGetSymbolAddress: ") +
+                                       hipGetErrorString(ctx->noalloc_rc));
+  if (n == 0) return LAM_OK;
+  HIPC(hipSetDevice(ctx->device));
+  int cus = ctx->cus;
+  if (cus <= 0) HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  // 8 blocks per CU, at most the grid the spill columns are sized for
+  const uint32_t blocks = std::min<uint32_t>(uint32_t(std::max(cus, 1)) * 8u, kNoAllocBlocks);
+  uint32_t *fetch = ctx->noalloc, *err = ctx->noalloc + 4, *spill = ctx->noalloc + kNoAllocHeaderWords;
+  // the queue counter and the overflow flag start at zero (a fill kernel,
+  // not a memset node, so the call can be captured)
+  HIPC(lam::launch_zero(ctx->noalloc, kNoAllocHeaderWords * 4, ctx->stream));
+  HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, d_hits, nullptr, nullptr, err, fetch, spill, blocks,
+                              ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  uint32_t flag = 0;
+  HIPC(hipMemcpy(&flag, err, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(ctx, LAM_E_BVH, "BVH traversal stack overflow");
+  return LAM_OK;
 }
 
 int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hits, uint32_t reps,
