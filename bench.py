@@ -41,6 +41,7 @@ Also reported:
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -128,10 +129,46 @@ def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     }
 
 
+def _cpu_info():
+    """CPU model, sockets, physical cores and logical CPUs from lscpu (or
+    /proc/cpuinfo), and the CPUs this process may run on"""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(ln.split(":", 1) for ln in out.splitlines() if ":" in ln)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name", "")
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        info["sockets"] = sockets
+        info["physical_cores"] = sockets * cores
+        info["logical_cpus"] = int(kv.get("CPU(s)", "0") or 0)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if not info.get("model"):
+        try:
+            with open("/proc/cpuinfo") as f:
+                info["model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+    try:
+        info["usable_cpus"] = len(os.sched_getaffinity(0))
+    except OSError:
+        info["usable_cpus"] = os.cpu_count()
+    return info
+
+
 def cpu_baseline(scene, spp, bounces, p, hero, budget_s=20.0):
+    """The FP32 oracle (OpenMP over pixels, collapse(2), recursive paths,
+    -O3 -mavx2) timed on this host: threads = OMP_NUM_THREADS where it is set
+    (the CPU share of one GPU on a shared host), else every usable CPU;
+    OMP_PROC_BIND=spread (set by main() before the OpenMP runtime loads).
+    It traverses one BVH over the flattened instances, so it is faster than
+    the reference's per-object linear loop on scenes of many instances."""
     from oracle.binding import Oracle
     orc = Oracle(scene)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    info = _cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info.get("usable_cpus") or 1
     h = scene.height
     w = scene.width
     pr = la.params(spp=1, max_scatterings=bounces, sample_p=p, hero=hero, seed=7)
@@ -149,15 +186,12 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=20.0):
         orc.render(pr, threads=threads)
         dt = time.perf_counter() - t0
     samples = h * w * spp_cpu
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
     return {"value": round(samples / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": f"full frame {w}x{h}, {spp_cpu} spp, {bounces} bounces, p={p}, hero {hero}; {dt:.1f} s",
-            "cpu": model, "host_logical_cpus": os.cpu_count(),
+            "cpu": info.get("model", ""), "sockets": info.get("sockets"),
+            "physical_cores": info.get("physical_cores"), "host_logical_cpus": info.get("logical_cpus"),
+            "usable_cpus": info.get("usable_cpus"),
+            "note": "one BVH over the flattened instances (the reference loops over objects linearly)",
             "omp": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_SCHEDULE", "OMP_NUM_THREADS")}}
 
 
@@ -389,6 +423,9 @@ def main():
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
                          "(the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP threads spread over the cores (read by the
+    # OpenMP runtime when the oracle library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
     if args.gpus != 1:
         print(json.dumps({"error": "this renderer runs on one GPU; --gpus > 1 is not supported "
                                    "(samples would shard by pixel tiles with one film all-reduce)"}))

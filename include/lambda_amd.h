@@ -256,6 +256,7 @@ int lam_trace_paths(lam_ctx *ctx, const lam_render_params *p,
  * coefficient; a flat spectrum's scalar gradient is their sum. */
 typedef struct {
   uint32_t n_params;
+  uint32_t n_materials; /* entries of each map: the uploaded scene's material count */
   const int32_t *mat_d;
   const int32_t *mat_ior;
   const int32_t *mat_albedo;

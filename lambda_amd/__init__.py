@@ -106,7 +106,7 @@ class SceneDesc(C.Structure):
 
 
 class DiffParamsC(C.Structure):
-    _fields_ = [("n_params", C.c_uint32), ("mat_d", C.POINTER(C.c_int32)),
+    _fields_ = [("n_params", C.c_uint32), ("n_materials", C.c_uint32), ("mat_d", C.POINTER(C.c_int32)),
                 ("mat_ior", C.POINTER(C.c_int32)), ("mat_albedo", C.POINTER(C.c_int32))]
 
 
@@ -532,8 +532,8 @@ class DiffParams:
 
     def c_struct(self):
         i32 = C.POINTER(C.c_int32)
-        return DiffParamsC(self.n_params, self.mat_d.ctypes.data_as(i32), self.mat_ior.ctypes.data_as(i32),
-                           self.mat_albedo.ctypes.data_as(i32))
+        return DiffParamsC(self.n_params, len(self.mat_d), self.mat_d.ctypes.data_as(i32),
+                           self.mat_ior.ctypes.data_as(i32), self.mat_albedo.ctypes.data_as(i32))
 
     def values(self, scene):
         """The parameter vector θ of a scene (or variant)."""
@@ -651,6 +651,8 @@ class Context:
     def trace_paths(self, p, pixel, sample):
         pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
         sample = np.ascontiguousarray(sample, dtype=np.uint32)
+        if pixel.shape != sample.shape or pixel.ndim != 1:
+            raise ValueError(f"pixel {pixel.shape} and sample {sample.shape} must be 1-D of one length")
         out = np.zeros((len(pixel), p.hero), dtype=np.float32)
         self._check(self._lib.lam_trace_paths(self._h, C.byref(p), _u32ptr(pixel), _u32ptr(sample),
                                               len(pixel), _fptr(out)))
@@ -672,6 +674,9 @@ class Context:
         g = np.ascontiguousarray(dloss_drgb, dtype=np.float32)
         if g.shape != (h, w, 3):
             raise ValueError(f"dloss_drgb has shape {g.shape}")
+        if self.scene is not None and dp.n_materials != self.scene.desc.n_materials:
+            raise ValueError(f"DiffParams built for {dp.n_materials} materials, the uploaded scene has "
+                             f"{self.scene.desc.n_materials}")
         grad = np.zeros(dp.n_params, dtype=np.float32)
         cs = dp.c_struct()
         self._check(self._lib.lam_adjoint(self._h, C.byref(p), C.byref(cs), _fptr(g), _fptr(grad)))

@@ -1194,6 +1194,10 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   if ((rc = check_params(ctx, p))) return rc;
   if (!dp || !dloss_drgb || !grad) return fail(ctx, LAM_E_INVALID, "NULL argument");
   if (!dp->mat_d || !dp->mat_ior || !dp->mat_albedo) return fail(ctx, LAM_E_INVALID, "parameter maps missing");
+  if (ctx->has_scene && dp->n_materials != ctx->sc.n_materials)
+    return fail(ctx, LAM_E_INVALID, "parameter maps have " + std::to_string(dp->n_materials) +
+                                        " entries, the uploaded scene " + std::to_string(ctx->sc.n_materials) +
+                                        " materials");
   if (dp->n_params == 0) return LAM_OK;
   if (dp->n_params > lam::kMaxDiffParams)
     return fail(ctx, LAM_E_LIMIT, "at most " + std::to_string(lam::kMaxDiffParams) + " parameters");

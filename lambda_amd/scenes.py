@@ -5,7 +5,8 @@ OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
                      blocks (24 tris), one 0.5 m x 0.5 m ceiling light with
                      a 6500 K blackbody spectrum, Lambertian surfaces with
                      seeded smooth reflectance spectra (sum of 3 Gaussians,
-                     clipped to [0.05, 0.9]); 256x256, hero 4 on 360-830 nm.
+                     clipped to [0.05, 0.9], 81 samples on 380-780 nm);
+                     256x256, hero 4 on 360-830 nm.
   icosphere_field(dir)
                      64 instances of one noise-displaced geodesic sphere
                      (frequency 28: 15,680 tris, ~1.0 M in total) with random
@@ -17,7 +18,8 @@ OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
 
   box_room_films(dir) the box room with each of its 10 quads and 3 face groups
                      of each block given a Lambertian material of its own (16
-                     albedos), plus 64 small thin-film spheres (d U[200,800]
+                     albedos, 81 samples on 380-780 nm each), plus 64 small
+                     thin-film spheres (d U[200,800]
                      nm, film IOR U[1.33,1.6] over base 1.5) on a jittered
                      8 x 8 grid in front of the back wall; 512x512.  The
                      workload of the path-replay adjoint.
@@ -121,7 +123,9 @@ def _block(x0, y0, x1, y1, h):
 def box_room(out_dir, seed=1, width=256, hero=4):
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
-    lam_ref = np.arange(355.0, 836.0, 5.0)
+    # reflectances on 81 samples, 380-780 nm in 5 nm steps (the grid's bins
+    # outside that range receive nothing from the box filter: albedo 0)
+    lam_ref = np.arange(380.0, 781.0, 5.0)
     lam_bb = np.arange(355.0, 841.0, 1.0)
     bb = blackbody(6500.0, lam_bb)
     write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])
@@ -213,7 +217,8 @@ def box_room_films(out_dir, seed=3, width=512, hero=4, n_films=64, film_freq=3, 
                    glossiness=32):
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
-    lam_ref = np.arange(355.0, 836.0, 5.0)
+    # 16 albedos on 81 samples, 380-780 nm in 5 nm steps: 16 x 81 CSV values
+    lam_ref = np.arange(380.0, 781.0, 5.0)
     lam_bb = np.arange(355.0, 841.0, 1.0)
     bb = blackbody(6500.0, lam_bb)
     write_csv(os.path.join(out_dir, "blackbody6500.csv"), lam_bb, bb / bb[lam_bb == 560.0][0])

@@ -44,6 +44,10 @@ def _load(f64):
     L.orc_math.argtypes = [C.c_int, fp, fp, sz, fp]
     L.orc_rng.restype = None
     L.orc_rng.argtypes = [C.c_uint64, u32p, u32p, u32p, sz, u32p]
+    L.orc_path_rays.restype = C.c_long
+    L.orc_path_rays.argtypes = [vp, vp, C.c_uint32, C.c_uint32, vp, u32p, sz]
+    L.orc_set_path_brute.restype = None
+    L.orc_set_path_brute.argtypes = [C.c_int]
     i32p = C.POINTER(C.c_int32)
     L.orc_probe_bxdf.restype = C.c_int
     L.orc_probe_bxdf.argtypes = [vp, C.c_uint32, C.c_int, fp, fp, fp, fp, i32p, sz, fp]
@@ -118,6 +122,19 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle trace_paths failed")
         return out
+
+    def path_rays(self, p, pixel, sample, cap=256):
+        """the queries one path makes, in order: (rays, kind) with kind 0
+        closest hit, 1 shadow"""
+        from lambda_amd import RAY_DTYPE
+        rays = np.zeros(cap, dtype=RAY_DTYPE)
+        kind = np.zeros(cap, dtype=np.uint32)
+        n = self.L.orc_path_rays(self._h, C.cast(C.pointer(p), C.c_void_p), int(pixel), int(sample),
+                                 rays.ctypes.data, _u(kind), cap)
+        if n < 0:
+            raise RuntimeError("oracle path_rays failed")
+        n = min(n, cap)
+        return rays[:n], kind[:n]
 
     def count_rays(self, p, pixel, sample):
         """(extend, shadow) rays tracePath traces for these paths"""

@@ -476,6 +476,11 @@ static int trace(const orc_scene *s, const oray *r, int any, int mode, ohit *bes
   return 0;
 }
 
+/* intersection mode of the path tracer's queries: 1 BVH (default), 0 brute
+ * force over every triangle (orc_set_path_brute, a test hook) */
+static int path_trace_mode = 1;
+void orc_set_path_brute(int brute) { path_trace_mode = brute ? 0 : 1; }
+
 static oray ray_from(const lam_ray *r) {
   oray o;
   o.o = fv(r->o);
@@ -830,12 +835,29 @@ typedef struct {
  * queries of the light samples), summed by orc_count_rays */
 static __thread uint64_t n_ext_rays, n_shd_rays;
 
+/* ray log of orc_path_rays: every query of one path, in order */
+static __thread lam_ray *rec_rays;
+static __thread uint32_t *rec_kind;
+static __thread size_t rec_n, rec_cap;
+static void rec_ray(const oray *r, uint32_t kind) {
+  if (rec_rays && rec_n < rec_cap) {
+    lam_ray *q = &rec_rays[rec_n];
+    q->o[0] = (float)r->o.x; q->o[1] = (float)r->o.y; q->o[2] = (float)r->o.z;
+    q->d[0] = (float)r->d.x; q->d[1] = (float)r->d.y; q->d[2] = (float)r->d.z;
+    q->tmax = (float)r->tmax;
+    q->skip = r->skip;
+    rec_kind[rec_n] = kind;
+  }
+  if (rec_rays) rec_n++;
+}
+
 static int shadowed(const orc_scene *s, v3 p, v3 dir, real tmax, uint32_t skip) {
   oray r;
   ohit h;
   r.o = p; r.d = dir; r.tmax = tmax; r.skip = skip;
+  rec_ray(&r, 1);
   n_shd_rays++;
-  trace(s, &r, 1, 1, &h);
+  trace(s, &r, 1, path_trace_mode, &h);
   return h.obj != NONE;
 }
 
@@ -1028,8 +1050,9 @@ static void trace_path(const orc_scene *s, const opath *pa, const olam *L, v3 ro
   for (j = 0; j < K; j++)
     for (k = 0; k < H; k++) dLi[j * HMAX + k] = 0;
   r.o = ro; r.d = rd; r.tmax = UBF; r.skip = NONE;
+  rec_ray(&r, 0);
   n_ext_rays++;
-  trace(s, &r, 0, 1, &h);
+  trace(s, &r, 0, path_trace_mode, &h);
   if (h.obj == NONE) { for (k = 0; k < H; k++) Li[k] = 0; return; }
   surface(s, &h, ro, rd, &sf);
   o = vneg(vnorm(rd));
@@ -1128,6 +1151,27 @@ int orc_trace_paths(const orc_scene *s, const lam_render_params *p, const uint32
     for (k = 0; k < H; k++) out[i * H + k] = (float)Li[k];
   }
   return 0;
+}
+
+/* The queries of one path in the order tracePath makes them: rays (up to
+ * cap) and kind (0 closest hit, 1 shadow); returns the number of queries
+ * (may exceed cap).  A test hook for locating where two tracers part. */
+long orc_path_rays(const orc_scene *s, const lam_render_params *p, uint32_t pixel, uint32_t sample, lam_ray *rays,
+                   uint32_t *kind, size_t cap) {
+  opath pa;
+  olam L;
+  v3 ro, rd;
+  real Li[HMAX];
+  int H = (int)p->hero;
+  if (H != 4 && H != 16) return -1;
+  pa.key = om_path_key(p->seed, pixel, sample);
+  pa.max_scat = p->max_scatterings;
+  pa.p = (real)p->sample_p;
+  start_path(s, pa.key, pixel, H, &L, &ro, &rd);
+  rec_rays = rays; rec_kind = kind; rec_n = 0; rec_cap = cap;
+  trace_path(s, &pa, &L, ro, rd, 0, Li, NULL, NULL);
+  rec_rays = NULL; rec_kind = NULL;
+  return (long)rec_n;
 }
 
 /* The extend and shadow rays tracePath traces for these (pixel, sample)

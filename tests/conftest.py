@@ -54,3 +54,12 @@ def random_rays(rng, n, lo, hi, tmax=3.4e38):
     rays["tmax"] = tmax
     rays["skip"] = la.LAM_NONE
     return rays
+
+
+@pytest.fixture(scope="session")
+def lights_scene(tmp_path_factory):
+    """the mixed scene plus a point, a directional and an ambient light"""
+    import lambda_amd as la
+    from tests.scene_text import all_lights_scene_text
+    d = tmp_path_factory.mktemp("lights")
+    return la.Scene.from_text(all_lights_scene_text(str(d)))

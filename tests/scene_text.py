@@ -44,3 +44,17 @@ def mixed_scene_text(d):
   {"name": "film", "type": "thinfilm", "glossiness": 512, "thickness": 700, "film_ior": {"val": 1.45},
    "base_ior": {"val": 1.6}}]
 }""" % {"d": d}
+
+
+def all_lights_scene_text(d):
+    """the mixed scene with every light type of the reference: its sphere and
+    quad "dir" lights plus a point, a directional and an ambient light
+    (cg.hpp:1560-1622); the directional light enters through the open front"""
+    import json
+    s = json.loads(mixed_scene_text(d))
+    s["lights"] += [
+        {"name": "pt", "type": "point", "pos": [-6.0, -8.0, 16.0], "radiant_intensity": {"val": 900.0}},
+        {"name": "sun", "type": "direction", "dir": [0.25, 1.0, -0.45], "radiant_intensity": {"val": 1.2}},
+        {"name": "sky", "type": "ambient", "radiance": {"val": 0.08}},
+    ]
+    return json.dumps(s)

@@ -298,3 +298,29 @@ def test_adjoint_on_csv_samples_matches_fd(ctx, tmp_path):
     ad = float(gs @ ve)
     assert abs(ad) > 0
     assert abs(fd - ad) <= 1e-2 * abs(ad), (fd, ad)
+
+
+@pytest.mark.gpu
+def test_adjoint_rejects_maps_of_another_scene(ctx, film_scene, box_scene):
+    """parameter maps sized for another scene's materials: a ValueError in
+    Python, LAM_E_INVALID through the C ABI (no read past the maps)"""
+    import ctypes as C
+    p = la.params(spp=1, max_scatterings=2, sample_p=0.8, hero=4)
+    ctx.upload(film_scene)
+    W = np.ones((film_scene.height, film_scene.width, 3), np.float32)
+    other = la.DiffParams(box_scene, albedo=[0])
+    with pytest.raises(ValueError):
+        ctx.adjoint(p, other, W)
+    cs = other.c_struct()
+    grad = np.zeros(other.n_params, np.float32)
+    rc = ctx._lib.lam_adjoint(ctx.handle, C.byref(p), C.byref(cs), W.ctypes.data_as(C.POINTER(C.c_float)),
+                              grad.ctypes.data_as(C.POINTER(C.c_float)))
+    assert rc == la.LAM_E_INVALID
+
+
+@pytest.mark.gpu
+def test_trace_paths_lengths_must_match(ctx, film_scene):
+    ctx.upload(film_scene)
+    p = la.params(spp=1, max_scatterings=2, sample_p=0.8, hero=4)
+    with pytest.raises(ValueError):
+        ctx.trace_paths(p, np.arange(10, dtype=np.uint32), np.zeros(9, np.uint32))

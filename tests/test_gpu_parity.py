@@ -84,7 +84,7 @@ def test_extend_bit_exact(ctx, request, scene_name):
     _assert_hits_equal(orc.closest(sub), orc.closest(sub, brute=True), "oracle bvh vs brute")
 
 
-@pytest.mark.parametrize("scene_name", ["box_scene", "field_scene", "mixed_scene"])
+@pytest.mark.parametrize("scene_name", ["box_scene", "field_scene", "mixed_scene", "lights_scene"])
 def test_shadow_bit_exact(ctx, request, scene_name):
     scene = request.getfixturevalue(scene_name)
     ctx.upload(scene)
@@ -114,11 +114,72 @@ def _check_paths(a, b, rel=1e-4, floor=1e-6):
     assert bad.size == 0, f"{len(bad)} path values out of tolerance: worst {err.max():.3g} at {bad[:3]}"
 
 
+def _degenerate_query(scene, ray, tri):
+    """the intersection of `ray` with triangle `tri` is decided by rounding:
+    the ray grazes the triangle's plane (|cos| < 1e-4), or it crosses the
+    plane within float32 uncertainty of an edge — the crossing point moves
+    by about 2^-24 (|o| + |v| + t) / |cos| when the ray's and the vertices'
+    coordinates are rounded, and there the reference's Plücker test
+    (cg.hpp:1341-1379) and a box test of the same point can disagree"""
+    v = scene.tri_v()[tri].astype(np.float64)
+    o = np.asarray(ray["o"], np.float64)
+    d = np.asarray(ray["d"], np.float64)
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    cos = abs(d @ n) / (np.linalg.norm(d) * np.linalg.norm(n))
+    if cos < 1e-4:
+        return True
+    t = ((v[0] - o) @ n) / (d @ n)
+    p = o + t * d
+    tol = 8 * 2.0 ** -24 * (np.abs(o).max() + np.abs(v).max() + abs(t) * np.abs(d).max()) / cos
+    dist = []
+    for k in range(3):
+        a, b = v[k], v[(k + 1) % 3]
+        e = b - a
+        dist.append(np.linalg.norm(np.cross(e, p - a)) / np.linalg.norm(e))
+    return min(dist) <= tol
+
+
+def _explain_divergence(ctx, scene, orc, p, pixel, sample):
+    """for a path whose GPU radiance differs from the oracle's: replay the
+    oracle's queries of that path (in order) through the GPU's extend and
+    occlusion kernels, and return (index, kind) of the first query they
+    answer differently, or None.  Every query before it is answered the
+    same, so both tracers followed the same path up to there."""
+    rays, kind = orc.path_rays(p, pixel, sample)
+    ho, hg = orc.closest(rays), ctx.extend(rays)
+    oo, og = orc.occluded(rays), ctx.occluded(rays)
+    for i in range(len(rays)):
+        if kind[i] == 0 and _bits(ho[i:i + 1]).tobytes() != _bits(hg[i:i + 1]).tobytes():
+            tri = ho["tri"][i] if ho["tri"][i] != la.LAM_NONE else hg["tri"][i]
+            return i, 0, rays[i], tri
+        if kind[i] == 1 and oo[i] != og[i]:
+            h = orc.closest(rays[i:i + 1]) if oo[i] else ctx.extend(rays[i:i + 1])
+            return i, 1, rays[i], h["tri"][0]
+    return None
+
+
+def _check_paths_explained(ctx, scene, orc, p, pixel, sample, g, o, max_frac=2e-5):
+    """_check_paths, except that a path may differ when the first query on
+    which the GPU and the oracle disagree is degenerate (a grazing ray or a
+    crossing on a triangle edge: the answer depends on which boxes a BVH
+    culls, not on the kernels' arithmetic); at most max_frac of the paths"""
+    scale = max(float(np.abs(o).mean()), 1e-30)
+    err = np.abs(g.astype(np.float64) - o) / (1e-4 * np.abs(o) + 1e-6 * scale)
+    bad = np.nonzero(err.max(axis=1) > 1.0)[0]
+    assert bad.size <= max(1, max_frac * len(pixel)), f"{bad.size} of {len(pixel)} paths differ"
+    for b in bad:
+        e = _explain_divergence(ctx, scene, orc, p, pixel[b], sample[b])
+        assert e is not None, f"path ({pixel[b]}, {sample[b]}) differs with every query answered the same"
+        i, kind, ray, tri = e
+        assert tri != la.LAM_NONE and _degenerate_query(scene, ray, tri), (pixel[b], sample[b], i, kind, ray, tri)
+    return bad.size
+
+
 @pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_TAIL, la.LAM_FLAG_MEGAKERNEL],
                          ids=["default", "wavefront", "megakernel"])
 @pytest.mark.parametrize("scene_name,hero,p,bounces", [
     ("box_scene", 4, 0.8, 8), ("field_scene", 4, 0.8, 12), ("mixed_scene", 16, 0.8, 8),
-    ("mixed_scene", 4, 0.5, 4)])
+    ("mixed_scene", 4, 0.5, 4), ("lights_scene", 16, 0.8, 8), ("lights_scene", 4, 0.6, 5)])
 def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces, flags):
     scene = request.getfixturevalue(scene_name)
     ctx.upload(scene)
@@ -137,7 +198,7 @@ def test_trace_paths_parity(ctx, request, scene_name, hero, p, bounces, flags):
 @pytest.mark.parametrize("flags", [0, la.LAM_FLAG_NO_GRAPH, la.LAM_FLAG_PROFILE, la.LAM_FLAG_MEGAKERNEL,
                                    la.LAM_FLAG_NO_TAIL],
                          ids=["graph", "nograph", "profile", "megakernel", "notail"])
-@pytest.mark.parametrize("scene_name,hero", [("box_scene", 4), ("mixed_scene", 16)])
+@pytest.mark.parametrize("scene_name,hero", [("box_scene", 4), ("mixed_scene", 16), ("lights_scene", 16)])
 def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero, flags):
     scene = request.getfixturevalue(scene_name)
     ctx.upload(scene)
@@ -378,3 +439,33 @@ def test_render_counts_rays(ctx, request, scene_name):
         ctx.render(la.params(spp=2, max_scatterings=0, sample_p=1.0, hero=4, seed=3))
         t = ctx.timing()
         assert (t.extend_rays, t.shadow_rays) == (4 * npix, 2 * npix)
+
+
+def test_every_light_type_contributes(ctx, lights_scene):
+    """point, directional and ambient lights (cg.hpp:1560-1622) on the HIP
+    path: each one's light estimate alone (a scene variant with only that
+    light, no BSDF continuation) matches the oracle per path, shadowed
+    delta lights give exact zeros where the oracle's shadow query is
+    blocked, and every type lights some paths"""
+    import json
+    from tests.scene_text import all_lights_scene_text
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        base = json.loads(all_lights_scene_text(d))
+        for name in ("pt", "sun", "sky"):
+            s = dict(base)
+            s["lights"] = [l for l in base["lights"] if l["name"] == name]
+            sc = la.Scene.from_text(json.dumps(s))
+            ctx.upload(sc)
+            orc = ob.Oracle(sc)
+            npix = sc.width * sc.height
+            pix = np.arange(npix, dtype=np.uint32)
+            smp = np.zeros(npix, np.uint32)
+            pr = la.params(spp=1, max_scatterings=0, sample_p=0.0, hero=16, seed=3)
+            g, o = ctx.trace_paths(pr, pix, smp), orc.trace_paths(pr, pix, smp)
+            _check_paths(g, o)
+            lit = (o > 0).any(axis=1)
+            assert 0.05 < lit.mean(), name
+            assert np.array_equal((g > 0).any(axis=1), lit), name
+            if name != "sky":  # delta lights are shadowed somewhere in the closed box
+                assert lit.mean() < 0.98, name
