@@ -14,15 +14,29 @@ Also reported:
   roofline      extend (closest-hit) kernel on a 2^22-ray batch (camera rays
                 + first-bounce rays) of the same scene: algorithmic bytes per
                 ray = 32 (ray) + 16 (hit) + 128 per BVH8 node fetched + 48
-                per triangle tested, counts from the kernel's counting
-                variant; time from HIP events around the kernel on the
-                library's stream.  Peak HBM 8.0 TB/s.
+                per triangle tested, counts from the host traversal of the
+                same 128-byte BVH8 records (lamh_trace_counts), checked equal
+                to the kernel's counting variant; time from HIP events around
+                the kernel on the library's stream.  Peak HBM 8.0 TB/s.
+                "achieved" is algorithmic bytes over time; the scene (75 MB)
+                stays in L2/MALL, so the HBM bytes the counters measure
+                (traffic, hbm_measured_gbs: from the committed rocprofv3
+                --pmc profile of the same batch) are fewer.
+  shadow_roofline  the any-hit kernel on shadow rays from the batch's hit
+                points to uniform points on the quad lights: 32 + 4 +
+                128 per node + 48 per triangle.
+  render_kernels_roofline  the render's own extend and shadow kernels in a
+                profiled render (one lane, HIP events per launch): their
+                node and triangle counts (LAM_FLAG_PROFILE_FETCH) over their
+                time in a second render of the same samples.
   cpu_baseline  the CPU oracle (C restatement of the reference render loop,
                 OpenMP over pixels) on a bounded row sample of the same
                 frame and settings, on this host's cores.
   adjoint       the differentiation workload: box room with 16 Lambertian
                 albedos and 64 thin-film spheres at 512x512; parameters 64
-                thicknesses + 16 albedos x 95 grid coefficients; target at
+                thicknesses + 16 albedos x 95 grid coefficients (and the same
+                gradient on the albedo CSVs' 81 samples each: 64 + 16 x 81 =
+                1,360 values); target at
                 parameters x (1 +- U[0,0.15]); L2 image loss; forward 64 spp
                 + path-replay adjoint 64 spp, 16 bounces, p=0.8.  grads/s =
                 complete gradients per second at that sample count.
@@ -87,7 +101,22 @@ def camera_rays(desc):
     return r
 
 
-def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
+def _traffic(name, n):
+    """HBM bytes per launch of a ray-batch kernel from the committed
+    rocprofv3 --pmc runs (FETCH_SIZE and WRITE_SIZE scaled by a copy of
+    known size), or None when no profile of this batch size is committed"""
+    rec_path = os.path.join(ROOT, "profiles", f"{name}.traffic.json")
+    if os.path.exists(rec_path):
+        rec = json.load(open(rec_path))
+        if rec.get("rays_per_launch") == n:
+            return round(rec["traffic_bytes"]), f"profiles/{name}.traffic.json (rocprofv3 --pmc)"
+    return None, None
+
+
+def extend_batch(scene, ctx, n_target=1 << 22):
+    """2^22 rays: the camera rays of the frame and first-bounce rays from
+    their hit points in uniformly random directions (seed 0x5EED)"""
+    desc = scene.desc
     cam = camera_rays(desc)
     hits = ctx.extend(cam)
     m = hits["obj"] != la.LAM_NONE
@@ -101,32 +130,120 @@ def extend_roofline(ctx, desc, n_target=1 << 22, reps=10):
     b["d"] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
     b["tmax"] = np.float32(3.4e38)
     b["skip"] = la.LAM_NONE
-    rays = np.concatenate([cam, b])
+    return np.concatenate([cam, b])
+
+
+def shadow_batch(scene, rays, hits, rng):
+    """shadow rays from the hit points of `rays` to uniform points on a
+    uniformly chosen quad light (tmax = distance, the light's object
+    skipped), as the light samples of tracePath make them"""
+    desc = scene.desc
+    m = hits["obj"] != la.LAM_NONE
+    p = rays["o"][m] + rays["d"][m] * hits["t"][m][:, None]
+    lights = [l for l in scene.lights() if l.type == la.LIGHT_QUADDIR]
+    pick = rng.integers(0, len(lights), len(p))
+    x = np.zeros_like(p)
+    skip = np.zeros(len(p), np.uint32)
+    for k, l in enumerate(lights):
+        sel = pick == k
+        M = np.array(desc.objects[l.object].T.M, np.float32).reshape(3, 4)
+        sx, sy = (2 * rng.random((2, sel.sum())) - 1).astype(np.float32)
+        x[sel] = M[:, 3] + M[:, 0] * sx[:, None] + M[:, 1] * sy[:, None]
+        skip[sel] = l.object
+    r = x - p
+    dist = np.linalg.norm(r, axis=1)
+    out = np.zeros(len(p), dtype=la.RAY_DTYPE)
+    out["o"] = p
+    out["d"] = r / dist[:, None]
+    out["tmax"] = dist
+    out["skip"] = skip
+    return out
+
+
+def _batch_roofline(ctx, scene, rays, any_hit, reps, name):
+    """one ray-batch kernel: per-ray nodes and triangles from the host
+    traversal of the product's BVH8 (lamh_trace_counts, independent of the
+    kernel), checked equal to the kernel's own counting variant; the mean
+    launch time from HIP events around `reps` launches on the library's
+    stream"""
     n = len(rays)
-    b_rays, b_hits, b_cnt = la.DeviceBuffer.from_array(rays), la.DeviceBuffer(n * 16), la.DeviceBuffer(n * 8)
-    d_rays, d_hits, d_cnt = b_rays.ptr, b_hits.ptr, b_cnt.ptr
-    ctx.extend_dev(d_rays, n, d_hits, d_cnt)
-    cnt = b_cnt.download(np.uint32, 2 * n)
-    nodes, tris = cnt[0::2].astype(np.float64), cnt[1::2].astype(np.float64)
-    bytes_per_launch = float(np.sum(32 + 16 + 128 * nodes + 48 * tris))
-    ctx.extend_dev(d_rays, n, d_hits)  # warm
-    avg_ms = ctx.extend_dev_timed(d_rays, n, d_hits, reps)
-    for b in (b_rays, b_hits, b_cnt):
+    host_out, hc = la.host_trace(scene, rays, any_hit=any_hit)
+    b_rays, b_out, b_cnt = la.DeviceBuffer.from_array(rays), la.DeviceBuffer(n * 16), la.DeviceBuffer(n * 8)
+    if any_hit:
+        ctx.occluded_dev(b_rays.ptr, n, b_out.ptr, b_cnt.ptr)
+        dev_out = b_out.download(np.uint32, n)
+        same = bool(np.array_equal(dev_out, host_out))
+    else:
+        ctx.extend_dev(b_rays.ptr, n, b_out.ptr, b_cnt.ptr)
+        dev_out = b_out.download(la.HIT_DTYPE, n)
+        same = bool(np.array_equal(dev_out.view(np.uint32), host_out.view(np.uint32)))
+    kc = b_cnt.download(np.uint32, 2 * n).reshape(n, 2)
+    nodes, tris = hc[:, 0].astype(np.float64), hc[:, 1].astype(np.float64)
+    per_ray_io = (32 + 4) if any_hit else (32 + 16)
+    bytes_per_launch = float(np.sum(per_ray_io + 128 * nodes + 48 * tris))
+    if any_hit:
+        ctx.occluded_dev(b_rays.ptr, n, b_out.ptr)  # warm
+        avg_ms = ctx.occluded_dev_timed(b_rays.ptr, n, b_out.ptr, reps)
+    else:
+        ctx.extend_dev(b_rays.ptr, n, b_out.ptr)  # warm
+        avg_ms = ctx.extend_dev_timed(b_rays.ptr, n, b_out.ptr, reps)
+    for b in (b_rays, b_out, b_cnt):
         b.free()
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    rec_path = os.path.join(ROOT, "profiles", "extend_icosphere_field_4Mrays.traffic.json")
-    if os.path.exists(rec_path):  # HBM bytes per launch from the committed rocprofv3 --pmc runs
-        rec = json.load(open(rec_path))
-        if rec.get("rays_per_launch") == n:
-            traffic = round(rec["traffic_bytes"])
-    return {
+    traffic, src = _traffic(name, n)
+    out = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": "k_trace_rays<closest>", "rays_per_launch": n, "avg_ms": round(avg_ms, 4),
-        "rays_per_s": round(n / (avg_ms * 1e-3), 1), "nodes_per_ray": round(nodes.mean(), 3),
-        "tris_per_ray": round(tris.mean(), 3), "bytes_per_launch": bytes_per_launch,
+        "achieved_is": "algorithmic bytes (counts of the host traversal) per mean launch time",
+        "kernel": "k_trace_rays<any>" if any_hit else "k_trace_rays<closest>", "rays_per_launch": n,
+        "avg_ms": round(avg_ms, 4), "rays_per_s": round(n / (avg_ms * 1e-3), 1),
+        "nodes_per_ray": round(nodes.mean(), 3), "tris_per_ray": round(tris.mean(), 3),
+        "bytes_per_launch": bytes_per_launch, "counts_from": "host traversal of the BVH8 records",
+        "kernel_counts_equal": bool(np.array_equal(kc, hc)), "results_equal_host": same,
     }
+    if traffic:
+        out["traffic_source"] = src
+        out["hbm_measured_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+    return out
+
+
+def extend_roofline(ctx, scene, reps=10):
+    rays = extend_batch(scene, ctx)
+    return _batch_roofline(ctx, scene, rays, False, reps, "extend_icosphere_field_4Mrays")
+
+
+def shadow_roofline(ctx, scene, reps=10):
+    rays = extend_batch(scene, ctx)
+    hits = ctx.extend(rays)
+    sh = shadow_batch(scene, rays, hits, np.random.default_rng(0x5EED + 1))
+    return _batch_roofline(ctx, scene, sh, True, reps, "shadow_icosphere_field_batch")
+
+
+def render_kernel_roofline(ctx, bounces, p, hero, spp=4):
+    """the render's own extend and shadow kernels: one render with
+    LAM_FLAG_PROFILE | LAM_FLAG_PROFILE_FETCH (one lane, every traversal
+    launch between HIP events, the kernels counting nodes and triangles)
+    gives the algorithmic bytes; a second render of the same samples with
+    LAM_FLAG_PROFILE alone gives the kernels' time without the counting"""
+    base = dict(spp=spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=11)
+    ctx.reset_timing()
+    ctx.render(la.params(flags=la.LAM_FLAG_PROFILE | la.LAM_FLAG_PROFILE_FETCH, **base))
+    c = ctx.timing()
+    ctx.reset_timing()
+    ctx.render(la.params(flags=la.LAM_FLAG_PROFILE, **base))
+    t = ctx.timing()
+    out = {}
+    for name, ms, nodes, tris, rays, io in (
+            ("extend", t.extend_ms, c.extend_nodes, c.extend_tris, c.extend_traced, 32 + 16),
+            ("shadow", t.shadow_ms, c.shadow_nodes, c.shadow_tris, c.shadow_traced, 32 + 4)):
+        b = io * rays + 128 * nodes + 48 * tris
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms": round(ms, 3), "rays": int(rays),
+                     "nodes_per_ray": round(nodes / max(rays, 1), 3), "tris_per_ray": round(tris / max(rays, 1), 3),
+                     "rays_per_s": round(rays / (ms * 1e-3), 1) if ms > 0 else 0.0}
+    out["workload"] = f"{spp} profiled passes of the render (one lane, no graph)"
+    return out
 
 
 def _cpu_info():
@@ -463,7 +580,7 @@ def main():
         f"{st.stack_bound}, setup {time.perf_counter() - t0:.1f}s")
     w, h = scene.width, scene.height
     if args.extend_only:
-        rl = extend_roofline(ctx, scene.desc, reps=3)
+        rl = extend_roofline(ctx, scene, reps=3)
         nbytes = 1 << 30
         a, b = la.DeviceBuffer(nbytes), la.DeviceBuffer(nbytes)
         for _ in range(3):  # device to device: the counters' scale reference
@@ -499,8 +616,12 @@ def main():
                         "shadow_rays_per_sample": round(tm.shadow_rays / samples, 4)},
     }
     if "roofline" in legs:
-        out["roofline"] = extend_roofline(ctx, scene.desc)
+        out["roofline"] = extend_roofline(ctx, scene)
         log(f"extend: {out['roofline']}")
+        out["shadow_roofline"] = shadow_roofline(ctx, scene)
+        log(f"shadow: {out['shadow_roofline']}")
+        out["render_kernels_roofline"] = render_kernel_roofline(ctx, bounces, p, hero)
+        log(f"render kernels: {out['render_kernels_roofline']}")
     ctx.close()
     if "adjoint" in legs:
         out["adjoint"] = adjoint_bench()

@@ -168,6 +168,8 @@ typedef struct {
 #define LAM_FLAG_NO_TAIL 0x8u    /* keep short queues in the wavefront (no finish kernel) */
 #define LAM_FLAG_NO_PIPE 0x10u   /* trace passes one after another on one stream (no overlap) */
 #define LAM_FLAG_ACCUMULATE 0x20u /* add the passes to the film of the previous render (progressive) */
+#define LAM_FLAG_PROFILE_FETCH 0x40u /* with LAM_FLAG_PROFILE: the extend and shadow kernels count the
+                                        BVH nodes and triangles they fetch (lam_timing) */
 
 /* lam_scene_upload_ex flags */
 #define LAM_UPLOAD_GPU_BVH 0x1u /* build the BVH8 on the device (linear BVH; faster build, slower trace) */
@@ -205,6 +207,11 @@ typedef struct {
   uint64_t guide_dropped;  /* continuations dropped: weight not finite */
   uint64_t guide_zero_pdf; /* guide samples whose mixture density was zero */
   double guide_loss_first, guide_loss_last; /* training loss, first and last step */
+  /* LAM_FLAG_PROFILE | LAM_FLAG_PROFILE_FETCH renders: interior nodes
+   * fetched, triangles tested and rays traced by the extend and the shadow
+   * kernels (the finish kernel's rays are not among them) */
+  uint64_t extend_nodes, extend_tris, extend_traced;
+  uint64_t shadow_nodes, shadow_tris, shadow_traced;
 } lam_timing;
 
 /* ---- context ---------------------------------------------------------- */
@@ -328,6 +335,14 @@ int lam_occluded(lam_ctx *ctx, const lam_ray *rays, size_t n,
  * triangles tested per ray. */
 int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
                    lam_hit *d_hits, uint32_t *d_counts);
+/* Device-pointer variants of lam_occluded (measurement: the shadow
+ * kernel's roofline); `counts` (optional, device, 2*n uint32) receives
+ * interior nodes fetched and triangles tested per ray, and the timed form
+ * runs `reps` (<= 4096) back-to-back launches between one pair of events. */
+int lam_occluded_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
+                     uint32_t *d_occluded, uint32_t *d_counts);
+int lam_occluded_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
+                           uint32_t *d_occluded, uint32_t reps, double *avg_ms);
 /* lam_extend_dev for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  The caller passes the

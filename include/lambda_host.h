@@ -58,6 +58,17 @@ int lamh_image_name(const char *scene_path, uint32_t spp, uint32_t bounces, floa
  * in *conservative when every decoded box contains its exact box). */
 int lamh_bvh8_check(const float *tri_v, uint32_t n_tris, lam_scene_stats *stats, int *conservative);
 
+/* The extend / shadow kernels' traversal on the host: builds the BVH8 as
+ * lam_scene_upload does and traverses its 128-byte records with the
+ * kernels' float operations (closest hit, or any hit when `any`), one ray
+ * per query as intersect::scene (lib/numli/cg.hpp:1469-1489).  hits (closest,
+ * may be NULL), occluded (may be NULL) as lam_extend / lam_occluded;
+ * counts (may be NULL): 2 per ray, interior nodes fetched and triangles
+ * tested — the independent count behind the extend roofline's bytes.
+ * LAM_E_BVH if a ray needs more than 128 stack entries. */
+int lamh_trace_counts(const lam_scene_desc *desc, const lam_ray *rays, size_t n, int any, lam_hit *hits,
+                      uint32_t *occluded, uint32_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

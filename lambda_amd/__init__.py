@@ -43,6 +43,7 @@ LAM_FLAG_PROFILE = 0x4
 LAM_FLAG_NO_TAIL = 0x8
 LAM_FLAG_NO_PIPE = 0x10
 LAM_FLAG_ACCUMULATE = 0x20
+LAM_FLAG_PROFILE_FETCH = 0x40
 LAM_UPLOAD_GPU_BVH = 0x1
 
 OBJ_SPHERE, OBJ_PLANE, OBJ_TRIMESH = 1, 2, 3
@@ -155,7 +156,9 @@ class Timing(C.Structure):
                 ("shadow_rays", C.c_uint64), ("render_ms", C.c_double),
                 ("guide_records", C.c_uint64), ("guide_train_ms", C.c_double),
                 ("guide_dropped", C.c_uint64), ("guide_zero_pdf", C.c_uint64),
-                ("guide_loss_first", C.c_double), ("guide_loss_last", C.c_double)]
+                ("guide_loss_first", C.c_double), ("guide_loss_last", C.c_double),
+                ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64), ("extend_traced", C.c_uint64),
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("shadow_traced", C.c_uint64)]
 
 
 def params(spp=4, max_scatterings=4, sample_p=0.5, hero=16, seed=0, spp_offset=0, flags=0):
@@ -201,6 +204,8 @@ def lib():
         "lam_extend_dev": (C.c_int, [vp, vp, sz, vp, vp]),
         "lam_extend_dev_timed": (C.c_int, [vp, vp, sz, vp, C.c_uint32, C.POINTER(C.c_double)]),
         "lam_extend_dev_noalloc": (C.c_int, [vp, vp, sz, vp]),
+        "lam_occluded_dev": (C.c_int, [vp, vp, sz, vp, vp]),
+        "lam_occluded_dev_timed": (C.c_int, [vp, vp, sz, vp, C.c_uint32, C.POINTER(C.c_double)]),
         "lam_get_timing": (C.c_int, [vp, C.POINTER(Timing)]),
         "lam_reset_timing": (C.c_int, [vp]),
         "lam_get_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
@@ -225,6 +230,7 @@ def lib():
                                       C.c_char_p, sz]),
         "lamh_bvh8_check": (C.c_int, [fp, C.c_uint32, C.POINTER(SceneStats), C.POINTER(C.c_int)]),
         "lamh_spectrum_grad_to_samples": (C.c_int, [C.POINTER(Grid), fp, sz, C.c_float, fp, fp]),
+        "lamh_trace_counts": (C.c_int, [vp, vp, sz, C.c_int, vp, u32p, u32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -639,6 +645,14 @@ class Context:
     def extend_dev(self, rays_ptr, n, hits_ptr, counts_ptr=None):
         self._check(self._lib.lam_extend_dev(self._h, rays_ptr, n, hits_ptr, counts_ptr))
 
+    def occluded_dev(self, rays_ptr, n, occl_ptr, counts_ptr=None):
+        self._check(self._lib.lam_occluded_dev(self._h, rays_ptr, n, occl_ptr, counts_ptr))
+
+    def occluded_dev_timed(self, rays_ptr, n, occl_ptr, reps):
+        ms = C.c_double()
+        self._check(self._lib.lam_occluded_dev_timed(self._h, rays_ptr, n, occl_ptr, int(reps), C.byref(ms)))
+        return ms.value
+
     def extend_dev_noalloc(self, rays_ptr, n, hits_ptr):
         """lam_extend_dev without device allocation (static device scratch)"""
         self._check(self._lib.lam_extend_dev_noalloc(self._h, rays_ptr, n, hits_ptr))
@@ -806,6 +820,26 @@ class Context:
         out = np.zeros_like(a)
         self._check(self._lib.lam_debug_math(self._h, int(fn), _fptr(a), _fptr(b), len(a), _fptr(out)))
         return out
+
+
+def host_trace(scene, rays, any_hit=False):
+    """lamh_trace_counts: the kernels' BVH8 traversal on the host.  Returns
+    (hits or occluded flags, counts n x 2: nodes fetched, triangles tested)."""
+    L = lib()
+    rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+    n = len(rays)
+    counts = np.zeros((n, 2), np.uint32)
+    if any_hit:
+        out = np.zeros(n, np.uint32)
+        rc = L.lamh_trace_counts(C.cast(scene.desc_ptr, C.c_void_p), rays.ctypes.data, n, 1, None, _u32ptr(out),
+                                 _u32ptr(counts))
+    else:
+        out = np.zeros(n, dtype=HIT_DTYPE)
+        rc = L.lamh_trace_counts(C.cast(scene.desc_ptr, C.c_void_p), rays.ctypes.data, n, 0, out.ctypes.data, None,
+                                 _u32ptr(counts))
+    if rc != LAM_OK:
+        raise LambdaError(rc, "lamh_trace_counts failed")
+    return out, counts
 
 
 def tone_map(rgb, y_mid=0.12):

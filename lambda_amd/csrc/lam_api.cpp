@@ -100,6 +100,7 @@ struct lam_ctx {
   uint64_t film_spp = 0;
   size_t img_npix = 0;       // pixels of the last rendered linear-RGB image (scratch[4])
   void *tone_buf = nullptr;  // tone map: 1024 per-block log sums (double) + the exposure
+  unsigned long long *fetch_counts = nullptr;  // LAM_FLAG_PROFILE_FETCH: 6 counters (WaveProfile::counts)
   lam::WaveLaunch wl{0, 0, 131072u, 12u, 0, 0};
   int cus = 0;
   // this device's copy of `scratch` (lam_extend_dev_noalloc), set by lam_create
@@ -629,6 +630,11 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
     prof.ext1 = prof.ext0 + iters;
     prof.shd0 = prof.ext1 + iters;
     prof.shd1 = prof.shd0 + iters;
+    if (p->flags & LAM_FLAG_PROFILE_FETCH) {
+      if (!ctx->fetch_counts) HIPC(hipMalloc(reinterpret_cast<void **>(&ctx->fetch_counts), 64));
+      HIPC(lam::launch_zero(ctx->fetch_counts, 64, st[0]));
+      prof.counts = ctx->fetch_counts;
+    }
   }
   for (size_t off = 0; off < npix; off += cap) {
     const uint32_t n = uint32_t(std::min(cap, npix - off));
@@ -750,6 +756,16 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
     HIPC(e);
     HIPC(es);
   }
+  if (prof.counts) {
+    unsigned long long c[6];
+    HIPC(hipMemcpy(c, prof.counts, sizeof c, hipMemcpyDeviceToHost));
+    ctx->timing.extend_nodes += c[0];
+    ctx->timing.extend_tris += c[1];
+    ctx->timing.extend_traced += c[2];
+    ctx->timing.shadow_nodes += c[3];
+    ctx->timing.shadow_tris += c[4];
+    ctx->timing.shadow_traced += c[5];
+  }
   return LAM_OK;
 }
 
@@ -797,6 +813,7 @@ void lam_destroy(lam_ctx *ctx) {
   free_guide(ctx);
   if (ctx->trace_ctr) (void)hipFree(ctx->trace_ctr);
   if (ctx->tone_buf) (void)hipFree(ctx->tone_buf);
+  if (ctx->fetch_counts) (void)hipFree(ctx->fetch_counts);
   for (lam_ctx::Lane &ln : ctx->lane) {
     if (ln.ctrl) (void)hipFree(ln.ctrl);
     if (ln.rays) (void)hipFree(ln.rays);
@@ -1046,6 +1063,54 @@ int lam_extend_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, lam_hit *d_hit
   ctx->timing.extend_ms += event_ms(ctx);
   ctx->timing.extend_launches++;
   ctx->timing.extend_rays += n;
+  return check_err_flag(ctx);
+}
+
+int lam_occluded_dev(lam_ctx *ctx, const lam_ray *d_rays, size_t n, uint32_t *d_occluded, uint32_t *d_counts) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n && (!d_rays || !d_occluded)) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (n == 0) return LAM_OK;
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(lam::launch_zero(ctx->trace_ctr, 16, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, nullptr, d_occluded, d_counts, ctx->d_err, ctx->trace_ctr,
+                              ctx->lane[0].spill, ctx->batch_blocks, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->timing.shadow_ms += event_ms(ctx);
+  ctx->timing.shadow_launches++;
+  ctx->timing.shadow_rays += n;
+  return check_err_flag(ctx);
+}
+
+int lam_occluded_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n, uint32_t *d_occluded, uint32_t reps,
+                           double *avg_ms) {
+  if (!ctx) return LAM_E_INVALID;
+  if (!d_rays || !d_occluded || !avg_ms || reps == 0 || n == 0) return fail(ctx, LAM_E_INVALID, "bad argument");
+  if (!ctx->has_scene) return fail(ctx, LAM_E_NOSCENE, "no scene uploaded");
+  if (n > (size_t(1) << 31)) return fail(ctx, LAM_E_LIMIT, "ray batch too large");
+  if (reps > kTraceCounters) return fail(ctx, LAM_E_LIMIT, "at most 4096 launches per call");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = ensure_launch(ctx)) || (rc = ensure_spill(ctx, 0))) return rc;
+  if ((rc = clear_err_flag(ctx))) return rc;
+  HIPC(lam::launch_zero(ctx->trace_ctr, size_t(kTraceCounters) * 4, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
+  for (uint32_t r = 0; r < reps; r++)
+    HIPC(lam::launch_trace_rays(ctx->sc, d_rays, n, nullptr, d_occluded, nullptr, ctx->d_err, ctx->trace_ctr + r,
+                                ctx->lane[0].spill, ctx->batch_blocks, ctx->stream));
+  HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  const double ms = event_ms(ctx);
+  *avg_ms = ms / reps;
+  ctx->timing.shadow_ms += ms;
+  ctx->timing.shadow_launches += reps;
+  ctx->timing.shadow_rays += n * reps;
   return check_err_flag(ctx);
 }
 

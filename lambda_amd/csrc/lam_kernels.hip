@@ -234,7 +234,10 @@ hipError_t launch_trace_rays(const DevScene &sc, const lam_ray *rays, size_t n, 
                              hipStream_t st) {
   if (n == 0) return hipSuccess;
   const dim3 g(std::min(blocks64(n), blocks)), b(64);
-  if (occl) {
+  if (occl && counts) {
+    hipLaunchKernelGGL((k_trace_rays<true, true>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err,
+                       fetch, spill);
+  } else if (occl) {
     hipLaunchKernelGGL((k_trace_rays<true, false>), g, b, 0, st, sc, rays, uint32_t(n), hits, occl, counts, err,
                        fetch, spill);
   } else if (counts) {

@@ -78,6 +78,9 @@ inline uint32_t wave_iterations(uint32_t max_scat, const WaveLaunch &wl) {
 struct WaveProfile {
   hipEvent_t ev[4];
   hipEvent_t *fin0, *ext0, *ext1, *shd0, *shd1;
+  // (nullable) fetch counts of the extend and shadow kernels: extend nodes,
+  // triangles, rays, then shadow nodes, triangles, rays
+  unsigned long long *counts;
 };
 constexpr uint32_t kCtrlWords = 16;
 constexpr uint32_t kCtrlOverflow = 15;

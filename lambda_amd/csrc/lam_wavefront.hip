@@ -129,7 +129,7 @@ __global__ void k_pass_next(uint32_t *__restrict__ ctrl, uint32_t stride) {
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
                                                uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ spill,
-                                               uint32_t *__restrict__ counts) {
+                                               unsigned long long *__restrict__ counts) {
   __shared__ uint32_t stack[kLdsStack * 64];
   if (ctrl[CTRL_TAIL]) return;
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
@@ -157,9 +157,9 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
   const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
   trace_queue<false, COUNT>(sc, n, ctrl + CTRL_FETCH_EXT, stk, load, done);
   if (COUNT) {
-    atomicAdd(counts + 0, c.nodes);
-    atomicAdd(counts + 1, c.tris);
-    atomicAdd(counts + 2, nrays);
+    atomicAdd(counts + 0, (unsigned long long)c.nodes);
+    atomicAdd(counts + 1, (unsigned long long)c.tris);
+    atomicAdd(counts + 2, (unsigned long long)nrays);
   }
 }
 
@@ -714,7 +714,8 @@ hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipS
 // ---- shadow: persistent any-hit, then the pending NEE term -----------------------------
 template <int H, bool COUNT, bool REC = false>
 __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32_t *__restrict__ ctrl,
-                                               uint32_t *__restrict__ counts, float4 *__restrict__ rec = nullptr,
+                                               unsigned long long *__restrict__ counts,
+                                               float4 *__restrict__ rec = nullptr,
                                                const uint32_t *__restrict__ rec_of_slot = nullptr,
                                                uint32_t rec_cap = 0) {
   __shared__ uint32_t stack[kLdsStack * 64];
@@ -751,9 +752,9 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
   const SplitStack stk = SplitStack::of(stack, ws.spill, blockIdx.x, gridDim.x);
   trace_queue<true, COUNT>(sc, n, ctrl + CTRL_FETCH_SHD, stk, load, done);
   if (COUNT) {
-    atomicAdd(counts + 3, c.nodes);
-    atomicAdd(counts + 4, c.tris);
-    atomicAdd(counts + 5, nrays);
+    atomicAdd(counts + 3, (unsigned long long)c.nodes);
+    atomicAdd(counts + 4, (unsigned long long)c.tris);
+    atomicAdd(counts + 5, (unsigned long long)nrays);
   }
 }
 
@@ -881,8 +882,12 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
       hipLaunchKernelGGL((k_finish<H>), dim3(wl.finish_blocks), dim3(64), 0, st, sc, pa, ids, ws, s, q, ctrl);
     if (prof) {
       (void)hipEventRecord(prof->ext0[s], st);
-      hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
-                         reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
+      if (prof->counts)
+        hipLaunchKernelGGL((k_extend<true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
+                           reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, prof->counts);
+      else
+        hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
+                           reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
       (void)hipEventRecord(prof->ext1[s], st);
     } else {
       hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
@@ -910,6 +915,9 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     if (gv && train)
       hipLaunchKernelGGL((k_shadow<H, false, true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
                          gv->rec, gv->rec_of_slot, gv->rec_cap);
+    else if (prof && prof->counts)
+      hipLaunchKernelGGL((k_shadow<H, true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, prof->counts,
+                         nullptr, nullptr, 0u);
     else
       hipLaunchKernelGGL((k_shadow<H, false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
                          nullptr, nullptr, 0u);

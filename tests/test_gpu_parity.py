@@ -469,3 +469,44 @@ def test_every_light_type_contributes(ctx, lights_scene):
             assert np.array_equal((g > 0).any(axis=1), lit), name
             if name != "sky":  # delta lights are shadowed somewhere in the closed box
                 assert lit.mean() < 0.98, name
+
+
+@pytest.mark.parametrize("any_hit", [False, True], ids=["closest", "any"])
+def test_kernel_counts_equal_host_traversal(ctx, field_scene, any_hit):
+    """the host traversal of the BVH8 records (lamh_trace_counts, the count
+    behind the roofline's bytes) gives the kernels' results and their
+    per-ray node and triangle counts exactly"""
+    ctx.upload(field_scene)
+    rng = np.random.default_rng(41)
+    v = field_scene.tri_v().reshape(-1, 3)
+    rays = random_rays(rng, 100000, v.min(axis=0), v.max(axis=0))
+    if any_hit:
+        rays["tmax"] = rng.uniform(0, 20, len(rays)).astype(np.float32)
+    host, hc = la.host_trace(field_scene, rays, any_hit=any_hit)
+    n = len(rays)
+    d_rays, d_out, d_cnt = la.DeviceBuffer.from_array(rays), la.DeviceBuffer(n * 16), la.DeviceBuffer(n * 8)
+    if any_hit:
+        ctx.occluded_dev(d_rays.ptr, n, d_out.ptr, d_cnt.ptr)
+        got = d_out.download(np.uint32, n)
+        assert np.array_equal(got, host) and np.array_equal(got, ctx.occluded(rays))
+    else:
+        ctx.extend_dev(d_rays.ptr, n, d_out.ptr, d_cnt.ptr)
+        got = d_out.download(la.HIT_DTYPE, n)
+        assert np.array_equal(_bits(got), _bits(host))
+    kc = d_cnt.download(np.uint32, 2 * n).reshape(n, 2)
+    assert np.array_equal(kc, hc)
+    assert hc[:, 0].mean() > 1
+
+
+def test_profiled_render_counts_fetches(ctx, box_scene):
+    """LAM_FLAG_PROFILE_FETCH counts the render kernels' nodes and
+    triangles and changes no pixel"""
+    ctx.upload(box_scene)
+    base = dict(spp=3, max_scatterings=8, sample_p=0.8, hero=4, seed=2)
+    a = ctx.render(la.params(**base))
+    ctx.reset_timing()
+    b = ctx.render(la.params(flags=la.LAM_FLAG_PROFILE | la.LAM_FLAG_PROFILE_FETCH, **base))
+    t = ctx.timing()
+    assert np.array_equal(a, b)
+    assert t.extend_traced > 0 and t.shadow_traced > 0
+    assert t.extend_nodes >= t.extend_traced and t.extend_ms > 0 and t.shadow_ms > 0
