@@ -526,10 +526,10 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-guiding", action="store_true")
-    ap.add_argument("--legs", default="render,roofline,adjoint,guiding,cpu",
-                    help="comma list of legs to run (profiling runs pick one); 'stress' (not in the default) "
-                         "adds the 10 M-triangle 4K guided + adjoint run, 'gpu_bvh' (not in the default) the "
-                         "device BVH build of the render scene")
+    ap.add_argument("--legs", default="render,roofline,adjoint,guiding,stress,cpu",
+                    help="comma list of legs to run (profiling runs pick one); 'stress' is the 10 M-triangle "
+                         "4K guided + adjoint run, 'gpu_bvh' (not in the default) the device BVH build of the "
+                         "render scene")
     ap.add_argument("--guide-spp", type=int, default=1024)
     ap.add_argument("--guide-train-passes", type=int, default=32)
     ap.add_argument("--guide-steps-per-pass", type=int, default=8)
