@@ -362,7 +362,7 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
 
 
 def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32, max_vertex=0,
-                  steps_per_pass=8, batch=16384, materials=0):
+                  steps_per_pass=8, batch=16384, materials=0, lobes=0):
     """Learned proposal on the glossy interior: two independent plain runs
     and two guided runs of `spp` samples per pixel (the guided runs train
     the network online on their first `train_passes` passes, inside the timed
@@ -395,7 +395,7 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     for s in (1, 2):
         ctx.guide_reset(s)
         gp = la.guide_params(alpha=alpha, train_passes=train_passes, steps_per_pass=steps_per_pass, batch=batch,
-                             lr=5e-3, seed=s, max_vertex=max_vertex, materials=materials)
+                             lr=5e-3, seed=s, max_vertex=max_vertex, materials=materials, lobes=lobes)
         t0 = time.perf_counter()
         guided.append(ctx.render_guided(pr(100 + s), gp))
         tg.append(time.perf_counter() - t0)
@@ -407,6 +407,7 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     return {"workload": f"glossy interior ({scene.n_tris:,} tris: room shell + 200 Blinn rounded boxes, "
                         f"hidden light), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
             "spp_per_run": spp, "train_passes": train_passes, "alpha": alpha, "max_vertex": max_vertex,
+            "lobes": lobes or 3,
             "steps_per_pass": steps_per_pass, "batch": batch,
             "plain": {"run_s": round(t_p, 3), "samples_per_s": round(samples / t_p, 1), "relmse": round(rp, 5),
                       "time_to_relmse_1e-3_s": round(t_p * rp / 1e-3, 2)},
@@ -535,6 +536,9 @@ def main():
     ap.add_argument("--guide-steps-per-pass", type=int, default=8)
     ap.add_argument("--guide-max-vertex", type=int, default=4,
                     help="guide path vertices of bounce index below this (0 = every vertex)")
+    ap.add_argument("--guide-lobes", type=int, default=1,
+                    help="guided lobes: 1 Lambertian, 2 Blinn Kd, 4 Blinn specular (0 = 1 + 2)")
+    ap.add_argument("--guide-alpha", type=float, default=0.5)
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
@@ -565,7 +569,8 @@ def main():
         if "guiding" in legs:
             out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
                                            max_vertex=args.guide_max_vertex,
-                                           steps_per_pass=args.guide_steps_per_pass)
+                                           steps_per_pass=args.guide_steps_per_pass, lobes=args.guide_lobes,
+                                           alpha=args.guide_alpha)
         if "gpu_bvh" in legs:
             out["gpu_bvh"] = gpu_bvh_bench()
         print(json.dumps(out), flush=True)
@@ -628,8 +633,8 @@ def main():
         log(f"adjoint: {out['adjoint']}")
     if "guiding" in legs:
         out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
-                                           max_vertex=args.guide_max_vertex,
-                                           steps_per_pass=args.guide_steps_per_pass)
+                                       max_vertex=args.guide_max_vertex, steps_per_pass=args.guide_steps_per_pass,
+                                       lobes=args.guide_lobes, alpha=args.guide_alpha)
         log(f"guiding: {out['guiding']}")
     if "stress" in legs:
         out["stress"] = stress_bench()

@@ -282,15 +282,16 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
 /* ---- learned path guiding -------------------------------------------- */
 /* No reference counterpart: a proposal distribution for the continuation
  * direction of tracePath's BSDF sample (src/renderer.cpp:116-145), learned
- * online.  A small MLP (6 inputs: position in the scene box, shading normal
- * as two angles, hero wavelength, each in [0,1]; 12-frequency
+ * online.  A small MLP (6 inputs: position in the scene box, the outgoing
+ * direction as two angles, hero wavelength, each in [0,1]; 12-frequency
  * encoding; 4 hidden layers of 64; bf16 weights on MFMA with fp32
  * accumulation) outputs 8 von Mises-Fisher lobes (40 raw values: per lobe
  * an unnormalised mean direction, log κ and a weight logit).  Where the
- * reference's sampler picks a diffuse lobe (Lambertian; the Kd lobe of
- * Blinn), the direction is drawn from the mixture with probability alpha and
- * from the lobe's cosine distribution otherwise, weighted by one-sample MIS
- * over both, so the image converges to lam_render's. */
+ * reference's sampler picks a lobe named in `lobes` (a Lambertian cosine
+ * lobe, the Kd lobe or the specular reflection lobe of Blinn), the
+ * direction is drawn from the mixture with probability alpha and from the
+ * lobe's own distribution otherwise, weighted by one-sample MIS over both,
+ * so the image converges to lam_render's. */
 typedef struct {
   float alpha;             /* guide-branch probability, [0, 1); 0 = lam_render */
   uint32_t train_passes;   /* the first passes train the network online */
@@ -300,7 +301,13 @@ typedef struct {
   uint64_t seed;           /* network initialisation (if none yet) and minibatches */
   uint32_t materials;      /* material types guided: bit (1 << LAM_MAT_*); 0 = all supported */
   uint32_t max_vertex;     /* guide path vertices of bounce index < max_vertex (0 = every vertex) */
+  uint32_t lobes;          /* LAM_GUIDE_* lobes proposed; 0 = LAM_GUIDE_LAMBERTIAN | LAM_GUIDE_BLINN_KD */
 } lam_guide_params;
+/* lam_guide_params.lobes: the sampler lobes the mixture stands in for */
+#define LAM_GUIDE_LAMBERTIAN 0x1u /* a Lambertian material's cosine lobe (cg.hpp:1755-1778) */
+#define LAM_GUIDE_BLINN_KD 0x2u   /* the Kd lobe of a Blinn material (cg.hpp:1853-1861) */
+#define LAM_GUIDE_BLINN_SPEC 0x4u /* the specular reflection lobe of a Blinn material without
+                                     transmission (Kt = 0; cg.hpp:1815-1847) */
 
 /* lam_render with the learned proposal.  The network persists in the
  * context (initialised from g->seed on first use, or by lam_guide_reset /

@@ -114,16 +114,20 @@ class DiffParamsC(C.Structure):
 class GuideParams(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train_passes", C.c_uint32), ("steps_per_pass", C.c_uint32),
                 ("batch", C.c_uint32), ("lr", C.c_float), ("seed", C.c_uint64), ("materials", C.c_uint32),
-                ("max_vertex", C.c_uint32)]
+                ("max_vertex", C.c_uint32), ("lobes", C.c_uint32)]
+
+
+GUIDE_LAMBERTIAN, GUIDE_BLINN_KD, GUIDE_BLINN_SPEC = 0x1, 0x2, 0x4
 
 
 def guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1, materials=0,
-                 max_vertex=0):
+                 max_vertex=0, lobes=0):
     """lam_guide_params: guide-branch probability, online-training schedule,
     guided material types (bit 1 << MAT_*; 0 = every supported type),
-    guided vertices (bounce index < max_vertex; 0 = every vertex)."""
+    guided vertices (bounce index < max_vertex; 0 = every vertex), guided
+    lobes (GUIDE_* bits; 0 = the two diffuse lobes, Lambertian and Blinn Kd)."""
     return GuideParams(float(alpha), int(train_passes), int(steps_per_pass), int(batch), float(lr), int(seed),
-                       int(materials), int(max_vertex))
+                       int(materials), int(max_vertex), int(lobes))
 
 
 GUIDE_IN, GUIDE_OUT, GUIDE_LOBES = 6, 40, 8

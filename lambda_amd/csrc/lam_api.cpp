@@ -1492,6 +1492,7 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   gd.mat_mask = g->materials ? g->materials
                              : (1u << LAM_MAT_LAMBERTIAN) | (1u << LAM_MAT_BLINN) | (1u << LAM_MAT_THINFILM);
   gd.max_vertex = g->max_vertex ? g->max_vertex : 0xffffffffu;
+  gd.lobes = g->lobes ? g->lobes : (LAM_GUIDE_LAMBERTIAN | LAM_GUIDE_BLINN_KD);
   gd.lo = lam::f3{ctx->box_lo[0], ctx->box_lo[1], ctx->box_lo[2]};
   gd.inv_ext = lam::f3{1.f / (ctx->box_hi[0] - ctx->box_lo[0]), 1.f / (ctx->box_hi[1] - ctx->box_lo[1]),
                        1.f / (ctx->box_hi[2] - ctx->box_lo[2])};

@@ -2,8 +2,8 @@
 // direction, hero wavelength) to a mixture of 8 von Mises-Fisher lobes over
 // the continuation direction.  It is evaluated with bf16 MFMA
 // (v_mfma_f32_16x16x32_bf16, fp32 accumulate) one wave at a time, and mixed
-// with the cosine sampling of diffuse lobes under one-sample MIS, so the
-// estimator keeps the expectation of the plain render loop.
+// with the sampling of a diffuse or glossy lobe under one-sample MIS, so
+// the estimator keeps the expectation of the plain render loop.
 //
 // Network: 6 inputs in [0,1] -> frequency encoding (sin/cos(π 2^j x) for
 // j = 0..11 and the raw inputs: 150 features, padded to 160) -> 64 -> 64 ->
@@ -324,33 +324,52 @@ __device__ __forceinline__ f3 mix_sample(const GuideMix &m, float ul, float u1, 
 }
 
 // ---- where the proposal applies ----------------------------------------------------------
-// The learned proposal stands in for the cosine sampling of a diffuse lobe:
-// all of a Lambertian material (cg.hpp:1755-1778) and the Kd lobe of a Blinn
-// material (cg.hpp:1853-1861).  Glossy and thin-film lobes keep their own
-// samplers.
-__device__ __forceinline__ bool guide_supported(const lam_material &m) {
-  return m.type == LAM_MAT_LAMBERTIAN || (m.type == LAM_MAT_BLINN && m.p_d > 0.f);
+// The learned proposal stands in for one lobe of the reference's sampler:
+// the cosine lobe of a Lambertian material (cg.hpp:1755-1778), the Kd lobe
+// of a Blinn material (cg.hpp:1853-1861) or the specular reflection lobe of
+// a Blinn material without transmission (Kt = 0, so fresnelSplit always
+// reflects; cg.hpp:1815-1847), as selected by GuideDev::lobes.  Thin film
+// and transmitting Blinn lobes keep their own samplers.
+constexpr int kGuideNone = 0, kGuideCosine = 1, kGuideBlinnSpec = 2;
+__device__ __forceinline__ bool guide_supported(const GuideDev &gd, const lam_material &m) {
+  if (m.type == LAM_MAT_LAMBERTIAN) return (gd.lobes & LAM_GUIDE_LAMBERTIAN) != 0;
+  if (m.type != LAM_MAT_BLINN) return false;
+  return ((gd.lobes & LAM_GUIDE_BLINN_KD) && m.p_d > 0.f) ||
+         ((gd.lobes & LAM_GUIDE_BLINN_SPEC) && m.w_t == 0.f && m.p_spec > 0.f);
 }
 
-// Does the guided shade draw this vertex's continuation from a diffuse lobe
-// the proposal applies to, i.e. read the network's mixture?  The
-// reference's lobe choice from the LOBE dimension xl (cg.hpp:1768, 1853)
-// with roulette probability P; pp_d is the chosen lobe's selection
-// probability.  The network is evaluated for exactly these vertices.
+// Does the guided shade draw this vertex's continuation from a lobe the
+// proposal applies to, i.e. read the network's mixture?  The reference's
+// lobe choice from the LOBE dimension xl (cg.hpp:1768, 1809-1853) with
+// roulette probability P; pp = the chosen lobe's selection probability.
+// The network is evaluated for exactly these vertices.
 __device__ __forceinline__ bool guide_vertex(const GuideDev &gd, const lam_material &m, uint32_t s) {
-  return gd.alpha > 0.f && ((gd.mat_mask >> m.type) & 1u) && guide_supported(m) && s < gd.max_vertex;
+  return gd.alpha > 0.f && ((gd.mat_mask >> m.type) & 1u) && guide_supported(gd, m) && s < gd.max_vertex;
 }
-__device__ __forceinline__ bool guide_diffuse_draw(const GuideDev &gd, const lam_material &m, uint32_t s, float xl,
-                                                   float P, float &pp_d) {
-  pp_d = 0.f;
-  if (!guide_vertex(gd, m, s)) return false;
+__device__ __forceinline__ int guide_draw(const GuideDev &gd, const lam_material &m, uint32_t s, float xl, float P,
+                                          bool front, float &pp) {
+  pp = 0.f;
+  if (!guide_vertex(gd, m, s)) return kGuideNone;
   if (m.type == LAM_MAT_LAMBERTIAN) {
-    pp_d = P;
-    return !(xl > P);
+    pp = P;
+    return !(xl > P) ? kGuideCosine : kGuideNone;
   }
-  const float pp_spec = P * m.p_spec;
-  pp_d = P * m.p_d;
-  return xl < P && !(xl < pp_spec) && xl < pp_spec + pp_d;
+  if (!(xl < P)) return kGuideNone;
+  const float pp_spec = P * m.p_spec, pp_d = P * m.p_d;
+  if (xl < pp_spec) {
+    // front faces only: on a back face the reference's lobe value is
+    // powf(h·F.z, α) of a negative base (cg.hpp:1830-1846)
+    if ((gd.lobes & LAM_GUIDE_BLINN_SPEC) && m.w_t == 0.f && front) {
+      pp = pp_spec;
+      return kGuideBlinnSpec;
+    }
+    return kGuideNone;
+  }
+  if (xl < pp_spec + pp_d && (gd.lobes & LAM_GUIDE_BLINN_KD)) {
+    pp = pp_d;
+    return kGuideCosine;
+  }
+  return kGuideNone;
 }
 
 // power-heuristic weight a² / (a² + b²) of a strategy with pdf a against one
@@ -361,17 +380,16 @@ __device__ __forceinline__ float mis_w(float a, float b2) {
 }
 
 // network inputs of a vertex, each in [0,1]: position in the scene box, the
-// shading normal on the visible side as (θ/π, φ/2π) (the guided lobes are
-// diffuse: their target cos·L_i depends on the normal, not on the outgoing
-// direction), hero wavelength on the grid
+// outgoing direction (towards the previous vertex, world space) as
+// (θ/π, φ/2π), hero wavelength on the grid
 template <int H>
-__device__ inline void guide_inputs(const GuideDev &gd, const DevScene &sc, f3 x, f3 n, const Lam<H> &L,
+__device__ inline void guide_inputs(const GuideDev &gd, const DevScene &sc, f3 x, f3 o, const Lam<H> &L,
                                     float *in) {
   in[0] = fminf(fmaxf((x.x - gd.lo.x) * gd.inv_ext.x, 0.f), 1.f);
   in[1] = fminf(fmaxf((x.y - gd.lo.y) * gd.inv_ext.y, 0.f), 1.f);
   in[2] = fminf(fmaxf((x.z - gd.lo.z) * gd.inv_ext.z, 0.f), 1.f);
-  in[3] = acosf(fminf(fmaxf(n.z, -1.f), 1.f)) * kInvPi;
-  in[4] = (atan2f(n.y, n.x) + kPi) * (.5f * kInvPi);
+  in[3] = acosf(fminf(fmaxf(o.z, -1.f), 1.f)) * kInvPi;
+  in[4] = (atan2f(o.y, o.x) + kPi) * (.5f * kInvPi);
   in[5] = (L.l[0] - sc.grid_inf) / (sc.grid_sup - sc.grid_inf);
 }
 

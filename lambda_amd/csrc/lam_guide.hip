@@ -8,11 +8,13 @@
 // loss is the weighted negative log-likelihood of the mixture,
 //   loss = -Σ_i (w_i / Σw) log p_g(ω_i | x_i),
 // whose minimiser is the mixture closest in KL divergence to the sampled
-// distribution of radiance (times the plain continuation's weight).  The
-// forward pass reproduces the MFMA network: bf16-rounded weights, features
-// and activations, fp32 accumulation; the backward pass is fp32 with the
-// bf16 roundings treated as identity.  Adam updates fp32 master parameters,
-// which are then repacked into bf16 fragments.
+// distribution of radiance (times the plain continuation's weight).  Every
+// GEMM of the forward and backward passes runs on bf16 MFMA with fp32
+// accumulation (k_gemm_mfma): the forward reproduces the inference network
+// (bf16 weights, features and activations); the backward rounds dZ to bf16
+// where it enters a product and treats the forward's bf16 roundings as
+// identity.  Adam updates fp32 master parameters, which are then repacked
+// into bf16 fragments.
 #include <algorithm>
 
 #include "lam_guide.h"
@@ -144,55 +146,71 @@ __global__ void k_guide_feat_rows(const float *__restrict__ rows, uint32_t B, fl
   tgt[i] = make_float4(r[6], r[7], r[8], r[9]);
 }
 
-// C[M x N] (+)= A[M x K] B[K x N], fp32; A(m,k) = TA ? A[k lda + m] : A[m lda + k],
-// B(k,n) = TB ? B[n ldb + k] : B[k ldb + n].  64 x 64 tiles, 256 threads with
-// 4 x 4 outputs each; blockIdx.z splits K and then adds with atomics.
+// C[M x N] (+)= A[M x K] B[K x N] on v_mfma_f32_16x16x32_bf16: A and B are
+// read as fp32, rounded to bf16 (nearest even) into LDS, and their products
+// summed in fp32.  A(m,k) = TA ? A[k lda + m] : A[m lda + k],
+// B(k,n) = TB ? B[n ldb + k] : B[k ldb + n].  64 x 64 block tiles, four waves
+// of 32 x 32 (2 x 2 MFMA tiles), K steps of 32; blockIdx.z splits K into
+// chunks of `kchunk` (a multiple of 32) whose partial sums are added with
+// atomics.  The operands of the network's layers are bf16 already
+// (weights, features, activations: exact); the backward's dZ is rounded.
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm(uint32_t M, uint32_t N, uint32_t K, const float *__restrict__ A,
-                                              uint32_t lda, const float *__restrict__ Bm, uint32_t ldb,
-                                              float *__restrict__ C, uint32_t ldc, uint32_t kchunk, int atomic) {
-  __shared__ float As[16][64 + 4], Bs[16][64 + 4];
-  const uint32_t tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+__global__ __launch_bounds__(256) void k_gemm_mfma(uint32_t M, uint32_t N, uint32_t K, const float *__restrict__ A,
+                                                   uint32_t lda, const float *__restrict__ Bm, uint32_t ldb,
+                                                   float *__restrict__ C, uint32_t ldc, uint32_t kchunk, int atomic) {
+  // [row][k] bf16; rows of 40 halves (80 B) keep every 8-half group 16-byte aligned
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];
+  __shared__ __attribute__((aligned(16))) uint16_t Bt[64][40];
+  const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64, ln = lane % 16, lg = lane / 16;
+  const uint32_t wm = wave / 2, wn = wave % 2;
   const uint32_t m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const uint32_t k_begin = blockIdx.z * kchunk, k_end = min(K, k_begin + kchunk);
-  float acc[4][4] = {};
-  for (uint32_t k0 = k_begin; k0 < k_end; k0 += 16) {
-    for (uint32_t e = threadIdx.x; e < 16 * 64; e += 256) {
-      const uint32_t kk = e / 64, mm = e % 64;
-      const uint32_t m = m0 + mm, k = k0 + kk;
-      float av = 0.f, bv = 0.f;
-      if (m < M && k < k_end) av = TA ? A[size_t(k) * lda + m] : A[size_t(m) * lda + k];
-      const uint32_t nn = n0 + mm;
-      if (nn < N && k < k_end) bv = TB ? Bm[size_t(nn) * ldb + k] : Bm[size_t(k) * ldb + nn];
-      As[kk][mm] = av;
-      Bs[kk][mm] = bv;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32) {
+    // 64 x 32 elements of each operand, 8 per thread; the global index that
+    // varies fastest follows the operand's storage order
+    for (uint32_t e = tid; e < 64u * 32u; e += 256u) {
+      const uint32_t am = TA ? e % 64u : e / 32u, ak = TA ? e / 64u : e % 32u;
+      const uint32_t gm = m0 + am, gk = k0 + ak;
+      float av = 0.f;
+      if (gm < M && gk < k_end) av = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
+      As[am][ak] = f2bf(av);
+      const uint32_t bn = TB ? e / 32u : e % 64u, bk = TB ? e % 32u : e / 64u;
+      const uint32_t gn = n0 + bn, gk2 = k0 + bk;
+      float bv = 0.f;
+      if (gn < N && gk2 < k_end) bv = TB ? Bm[size_t(gn) * ldb + gk2] : Bm[size_t(gk2) * ldb + gn];
+      Bt[bn][bk] = f2bf(bv);
     }
     __syncthreads();
+    // lane (n, g) supplies A[n][8g .. 8g+7] and B[8g .. 8g+7][n]
 #pragma unroll
-    for (int kk = 0; kk < 16; kk++) {
-      float a[4], b[4];
+    for (int mi = 0; mi < 2; mi++) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(&As[32 * wm + 16 * mi + ln][8 * lg]);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        a[i] = As[kk][ty * 4 + i];
-        b[i] = Bs[kk][tx * 4 + i];
+      for (int ni = 0; ni < 2; ni++) {
+        const uint4 b = *reinterpret_cast<const uint4 *>(&Bt[32 * wn + 16 * ni + ln][8 * lg]);
+        acc[mi][ni] = mfma16(a, b, acc[mi][ni]);
       }
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
     }
     __syncthreads();
   }
+  // lane (n, g) holds D[4g + i][n] of each tile
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int mi = 0; mi < 2; mi++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t m = m0 + ty * 4 + i, n = n0 + tx * 4 + j;
-      if (m < M && n < N) {
-        if (atomic) atomicAdd(C + size_t(m) * ldc + n, acc[i][j]);
-        else C[size_t(m) * ldc + n] = acc[i][j];
+    for (int ni = 0; ni < 2; ni++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t m = m0 + 32 * wm + 16 * mi + 4 * lg + i, n = n0 + 32 * wn + 16 * ni + ln;
+        if (m < M && n < N) {
+          if (atomic) atomicAdd(C + size_t(m) * ldc + n, acc[mi][ni][i]);
+          else C[size_t(m) * ldc + n] = acc[mi][ni][i];
+        }
       }
-    }
 }
 
 // Z[b][j] += bias[j]; X = bf16(relu(Z)) (hidden layers) or X = Z (output)
@@ -321,15 +339,15 @@ static inline uint32_t cdiv(size_t a, uint32_t b) { return uint32_t((a + b - 1) 
 template <bool TA, bool TB>
 static void gemm(uint32_t M, uint32_t N, uint32_t K, const float *A, uint32_t lda, const float *B, uint32_t ldb,
                  float *C, uint32_t ldc, bool split, hipStream_t st) {
-  uint32_t zs = 1, kchunk = K;
+  uint32_t zs = 1, kchunk = (K + 31) / 32 * 32;
   if (split) {  // K = batch: spread over about 256 blocks with atomics
     const uint32_t tiles = cdiv(M, 64) * cdiv(N, 64);
     zs = std::max(1u, std::min(cdiv(K, 256), 256u / std::max(tiles, 1u)));
-    kchunk = (cdiv(K, zs) + 15) / 16 * 16;
+    kchunk = (cdiv(K, zs) + 31) / 32 * 32;
     zs = cdiv(K, kchunk);
   }
-  hipLaunchKernelGGL((k_gemm<TA, TB>), dim3(cdiv(N, 64), cdiv(M, 64), zs), dim3(256), 0, st, M, N, K, A, lda, B,
-                     ldb, C, ldc, kchunk, split ? 1 : 0);
+  hipLaunchKernelGGL((k_gemm_mfma<TA, TB>), dim3(cdiv(N, 64), cdiv(M, 64), zs), dim3(256), 0, st, M, N, K, A, lda,
+                     B, ldb, C, ldc, kchunk, split ? 1 : 0);
 }
 
 const float *guide_train_raw(const float *ws, uint32_t B) {

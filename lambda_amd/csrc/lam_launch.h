@@ -94,6 +94,7 @@ struct GuideDev {
   uint4 *mix;             // network outputs per extend-queue entry: 40 fp16 (5 uint4)
   float alpha;            // probability of the guide branch at supported vertices
   uint32_t mat_mask;      // guided material types (bit 1 << type)
+  uint32_t lobes;         // LAM_GUIDE_* lobes the mixture stands in for
   uint32_t max_vertex;    // guided vertices: bounce index < max_vertex
   f3 lo, inv_ext;         // scene box
   float4 *rec;            // training records (4 float4 each); null when not training

@@ -333,17 +333,20 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
 
 // ---- guided shade ----------------------------------------------------------------------
 // k_shade with the learned proposal.  Where the reference's sampler picks a
-// diffuse lobe (a Lambertian material, or the Kd lobe of a Blinn material:
-// the same lobe choice from dimension LOBE), the direction is drawn from the
-// vMF mixture with probability α (dimension GUIDE) and from the lobe's
-// cosine distribution otherwise, and both are weighted by one-sample MIS
-// over the mixture of the two:
-//   β ← β ⊙ f_d(ω) w_d(ω) / (pp_d M(ω)),  M = (1-α) cos/π + α p_g,
-// pp_d = the lobe's selection probability, f_d its BSDF·cos and w_d the
-// reference's MIS weight against the light (with its pdf pp_d cos/π).  The
-// expectation of the lobe's term is the plain one: pp_d E_M[f_d w_d L/(pp_d M)]
-// = ∫ f_d w_d L.  Other lobes, the NEE term and its weight, and roulette
-// are unchanged; α = 0 reproduces k_shade.
+// lobe the mixture stands in for (guide_draw: a Lambertian cosine lobe, the
+// Kd lobe or the specular reflection lobe of a Blinn material, with the same
+// lobe choice from dimension LOBE), the direction is drawn from the vMF
+// mixture with probability α (dimension GUIDE) and from the lobe's own
+// distribution otherwise, and both are weighted by one-sample MIS over the
+// mixture of the two:
+//   β ← β ⊙ f(ω) w(ω) r(ω) / (pp M(ω)),  M = (1-α) p_lobe + α p_g,
+// pp = the lobe's selection probability, f its BSDF·cos, w the reference's
+// MIS weight against the light (with the reference's lobe pdf q) and
+// r = p_lobe / (q / pp) the ratio of the lobe's true sampling density to the
+// reference's pdf (1 for cosine lobes, 1/(cos θh |o·h|) for the Blinn
+// reflection).  The lobe's term keeps the plain expectation ∫ f w r L.
+// Other lobes, the NEE term and its weight, and roulette are unchanged;
+// α = 0 reproduces k_shade.
 // TRAIN: each guided continuation appends a training record (network inputs,
 // ω, Σβ, ΣL before the vertex, ΣNEE term, slot; k_shadow marks a visible
 // NEE term).
@@ -363,9 +366,9 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t k) {
 }
 
 // The network for the entries of the extend queue (after k_extend) whose
-// continuation k_shade_guided will draw from the mixture (guide_diffuse_draw
-// of bounce s): inputs from the hit's surface (position, shading normal) and
-// the path's hero wavelength; 40 outputs stored as fp16 at the entry.  A
+// continuation k_shade_guided will draw from the mixture (guide_draw of
+// bounce s): inputs from the hit (position, outgoing direction) and the
+// path's hero wavelength; 40 outputs stored as fp16 at the entry.  A
 // wave scans 64 entries at a time and packs the ones that need the network
 // into full waves of 64 queries (ballot, k-th set bit, lane shuffles; one
 // wave of overflow), so that each MFMA evaluation serves 64 queries; the
@@ -429,13 +432,15 @@ __global__ __launch_bounds__(256) void k_guide_queue(DevScene sc, PathArgs pa, P
           uint32_t pixel, sample;
           path_id(ids, ctrl, __float_as_uint(b.w), pixel, sample);
           const uint64_t key = lam_path_key(pa.seed, pixel, sample);
-          float pp_d;
-          if (guide_diffuse_draw(gd, mat, s, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)), pa.sample_p, pp_d)) {
-            const float4 a = rin[2 * size_t(q)];
-            const Surf sf = surface(sc, hit, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z));
+          float pp;
+          const float4 a = rin[2 * size_t(q)];
+          const f3 rd = mk(b.x, b.y, b.z);
+          const Surf sf = surface(sc, hit, mk(a.x, a.y, a.z), rd);
+          if (guide_draw(gd, mat, s, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)), pa.sample_p, sf.front, pp) !=
+              kGuideNone) {
             Lam<H> lam;
             lam_for<H>(sc, key, lam);
-            guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
+            guide_inputs<H>(gd, sc, sf.p, neg(normalized(rd)), lam, gin);
             need = true;
           }
         }
@@ -533,7 +538,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
         } else if (s <= pa.max_scat) {
           live = true;
           guided = guide_vertex(gd, mat, s);
-          if (TRAIN && guided) guide_inputs<H>(gd, sc, sf.p, sf.n(), lam, gin);
+          if (TRAIN && guided) guide_inputs<H>(gd, sc, sf.p, o, lam, gin);
         }
       }
     }
@@ -557,13 +562,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       const float alpha = guided ? gd.alpha : 0.f;
       const float P = pa.sample_p;
       const float xl = lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE));
-      // does the reference's sampler take the diffuse lobe (cg.hpp:1768, 1853)?
-      float pp_d = 0.f;
-      const bool diffuse = guide_diffuse_draw(gd, mat, s, xl, P, pp_d);
+      // does the reference's sampler take a lobe the mixture stands in for
+      // (cg.hpp:1768, 1809-1853)?
+      float pp = 0.f;
+      const int kind = guide_draw(gd, mat, s, xl, P, sf.front, pp);
       bool ok = false;
       f3 wi = mk(0.f, 0.f, 0.f);
       float fac[H];
-      if (diffuse) {
+      if (kind != kGuideNone) {
         ok = true;
         const float xa = lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), xb = lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B));
         GuideMix gm;
@@ -572,29 +578,65 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
           load_mix(gd.mix + 5 * size_t(q), raw);
           mix_from_raw(raw, gm);
         }
-        if (lam_draw_f(key, lam_vdim(s, LAM_VDIM_GUIDE)) < alpha) {
+        const bool from_guide = lam_draw_f(key, lam_vdim(s, LAM_VDIM_GUIDE)) < alpha;
+        if (from_guide) {
           wi = mix_sample(gm, lam_draw_f(key, lam_vdim(s, LAM_VDIM_SPARE)), xa, xb);
-        } else {  // the lobe's own cosine sample (cg.hpp:1762-1770)
+        } else if (kind == kGuideCosine) {  // the lobe's own cosine sample (cg.hpp:1762-1770)
           f3 dir = cos_hemi(xa, xb);
           if (!sf.front) dir.z *= -1;
           wi = to_basis(sf.F, dir);
+        } else {  // the Blinn half vector, reflected (cg.hpp:1815-1820)
+          const f3 h = to_basis(sf.F, blinn_h(mat.alpha, xa, xb));
+          wi = reflect(o, h);
         }
-        const float c = dot(wi, nn);
         const float pg = mix_pdf(gm, wi);
-        const float M = (1.f - alpha) * fmax_ref(c, 0.f) * kInvPi + alpha * pg;
-        if (c > 0.f && M > 0.f) {
-          float kd[H];
-          spec_eval<H>(sc, mat.spec[0], lam, kd);
-          const float pi = pp_d * (c * kInvPi);
-          const float pLw = prob_light(sc, lt, sf.p, wi) * pl;
-          const float w_d = mis_w(pi, pLw * pLw);
-          const float g = w_d / (pp_d * M);
+        const float pLw = prob_light(sc, lt, sf.p, wi) * pl;
+        if (kind == kGuideCosine) {
+          // β ← β f_d w_d / (pp M), M = (1-α) cos/π + α p_g, w_d against the
+          // light with the reference's lobe pdf pp cos/π
+          const float c = dot(wi, nn);
+          const float M = (1.f - alpha) * fmax_ref(c, 0.f) * kInvPi + alpha * pg;
+          if (c > 0.f && M > 0.f) {
+            float kd[H];
+            spec_eval<H>(sc, mat.spec[0], lam, kd);
+            const float pi = pp * (c * kInvPi);
+            const float w_d = mis_w(pi, pLw * pLw);
+            const float g = w_d / (pp * M);
 #pragma unroll
-          for (int k = 0; k < H; k++) fac[k] = kd[k] * kInvPi * c * g;
+            for (int k = 0; k < H; k++) fac[k] = kd[k] * kInvPi * c * g;
+          } else {
+#pragma unroll
+            for (int k = 0; k < H; k++) fac[k] = 0.f;
+            if (!(M > 0.f)) atomicAdd(ctrl + CTRL_GUIDE_ZERO, 1u);
+          }
         } else {
+          // Blinn reflection (front face, Kt = 0): the half vector h of wi on
+          // the normal's side, the lobe's true density p_h(h)/(4|o·h|) with
+          // p_h = (α+1)/(2π) cos^α θh, and the reference's pdf
+          // q = pp (α+1)/(2π) cos^(α+1) θh / 4 (no 1/(o·h)).  The plain lobe
+          // term has the expectation ∫ f w_q L / (cos θh |o·h|) dω (its
+          // samples have the true density but are divided by q); the mixture
+          // estimate f w_q L / (cos θh |o·h| pp M) keeps it.
+          f3 h = normalized(add(wi, o));
+          if (dot(h, nn) < 0.f) h = neg(h);
+          const float ch = dot(h, nn), oh = fabsf(dot(o, h));
+          const float ap1 = mat.alpha + 1.f;
+          const float pw = lam_powf(ch, mat.alpha);
+          const float plobe = oh > 0.f ? ap1 * .5f * kInvPi * pw / (4.f * oh) : 0.f;
+          const float M = (1.f - alpha) * plobe + alpha * pg;
+          if (ch > 0.f && oh > 0.f && M > 0.f) {
+            float ks[H];
+            spec_eval<H>(sc, mat.spec[1], lam, ks);
+            const float qref = pp * (ap1 * .5f * kInvPi * (pw * ch)) * .25f;
+            const float w_q = mis_w(qref, pLw * pLw);
+            const float g = w_q / (ch * oh * pp * M);
 #pragma unroll
-          for (int k = 0; k < H; k++) fac[k] = 0.f;
-          if (!(M > 0.f)) atomicAdd(ctrl + CTRL_GUIDE_ZERO, 1u);
+            for (int k = 0; k < H; k++) fac[k] = ks[k] * (mat.alpha + 2) * .125f * kInvPi * pw * g;
+          } else {
+#pragma unroll
+            for (int k = 0; k < H; k++) fac[k] = 0.f;
+            if (!(M > 0.f)) atomicAdd(ctrl + CTRL_GUIDE_ZERO, 1u);
+          }
         }
       } else {
         MatSample<H> ms;
@@ -634,7 +676,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
         nxt.o = sf.p;
         nxt.d = wi;
       }
-      if (TRAIN) want_rec = diffuse && push_next;
+      if (TRAIN) want_rec = kind != kGuideNone && push_next;
       if (TRAIN && want_rec) {
 #pragma unroll
         for (int k = 0; k < H; k++) {

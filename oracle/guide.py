@@ -17,7 +17,9 @@ library's own definition:
            log κ (clamped to [-4, 9]), raw[5L+4] weight logit (softmax)
   loss     -Σ_i (w_i / Σw) log p(ω_i), p = Σ_L π_L vMF(ω; μ_L, κ_L),
            vMF = κ / (2π (1 - e^{-2κ})) e^{κ (μ·ω - 1)}
-  backward float64, the bf16 roundings treated as identity (straight-through)
+  backward float64, the bf16 roundings treated as identity (straight-through);
+           optionally dZ rounded to bf16 in the products, as the library's
+           bf16 MFMA training GEMMs take it
 """
 from __future__ import annotations
 
@@ -139,8 +141,11 @@ def mix_pdf(raw, om):
     return (w * np.exp(log_vmf(mu, kappa, om))).sum(axis=1)
 
 
-def loss_grad(theta, rows):
-    """(loss, gradient) of -Σ (w_i/Σw) log p(ω_i | in_i) for rows (n x 10)"""
+def loss_grad(theta, rows, bf16_dz=False):
+    """(loss, gradient) of -Σ (w_i/Σw) log p(ω_i | in_i) for rows (n x 10).
+    bf16_dz: dZ of each layer rounded to bf16 where it enters the weight
+    gradient and the back-propagated product (as the library's bf16 MFMA
+    training GEMMs take it); the bias gradients sum the unrounded dZ"""
     rows = np.asarray(rows, dtype=np.float64).reshape(-1, 10)
     inp, om, wt = rows[:, :6], rows[:, 6:9], rows[:, 9]
     raw, xs, zs = forward(theta, inp, keep=True)
@@ -179,10 +184,11 @@ def loss_grad(theta, rows):
     grads = {}
     for l in range(5, 0, -1):
         xin = xs[l - 1]
-        grads[f"W{l}"] = dz.T @ xin
+        dzq = bf16(dz) if bf16_dz else dz
+        grads[f"W{l}"] = dzq.T @ xin
         grads[f"b{l}"] = dz.sum(axis=0)
         if l > 1:
-            dx = dz @ bf16(P[f"W{l}"])
+            dx = dzq @ bf16(P[f"W{l}"])
             dz = np.where(zs[l - 2] > 0, dx, 0.0)
     # W1 covers the 150 real features
     return loss, pack(grads)
