@@ -334,3 +334,40 @@ def test_oracle_bf16_dz_gradient_is_close_to_straight_through():
         if name == "b5":
             assert np.array_equal(a, b)
         off += n
+
+
+@pytest.mark.gpu
+def test_guided_and_plain_match_the_fp64_reference(ctx, tmp_path):
+    """The guiding workload's scene (glossy interior, seed 4) at 64x64:
+    the plain GPU render and the guided one (network trained online on its
+    first 32 passes) both converge to the FP64 oracle's 4096-spp image
+    (tests/golden/glossy_interior_64_fp64.npz, written by
+    tests/golden/make_glossy_fp64.py): image sums within 4 standard errors,
+    per-pixel |z| of median below 1.2, relMSE against the reference printed"""
+    import os
+    from lambda_amd import scenes
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "glossy_interior_64_fp64.npz"))
+    sc = la.load_nls(scenes.glossy_interior(str(tmp_path), seed=int(ref["scene_seed"]), width=int(ref["width"])))
+    ctx.upload(sc)
+    spp, b, p = 2048, int(ref["bounces"]), float(ref["sample_p"])
+    mk = lambda seed, off=0: la.params(spp=spp // 2, max_scatterings=b, sample_p=p, hero=sc.hero, seed=seed,
+                                       spp_offset=off)
+    plain = [ctx.render(mk(3, k * spp // 2)).astype(np.float64) for k in range(2)]
+    ctx.guide_reset(5)
+    guided = [ctx.render_guided(mk(5 + k), la.guide_params(alpha=0.5, train_passes=32 if k == 0 else 0,
+                                                           steps_per_pass=8, batch=4096, seed=5, max_vertex=4,
+                                                           lobes=la.GUIDE_LAMBERTIAN)).astype(np.float64)
+              for k in range(2)]
+    r, vr = ref["mean"].astype(np.float64), ref["var_of_mean"].astype(np.float64)
+    w = np.array([0.2126, 0.7152, 0.0722])
+    out = {}
+    for name, runs in (("plain", plain), ("guided", guided)):
+        m = (runs[0] + runs[1]) / 2
+        v = (runs[0] - runs[1]) ** 2 / 4
+        sd = np.sqrt(v.sum() + vr.sum())
+        assert abs(m.sum() - r.sum()) < 4 * sd, (name, m.sum(), r.sum(), sd)
+        z = (m - r) / np.sqrt(v + vr + 1e-30)
+        assert np.median(np.abs(z)) < 1.2, (name, np.median(np.abs(z)))
+        y, yr = m @ w, r @ w
+        out[name] = float(np.mean((y - yr) ** 2 / (yr ** 2 + 0.01 * yr.mean() ** 2)))
+    print(f"relMSE against the FP64 reference at {spp} spp: {out}")
