@@ -540,6 +540,9 @@ def main():
                     help="guided lobes: 1 Lambertian, 2 Blinn Kd, 4 Blinn specular (0 = 1 + 2)")
     ap.add_argument("--guide-alpha", type=float, default=0.5)
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0, help="pass lanes (0 = the library's default)")
+    ap.add_argument("--samples-per-pass", type=int, default=0,
+                    help="samples of every pixel per wavefront pass (0 = the library's default)")
     ap.add_argument("--extend-only", action="store_true",
                     help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
                          "(the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
@@ -580,6 +583,10 @@ def main():
     scene = la.load_nls(scene_path())
     ctx = la.Context(0)
     ctx.upload(scene)
+    if args.samples_per_pass:
+        ctx.set_wave_config(samples_per_pass=args.samples_per_pass)
+    if args.lanes:
+        ctx.set_wave_config(lanes=args.lanes)
     st = ctx.stats()
     log(f"scene: {scene.n_tris} tris, {st.n_nodes} BVH8 nodes, depth {st.depth}, stack bound "
         f"{st.stack_bound}, setup {time.perf_counter() - t0:.1f}s")

@@ -149,7 +149,7 @@ class WaveConfig(C.Structure):
     """lam_wave_config"""
     _fields_ = [("lanes", C.c_uint32), ("tail_threshold", C.c_uint32), ("tail_div", C.c_uint32),
                 ("max_iters", C.c_uint32), ("trace_per_cu", C.c_uint32), ("batch_per_cu", C.c_uint32),
-                ("finish_per_cu", C.c_uint32), ("shade_per_cu", C.c_uint32)]
+                ("finish_per_cu", C.c_uint32), ("shade_per_cu", C.c_uint32), ("samples_per_pass", C.c_uint32)]
 
 
 class Timing(C.Structure):

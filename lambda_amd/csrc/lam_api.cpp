@@ -114,6 +114,7 @@ struct lam_ctx {
   uint32_t *trace_ctr = nullptr;  // fetch counters of the batch trace kernel (one per launch)
   uint32_t lanes = 3;     // passes in flight
   uint32_t tail_div = 16;  // the finish kernel's threshold is at most 1/tail_div of a chunk
+  uint32_t samples_per_pass = 8;  // samples of every pixel traced together in one plain wavefront pass
   std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
   float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // scene box (guiding inputs)
   // guiding network: fp32 master parameters, Adam moments, gradient,
@@ -134,6 +135,9 @@ struct lam_ctx {
 // Path slots per wavefront pass.  The frame is processed in chunks of this
 // many paths; 4 Mi slots with 16 hero wavelengths is ~0.9 GB of path state.
 static constexpr size_t kMaxWaveCap = size_t(1) << 22;
+// path slots of one plain wavefront pass of several samples per pixel
+// (samples_per_pass): ~3.6 GB of path state per lane at 16 hero wavelengths
+static constexpr size_t kMaxPassSlots = size_t(1) << 24;
 
 namespace {
 
@@ -582,7 +586,15 @@ struct GuideRun {
 // LAM_FLAG_NO_GRAPH launches without a graph.
 int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix,
                      const GuideRun *gr = nullptr) {
-  const size_t cap = std::min(npix, kMaxWaveCap);
+  // pixels per chunk, and samples per wavefront pass: a plain render of a
+  // frame that fits takes up to samples_per_pass samples of every pixel into
+  // one pass (longer queues, one tail per several samples)
+  const size_t cpix = std::min(npix, kMaxWaveCap);
+  const uint32_t m_cfg =
+      (!gr && npix <= kMaxWaveCap) ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass,
+                                                                                     kMaxPassSlots / npix)))
+                                   : 1u;
+  const size_t cap = cpix * std::min(m_cfg, std::max(p->spp, 1u));
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
   // every configured lane gets its state on the first render, also when
@@ -636,8 +648,8 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
       prof.counts = ctx->fetch_counts;
     }
   }
-  for (size_t off = 0; off < npix; off += cap) {
-    const uint32_t n = uint32_t(std::min(cap, npix - off));
+  for (size_t off = 0; off < npix; off += cpix) {
+    const uint32_t n = uint32_t(std::min(cpix, npix - off));
     // the finish kernel takes over below 1/tail_div of the chunk's paths
     // (and at most tail_threshold): guided passes then shade most vertices
     // with the learned proposal, and small frames keep a full wavefront
@@ -676,7 +688,13 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     if (first >= p->spp) continue;
-    const uint32_t nl = std::min(lanes, p->spp - first);
+    // passes of m samples of every pixel of the chunk (the last may hold fewer)
+    const uint32_t left = p->spp - first;
+    const uint32_t m = std::min(m_cfg, left);
+    const uint32_t ng = (left + m - 1) / m, nfull = left / m;
+    const uint32_t nl = std::min(lanes, ng);
+    const uint32_t npass = n * m;
+    wl.tail_threshold = std::min(wl_all.tail_threshold, npass / ctx->tail_div);
     // with several lanes the film kernel is enqueued on its own, after the wait
     float *pass_film = nl > 1 ? nullptr : film;
     if (nl > 1) {  // the other lanes start after everything enqueued on lane 0 so far
@@ -689,33 +707,35 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
       for (uint32_t l = 0; l < nl && e == hipSuccess; l++) {
         hipGraph_t g = nullptr;
         if ((e = hipStreamBeginCapture(st[l], hipStreamCaptureModeThreadLocal)) != hipSuccess) break;
-        hipError_t e1 = lam::enqueue_pass_next(ctrl[l], nl, st[l]);
-        hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], n,
+        hipError_t e1 = lam::enqueue_pass_next(ctrl[l], nl * m, st[l]);
+        hipError_t e2 = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], npass,
                                           ctrl[l], pass_film, nullptr, wl, st[l], nullptr, gr ? &gdl[l] : nullptr,
-                                          false);
+                                          false, m);
         hipError_t e3 = hipStreamEndCapture(st[l], &g);
         e = e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3;
         if (e == hipSuccess) e = hipGraphInstantiate(&ge[l], g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
-        // the graph's first node advances the pass by nl: start one step back
-        if (e == hipSuccess) e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + first + l - nl, st[l]);
+        // the graph's first node advances the first sample by nl m: start one
+        // step back (modulo 2^32)
+        if (e == hipSuccess) e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + first + (l - nl) * m, st[l]);
       }
     }
-    for (uint32_t s = first; s < p->spp && e == hipSuccess; s++) {
-      const uint32_t l = (s - first) % nl;
-      if (graph) {
+    for (uint32_t gi = 0; gi < ng && e == hipSuccess; gi++) {
+      const uint32_t l = gi % nl;
+      const uint32_t mg = gi < nfull ? m : left - nfull * m;  // samples of this pass
+      if (graph && mg == m) {
         e = hipGraphLaunch(ge[l], st[l]);
       } else {
-        e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + s, st[l]);
+        e = lam::enqueue_pass_set(ctrl[l], p->spp_offset + first + gi * m, st[l]);
         if (e == hipSuccess)
-          e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], n, ctrl[l],
+          e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[l], n * mg, ctrl[l],
                                 pass_film, nullptr, wl, st[l], profile ? &prof : nullptr, gr ? &gdl[l] : nullptr,
-                                false);
+                                false, mg);
       }
       if (nl > 1 && e == hipSuccess) {
-        if (s > first) e = hipStreamWaitEvent(st[l], ctx->lane[(l + nl - 1) % nl].ev_film, 0);
-        if (e == hipSuccess) e = lam::enqueue_film(int(p->hero), ctx->sc, pa, uint32_t(off), ws[l], n, ctrl[l],
-                                                   film, st[l]);
+        if (gi > 0) e = hipStreamWaitEvent(st[l], ctx->lane[(l + nl - 1) % nl].ev_film, 0);
+        if (e == hipSuccess) e = lam::enqueue_film(int(p->hero), ctx->sc, pa, uint32_t(off), ws[l], n * mg, ctrl[l],
+                                                   film, st[l], mg);
         if (e == hipSuccess) e = hipEventRecord(ctx->lane[l].ev_film, st[l]);
       }
       if (profile && e == hipSuccess) {
@@ -1636,6 +1656,7 @@ int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out) {
   out->batch_per_cu = ctx->batch_per_cu;
   out->finish_per_cu = ctx->finish_per_cu;
   out->shade_per_cu = ctx->shade_per_cu;
+  out->samples_per_pass = ctx->samples_per_pass;
   return LAM_OK;
 }
 
@@ -1652,6 +1673,8 @@ int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
   if (cfg->finish_per_cu < 1 || cfg->finish_per_cu > 32)
     return fail(ctx, LAM_E_INVALID, "finish_per_cu must be 1..32");
   if (cfg->shade_per_cu < 1 || cfg->shade_per_cu > 8) return fail(ctx, LAM_E_INVALID, "shade_per_cu must be 1..8");
+  if (cfg->samples_per_pass < 1 || cfg->samples_per_pass > 8)
+    return fail(ctx, LAM_E_INVALID, "samples_per_pass must be 1..8");
   ctx->lanes = cfg->lanes;
   ctx->wl.tail_threshold = cfg->tail_threshold;
   ctx->tail_div = cfg->tail_div;
@@ -1660,6 +1683,7 @@ int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
   ctx->batch_per_cu = cfg->batch_per_cu;
   ctx->finish_per_cu = cfg->finish_per_cu;
   ctx->shade_per_cu = cfg->shade_per_cu;
+  ctx->samples_per_pass = cfg->samples_per_pass;
   // before the first launch the CU count is not known yet: ensure_launch
   // sets the grids from the per-CU counts then
   if (ctx->cus > 0) {

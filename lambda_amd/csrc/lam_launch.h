@@ -126,14 +126,18 @@ hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipS
 // gd: null for the plain render loop; train: write training records.
 // With film and out both null the pass ends before its film kernel
 // (enqueue_film then splats it, e.g. after waiting for another stream).
+// m: samples per pass (without lists): n = m x pixels, slot q npix + i is
+// pixel pixel_offset + i at sample ctrl[CTRL_PASS] + q.
 hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
                         const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
-                        WaveProfile *prof, const GuideDev *gd = nullptr, bool train = false);
+                        WaveProfile *prof, const GuideDev *gd = nullptr, bool train = false, uint32_t m = 1);
 hipError_t enqueue_pass_set(uint32_t *ctrl, uint32_t pass, hipStream_t st);
-// the film kernel of a pass over pixels pixel_offset .. pixel_offset+n-1
+// the film kernel of a pass of n paths (m samples of n/m pixels from
+// pixel_offset), each pixel's samples added in sample order
 hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset,
-                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st);
+                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st,
+                        uint32_t m = 1);
 
 // ---- BVH8 build on the GPU (lam_bvh.hip) -----------------------------------------
 struct GpuBvhStats {

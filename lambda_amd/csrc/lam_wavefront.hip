@@ -49,16 +49,31 @@ constexpr uint32_t kShadeBlock = 256;
 // largest spectra table the shade kernel stages in LDS (3 blocks per CU)
 constexpr uint32_t kSpecLdsBytes = 48 * 1024;
 
+// Identity of a path slot.  From lists (test hooks), or a pass of m samples
+// of npix pixels: slot = q npix + i is pixel pixel_offset + i, sample
+// ctrl[CTRL_PASS] + q (npix = 0: one sample, slot = i).
 struct PathIds {
-  const uint32_t *pixel;   // nullable: then pixel = slot + pixel_offset
-  const uint32_t *sample;  // nullable: then sample = ctrl[CTRL_PASS]
+  const uint32_t *pixel;   // nullable
+  const uint32_t *sample;  // nullable
   uint32_t pixel_offset;
+  uint32_t npix;
 };
 
 __device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
                                         uint32_t &sample) {
-  pixel = ids.pixel ? ids.pixel[slot] : slot + ids.pixel_offset;
-  sample = ids.sample ? ids.sample[slot] : ctrl[CTRL_PASS];
+  if (ids.pixel) {
+    pixel = ids.pixel[slot];
+    sample = ids.sample[slot];
+    return;
+  }
+  uint32_t q = 0;
+  if (ids.npix)
+    while (slot >= ids.npix) {  // a few samples per pass at most
+      slot -= ids.npix;
+      q++;
+    }
+  pixel = slot + ids.pixel_offset;
+  sample = ctrl[CTRL_PASS] + q;
 }
 
 template <int H>
@@ -875,26 +890,33 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
 }
 
 // ---- film / output ----------------------------------------------------------------------
+// One thread per pixel of the pass: its m samples (slots i, i + npix, ...)
+// are added to the film in sample order, so the film receives the sums of a
+// one-sample-per-pass render in the same order.
 template <int H>
 __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t n,
                                               const uint32_t *__restrict__ ctrl, float *__restrict__ film,
                                               float *__restrict__ out) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (out) {  // test hook: raw spectral radiance per path
+    if (t >= n) return;
 #pragma unroll
-    for (int k = 0; k < H; k++) out[size_t(p) * H + k] = ws.L[size_t(k) * ws.cap + p];
+    for (int k = 0; k < H; k++) out[size_t(t) * H + k] = ws.L[size_t(k) * ws.cap + t];
     return;
   }
-  uint32_t pixel, sample;
-  path_id(ids, ctrl, p, pixel, sample);
-  const uint64_t key = lam_path_key(pa.seed, pixel, sample);
-  Lam<H> lam;
-  const float inv_om = lam_for<H>(sc, key, lam);
-  float X = 0.f, Y = 0.f, Z = 0.f;
+  const uint32_t npx = ids.npix ? ids.npix : n;
+  if (t >= npx) return;
+  for (uint32_t p = t; p < n; p += npx) {
+    uint32_t pixel, sample;
+    path_id(ids, ctrl, p, pixel, sample);
+    const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+    Lam<H> lam;
+    const float inv_om = lam_for<H>(sc, key, lam);
+    float X = 0.f, Y = 0.f, Z = 0.f;
 #pragma unroll
-  for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z);
-  film_add(film, pixel, X, Y, Z);
+    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z);
+    film_add(film, pixel, X, Y, Z);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -976,16 +998,17 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
 hipError_t enqueue_pass(int hero, const DevScene &sc, const PathArgs &pa, const uint32_t *pixel,
                         const uint32_t *sample, uint32_t pixel_offset, const WaveState &ws, uint32_t n,
                         uint32_t *ctrl, float *film, float *out, const WaveLaunch &wl, hipStream_t st,
-                        WaveProfile *prof, const GuideDev *gd, bool train) {
-  PathIds ids{pixel, sample, pixel_offset};
+                        WaveProfile *prof, const GuideDev *gd, bool train, uint32_t m) {
+  PathIds ids{pixel, sample, pixel_offset, m > 1 ? n / m : 0u};
   if (hero == 16) return enqueue_pass_t<16>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof, gd, train);
   return enqueue_pass_t<4>(sc, pa, ids, ws, n, ctrl, film, out, wl, st, prof, gd, train);
 }
 
 hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset,
-                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st) {
-  const PathIds ids{nullptr, nullptr, pixel_offset};
-  const uint32_t b256 = (n + 255) / 256;
+                        const WaveState &ws, uint32_t n, const uint32_t *ctrl, float *film, hipStream_t st,
+                        uint32_t m) {
+  const PathIds ids{nullptr, nullptr, pixel_offset, m > 1 ? n / m : 0u};
+  const uint32_t b256 = (n / m + 255) / 256;
   if (hero == 16)
     hipLaunchKernelGGL((k_film<16>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
   else
