@@ -42,11 +42,15 @@ Also reported:
                 complete gradients per second at that sample count.
   guiding       the learned proposal on the glossy interior (200 instanced
                 2,352-tri rounded boxes with Blinn lobes, room shell, one
-                hidden light; 470,424 tris), 1024x1024, 32 bounces, p=0.8:
-                two independent plain renders and two independent guided
+                light hidden on a rimmed cabinet top, facing the ceiling
+                corner; 470,448 tris), 1024x1024, 32 bounces, p=0.8: two
+                independent plain renders and two independent guided
                 renders (each with a fresh network trained online on its
-                first 32 passes, training time included; the proposal is
-                used at the first 4 path vertices) of the same spp.
+                first 32 passes, 128 Adam steps after each, training time
+                included; the proposal, α = 0.6, stands in for the
+                Lambertian and Blinn Kd lobes at every vertex) of the same
+                spp.  --guide-layout plate: the light under the ceiling
+                above a plate instead.
                 relMSE = mean over pixels of Var(Y) / (Ȳ² + 0.01 mean(Ȳ)²),
                 Y = luminance, Var estimated from the two runs (both
                 estimators are unbiased); time to relMSE 1e-3 =
@@ -361,16 +365,16 @@ def adjoint_bench(spp=64, bounces=16, p=0.8, seed=3):
             "grad_finite": bool(np.isfinite(grad).all())}
 
 
-def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, train_passes=32, max_vertex=0,
-                  steps_per_pass=8, batch=16384, materials=0, lobes=0):
+def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.6, train_passes=32, max_vertex=0,
+                  steps_per_pass=128, batch=16384, materials=0, lobes=0, layout="cabinet"):
     """Learned proposal on the glossy interior: two independent plain runs
     and two guided runs of `spp` samples per pixel (the guided runs train
     the network online on their first `train_passes` passes, inside the timed
     call); relMSE of one run from the difference of its two runs."""
-    d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c4_seed{seed}_{width}")
+    d = os.path.join(tempfile.gettempdir(), f"lambda_amd_c4_{layout}_seed{seed}_{width}")
     path = os.path.join(d, "glossy_interior.nls")
     if not os.path.exists(path):
-        scenes.glossy_interior(d, seed=seed, width=width)
+        scenes.glossy_interior(d, seed=seed, width=width, layout=layout)
     scene = la.load_nls(path)
     ctx = la.Context(0)
     ctx.upload(scene)
@@ -404,8 +408,11 @@ def guiding_bench(spp=1024, bounces=32, p=0.8, seed=4, width=1024, alpha=0.5, tr
     t_p, t_g = float(np.mean(tp)), float(np.mean(tg))
     ctx.close()
     samples = scene.width * scene.height * spp
+    hidden = {"cabinet": "light on a rimmed cabinet top facing the ceiling corner",
+              "plate": "light under the ceiling above a plate"}[layout]
     return {"workload": f"glossy interior ({scene.n_tris:,} tris: room shell + 200 Blinn rounded boxes, "
-                        f"hidden light), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
+                        f"{hidden}), {scene.width}x{scene.height}, {bounces} bounces, p={p}, hero 4",
+            "layout": layout,
             "spp_per_run": spp, "train_passes": train_passes, "alpha": alpha, "max_vertex": max_vertex,
             "lobes": lobes or 3,
             "steps_per_pass": steps_per_pass, "batch": batch,
@@ -460,7 +467,7 @@ def stress_bench(spp=256, bounces=32, p=0.8, seed=5, adj_spp=32):
     same = bool(np.array_equal(ref16, gpu16))
     ctx.upload(scene)
     pr = la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=scene.hero, seed=1)
-    gp = la.guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1, max_vertex=4)
+    gp = la.guide_params(alpha=0.6, train_passes=32, steps_per_pass=128, batch=16384, lr=5e-3, seed=1)
     ctx.guide_reset(1)
     t0 = time.perf_counter()
     img = ctx.render_guided(pr, gp)
@@ -533,12 +540,14 @@ def main():
                          "render scene")
     ap.add_argument("--guide-spp", type=int, default=1024)
     ap.add_argument("--guide-train-passes", type=int, default=32)
-    ap.add_argument("--guide-steps-per-pass", type=int, default=8)
-    ap.add_argument("--guide-max-vertex", type=int, default=4,
+    ap.add_argument("--guide-steps-per-pass", type=int, default=128)
+    ap.add_argument("--guide-max-vertex", type=int, default=0,
                     help="guide path vertices of bounce index below this (0 = every vertex)")
-    ap.add_argument("--guide-lobes", type=int, default=1,
+    ap.add_argument("--guide-lobes", type=int, default=3,
                     help="guided lobes: 1 Lambertian, 2 Blinn Kd, 4 Blinn specular (0 = 1 + 2)")
-    ap.add_argument("--guide-alpha", type=float, default=0.5)
+    ap.add_argument("--guide-alpha", type=float, default=0.6)
+    ap.add_argument("--guide-layout", default="cabinet", choices=("cabinet", "plate"),
+                    help="where the glossy interior hides its light (scenes.glossy_interior)")
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="pass lanes (0 = the library's default)")
     ap.add_argument("--samples-per-pass", type=int, default=0,
@@ -573,7 +582,7 @@ def main():
             out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
                                            max_vertex=args.guide_max_vertex,
                                            steps_per_pass=args.guide_steps_per_pass, lobes=args.guide_lobes,
-                                           alpha=args.guide_alpha)
+                                           alpha=args.guide_alpha, layout=args.guide_layout)
         if "gpu_bvh" in legs:
             out["gpu_bvh"] = gpu_bvh_bench()
         print(json.dumps(out), flush=True)
@@ -641,7 +650,7 @@ def main():
     if "guiding" in legs:
         out["guiding"] = guiding_bench(spp=args.guide_spp, train_passes=args.guide_train_passes,
                                        max_vertex=args.guide_max_vertex, steps_per_pass=args.guide_steps_per_pass,
-                                       lobes=args.guide_lobes, alpha=args.guide_alpha)
+                                       lobes=args.guide_lobes, alpha=args.guide_alpha, layout=args.guide_layout)
         log(f"guiding: {out['guiding']}")
     if "stress" in legs:
         out["stress"] = stress_bench()

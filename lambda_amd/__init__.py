@@ -120,7 +120,7 @@ class GuideParams(C.Structure):
 GUIDE_LAMBERTIAN, GUIDE_BLINN_KD, GUIDE_BLINN_SPEC = 0x1, 0x2, 0x4
 
 
-def guide_params(alpha=0.5, train_passes=32, steps_per_pass=8, batch=16384, lr=5e-3, seed=1, materials=0,
+def guide_params(alpha=0.6, train_passes=32, steps_per_pass=128, batch=16384, lr=5e-3, seed=1, materials=0,
                  max_vertex=0, lobes=0):
     """lam_guide_params: guide-branch probability, online-training schedule,
     guided material types (bit 1 << MAT_*; 0 = every supported type),

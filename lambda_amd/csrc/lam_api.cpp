@@ -120,7 +120,7 @@ struct lam_ctx {
   // guiding network: fp32 master parameters, Adam moments, gradient,
   // bf16-rounded copy (one allocation), packed fragments, scalars
   bool guide_ready = false;
-  float *g_th = nullptr, *g_m = nullptr, *g_v = nullptr, *g_grad = nullptr, *g_thq = nullptr, *g_scal = nullptr;
+  float *g_th = nullptr, *g_m = nullptr, *g_v = nullptr, *g_grad = nullptr, *g_scal = nullptr;
   void *g_pack = nullptr;
   uint32_t g_step = 0;
   float *g_ws = nullptr;  // training batch workspace
@@ -466,7 +466,7 @@ void free_guide(lam_ctx *ctx) {
     if (p) (void)hipFree(p);
   for (lam_ctx::Lane &ln : ctx->lane)
     if (ln.mix) (void)hipFree(ln.mix);
-  ctx->g_th = ctx->g_m = ctx->g_v = ctx->g_grad = ctx->g_thq = ctx->g_scal = nullptr;
+  ctx->g_th = ctx->g_m = ctx->g_v = ctx->g_grad = ctx->g_scal = nullptr;
   ctx->g_pack = nullptr;
   ctx->g_ws = nullptr;
   ctx->g_ws_bytes = 0;
@@ -485,14 +485,13 @@ int ensure_guide(lam_ctx *ctx) {
   if (ctx->g_th) return LAM_OK;
   const size_t np = (size_t(lam::kGParams) + 63) / 64 * 64;
   void *p = nullptr;
-  HIPC(hipMalloc(&p, np * 4 * 5 + 64));
+  HIPC(hipMalloc(&p, np * 4 * 4 + 64));
   float *f = static_cast<float *>(p);
   ctx->g_th = f;
   ctx->g_m = f + np;
   ctx->g_v = f + 2 * np;
   ctx->g_grad = f + 3 * np;
-  ctx->g_thq = f + 4 * np;
-  ctx->g_scal = f + 5 * np;
+  ctx->g_scal = f + 4 * np;
   HIPC(hipMalloc(&ctx->g_pack, lam::kGPackBytes));
   return LAM_OK;
 }
@@ -536,7 +535,7 @@ int guide_upload_params(lam_ctx *ctx, const float *th) {
   const size_t np = (size_t(lam::kGParams) + 63) / 64 * 64;
   HIPC(hipMemcpyAsync(ctx->g_th, th, size_t(lam::kGParams) * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(lam::launch_zero(ctx->g_m, np * 8, ctx->stream));  // m and v
-  HIPC(lam::guide_pack(ctx->g_th, ctx->g_pack, ctx->g_thq, ctx->stream));
+  HIPC(lam::guide_pack(ctx->g_th, ctx->g_pack, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   ctx->g_step = 0;
   ctx->guide_ready = true;
@@ -548,8 +547,6 @@ lam::GuideTrain guide_train_state(lam_ctx *ctx, uint32_t batch, float lr, uint64
   gt.th = ctx->g_th;
   gt.m = ctx->g_m;
   gt.v = ctx->g_v;
-  gt.grad = ctx->g_grad;
-  gt.thq = ctx->g_thq;
   gt.scal = ctx->g_scal;
   gt.ws = ctx->g_ws;
   gt.pack = ctx->g_pack;
@@ -586,14 +583,15 @@ struct GuideRun {
 // LAM_FLAG_NO_GRAPH launches without a graph.
 int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix,
                      const GuideRun *gr = nullptr) {
-  // pixels per chunk, and samples per wavefront pass: a plain render of a
-  // frame that fits takes up to samples_per_pass samples of every pixel into
-  // one pass (longer queues, one tail per several samples)
+  // pixels per chunk, and samples per wavefront pass: a render of a frame
+  // that fits takes up to samples_per_pass samples of every pixel into one
+  // pass (longer queues, one tail per several samples); a guided render's
+  // training passes keep one sample each
   const size_t cpix = std::min(npix, kMaxWaveCap);
   const uint32_t m_cfg =
-      (!gr && npix <= kMaxWaveCap) ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass,
-                                                                                     kMaxPassSlots / npix)))
-                                   : 1u;
+      npix <= kMaxWaveCap
+          ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass, kMaxPassSlots / npix)))
+          : 1u;
   const size_t cap = cpix * std::min(m_cfg, std::max(p->spp, 1u));
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
@@ -1436,14 +1434,15 @@ int lam_guide_loss_grad(lam_ctx *ctx, const float *rows, size_t n, float *loss, 
   if (!ctx) return LAM_E_INVALID;
   int rc;
   if ((rc = guide_rows_in(ctx, rows, n))) return rc;
-  HIPC(lam::guide_grad_rows(ctx->g_th, ctx->g_thq, static_cast<const float *>(ctx->scratch[0]), uint32_t(n),
-                            ctx->g_ws, ctx->g_grad, ctx->g_scal, ctx->stream));
+  if (raw && (rc = ensure_scratch(ctx, 1, n * lam::kGOut * 4))) return rc;
+  lam::GuideTrain gt = guide_train_state(ctx, uint32_t(n), 0.f, 0);
+  float *d_raw = raw ? static_cast<float *>(ctx->scratch[1]) : nullptr;
+  HIPC(lam::guide_loss_grad_rows(gt, static_cast<const float *>(ctx->scratch[0]), uint32_t(n), ctx->g_grad, d_raw,
+                                 ctx->stream));
   float sc[2] = {0.f, 0.f};
   HIPC(hipMemcpyAsync(sc, ctx->g_scal, 8, hipMemcpyDeviceToHost, ctx->stream));
   if (grad) HIPC(hipMemcpyAsync(grad, ctx->g_grad, size_t(lam::kGParams) * 4, hipMemcpyDeviceToHost, ctx->stream));
-  if (raw)
-    HIPC(hipMemcpyAsync(raw, lam::guide_train_raw(ctx->g_ws, uint32_t(n)), n * lam::kGOut * 4,
-                        hipMemcpyDeviceToHost, ctx->stream));
+  if (raw) HIPC(hipMemcpyAsync(raw, d_raw, n * lam::kGOut * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   if (loss) *loss = sc[1];
   return LAM_OK;

@@ -34,10 +34,15 @@ constexpr int kGW1 = 0, kGB1 = kGW1 + kGHid * kGFeat, kGW2 = kGB1 + kGHid, kGB2 
               kGW5 = kGB4 + kGHid, kGB5 = kGW5 + kGOut * kGHid, kGParams = kGB5 + kGOut;
 // packed weight fragments: layer 1 (4 row tiles x 5 K chunks), layers 2-4
 // (4 x 2 each), layer 5 (3 x 2); 64 lanes x 16 bytes each; then the biases
-// as fp32 (64 x 4 + 48)
+// as fp32 (64 x 4 + 48); then the training backward's fragments of the
+// transposed weights of layers 2-5 (4 input tiles x 2 K chunks each; K runs
+// over the layer's outputs, permuted as the forward's B operand for layers
+// 2-4, in natural order for layer 5)
 constexpr int kGFragL1 = 0, kGFragL2 = 20, kGFragL5 = 44, kGFrags = 50;
 constexpr int kGBiasWords = 4 * kGHid + kGOutPad;
-constexpr size_t kGPackBytes = size_t(kGFrags) * 64 * 16 + size_t(kGBiasWords) * 4;
+constexpr int kGBFragL2 = 0, kGBFragL5 = 24, kGBFrags = 32;
+constexpr size_t kGPackFwdBytes = size_t(kGFrags) * 64 * 16 + size_t(kGBiasWords) * 4;
+constexpr size_t kGPackBytes = kGPackFwdBytes + size_t(kGBFrags) * 64 * 16;
 constexpr float kGLogKMin = -4.f, kGLogKMax = 9.f;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -62,6 +67,10 @@ __host__ __device__ inline float bfround(float f) { return bf2f(f2bf(f)); }
 __host__ __device__ inline int g_perm(int k) {
   const int c = k >> 5, g = (k >> 3) & 3, j = k & 7;
   return 32 * c + 16 * (j >> 2) + 4 * g + (j & 3);
+}
+// its inverse: the K position that holds neuron k
+__host__ __device__ inline int g_perm_inv(int k) {
+  return 32 * (k >> 5) + 8 * ((k >> 2) & 3) + 4 * ((k >> 4) & 1) + (k & 3);
 }
 
 // sin and cos of 2π fr for fr in [0, 1): reduction to the nearest quarter

@@ -8,13 +8,22 @@
 // loss is the weighted negative log-likelihood of the mixture,
 //   loss = -Σ_i (w_i / Σw) log p_g(ω_i | x_i),
 // whose minimiser is the mixture closest in KL divergence to the sampled
-// distribution of radiance (times the plain continuation's weight).  Every
-// GEMM of the forward and backward passes runs on bf16 MFMA with fp32
-// accumulation (k_gemm_mfma): the forward reproduces the inference network
-// (bf16 weights, features and activations); the backward rounds dZ to bf16
-// where it enters a product and treats the forward's bf16 roundings as
-// identity.  Adam updates fp32 master parameters, which are then repacked
-// into bf16 fragments.
+// distribution of radiance (times the plain continuation's weight).
+//
+// One Adam step is two kernels.  k_guide_fwdbwd runs the forward and the
+// backward of 64 batch rows per wave, entirely on bf16 MFMA with fp32
+// accumulation: the forward is the inference network's own evaluation (the
+// same fragments, features and ReLU packing, so its outputs are the
+// inference outputs bit for bit); the backward takes each layer's dZ,
+// rounded to bf16, as an MFMA operand in registers for dX = dZ W (against
+// fragments of the transposed weights) and, through a transpose in LDS, for
+// dW = dZᵀ X over the wave's 64 rows; the ReLU masks are kept from the
+// forward as bits.  Each block (one wave, weights and activations in LDS)
+// writes its gradient sums to a partial buffer.  k_guide_update then sums the
+// partials in a fixed order, applies Adam to the fp32 master parameters and
+// writes each updated parameter into the packed bf16 fragments in place; its
+// threads also draw the next step's batch from the records.  Sums are in a
+// fixed order throughout, so training is deterministic.
 #include <algorithm>
 
 #include "lam_guide.h"
@@ -22,15 +31,59 @@
 namespace lam {
 
 // ---- packing ------------------------------------------------------------------------
-__global__ void k_guide_pack(const float *__restrict__ th, uint4 *__restrict__ frag, float *__restrict__ bias,
-                             float *__restrict__ thq) {
+// one parameter into its packed places: forward fragment (or bias) and, for
+// the weights of layers 2-5, the backward's transposed fragment
+__device__ inline void pack_param(uint4 *__restrict__ pack, int p, float w) {
+  uint16_t *h = reinterpret_cast<uint16_t *>(pack);
+  float *bias = reinterpret_cast<float *>(pack + kGFrags * 64);
+  uint16_t *hb = reinterpret_cast<uint16_t *>(reinterpret_cast<char *>(pack) + kGPackFwdBytes);
+  const uint16_t b = f2bf(w);
+  auto put = [](uint16_t *base, int f, int g, int n, int j, uint16_t v) { base[(f * 64 + 16 * g + n) * 8 + j] = v; };
+  if (p < kGB1) {
+    const int row = p / kGFeat, col = p % kGFeat;
+    put(h, kGFragL1 + (row / 16) * 5 + col / 32, (col % 32) / 8, row % 16, col % 8, b);
+  } else if (p < kGW2) {
+    bias[p - kGB1] = w;
+  } else if (p < kGW5) {
+    const int l = (p - kGW2) / (kGHid * kGHid + kGHid), idx = (p - kGW2) % (kGHid * kGHid + kGHid);
+    if (idx >= kGHid * kGHid) {
+      bias[kGHid * (l + 1) + idx - kGHid * kGHid] = w;
+      return;
+    }
+    const int row = idx / kGHid, col = idx % kGHid;
+    const int kf = g_perm_inv(col), kb = g_perm_inv(row);
+    put(h, kGFragL2 + 8 * l + 2 * (row / 16) + kf / 32, (kf >> 3) & 3, row % 16, kf & 7, b);
+    put(hb, kGBFragL2 + 8 * l + 2 * (col / 16) + kb / 32, (kb >> 3) & 3, col % 16, kb & 7, b);
+  } else if (p < kGB5) {
+    const int row = (p - kGW5) / kGHid, col = (p - kGW5) % kGHid;
+    const int kf = g_perm_inv(col);
+    put(h, kGFragL5 + 2 * (row / 16) + kf / 32, (kf >> 3) & 3, row % 16, kf & 7, b);
+    put(hb, kGBFragL5 + 2 * (col / 16) + row / 32, (row % 32) / 8, col % 16, row % 8, b);
+  } else {
+    bias[4 * kGHid + p - kGB5] = w;
+  }
+}
+
+// the whole pack from fp32 parameters, padding included (zero)
+__global__ void k_guide_pack(const float *__restrict__ th, uint4 *__restrict__ pack) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid < uint32_t(kGFrags) * 64) {
-    const uint32_t f = tid / 64, lane = tid % 64, n = lane % 16, g = lane / 16;
+  uint4 *bpack = reinterpret_cast<uint4 *>(reinterpret_cast<char *>(pack) + kGPackFwdBytes);
+  if (tid < uint32_t(kGFrags + kGBFrags) * 64) {
+    const bool fwd = tid < uint32_t(kGFrags) * 64;
+    const uint32_t f = fwd ? tid / 64 : tid / 64 - kGFrags, lane = tid % 64, n = lane % 16, g = lane / 16;
     uint16_t v[8];
     for (int j = 0; j < 8; j++) {
       float w = 0.f;
-      if (f < uint32_t(kGFragL2)) {  // layer 1: natural feature order
+      if (!fwd) {  // transposed weights: row tiles over inputs, K over outputs
+        const uint32_t l = f / 8, t = (f % 8) / 2, c = f % 2, in = 16 * t + n;
+        if (f < uint32_t(kGBFragL5)) {
+          const int base = l == 0 ? kGW2 : (l == 1 ? kGW3 : kGW4);
+          w = th[base + g_perm(int(32 * c + 8 * g + j)) * kGHid + in];
+        } else {
+          const uint32_t out = 32 * c + 8 * g + j;
+          if (out < uint32_t(kGOut)) w = th[kGW5 + out * kGHid + in];
+        }
+      } else if (f < uint32_t(kGFragL2)) {  // layer 1: natural feature order
         const uint32_t t = f / 5, c = f % 5, row = 16 * t + n, col = 32 * c + 8 * g + j;
         if (col < uint32_t(kGFeat)) w = th[kGW1 + row * kGFeat + col];
       } else if (f < uint32_t(kGFragL5)) {
@@ -45,8 +98,9 @@ __global__ void k_guide_pack(const float *__restrict__ th, uint4 *__restrict__ f
       }
       v[j] = f2bf(w);
     }
-    frag[tid] = make_uint4(v[0] | (uint32_t(v[1]) << 16), v[2] | (uint32_t(v[3]) << 16),
-                           v[4] | (uint32_t(v[5]) << 16), v[6] | (uint32_t(v[7]) << 16));
+    (fwd ? pack : bpack)[fwd ? tid : tid - uint32_t(kGFrags) * 64] =
+        make_uint4(v[0] | (uint32_t(v[1]) << 16), v[2] | (uint32_t(v[3]) << 16), v[4] | (uint32_t(v[5]) << 16),
+                   v[6] | (uint32_t(v[7]) << 16));
   }
   if (tid < uint32_t(kGBiasWords)) {
     float b = 0.f;
@@ -57,14 +111,8 @@ __global__ void k_guide_pack(const float *__restrict__ th, uint4 *__restrict__ f
     } else if (tid - 4u * kGHid < uint32_t(kGOut)) {
       b = th[kGB5 + (tid - 4 * kGHid)];
     }
-    bias[tid] = b;
+    reinterpret_cast<float *>(pack + kGFrags * 64)[tid] = b;
   }
-  // bf16-rounded copy of every parameter for the training passes, which take
-  // the weights from it and the (fp32) biases from th.  (A per-range select
-  // between th and its rounding here was compiled for gfx950 with the store
-  // address of the bias ranges clobbered to the first lane's; a plain copy
-  // has no such select.)
-  if (tid < uint32_t(kGParams)) thq[tid] = bfround(th[tid]);
 }
 
 // ---- batched inference (test hook) ------------------------------------------------------
@@ -103,6 +151,12 @@ __global__ __launch_bounds__(256) void k_guide_infer(const uint4 *__restrict__ p
 }
 
 // ---- training ---------------------------------------------------------------------------
+constexpr uint32_t kGTrainBlocks = 256;  // blocks (one wave each) of k_guide_fwdbwd
+constexpr int kGParamsPad = (kGParams + 63) / 64 * 64;
+constexpr int kGRow = 10;  // batch row: in[6], ω[3], w
+constexpr int kXS = 72;    // LDS row stride (halves) of transposed activations
+constexpr int kZS = 49;    // LDS row stride (floats) of the output exchange
+
 __device__ inline uint32_t hash32(uint64_t x) {
   x ^= x >> 33;
   x *= 0xff51afd7ed558ccdull;
@@ -112,174 +166,76 @@ __device__ inline uint32_t hash32(uint64_t x) {
   return uint32_t(x);
 }
 
-// training batch from records: features (bf16-rounded) and (ω, w)
-__global__ void k_guide_gather(const float4 *__restrict__ rec, const uint32_t *__restrict__ rec_count,
-                               uint32_t rec_cap, uint32_t B, uint64_t seed, float *__restrict__ X0,
-                               float4 *__restrict__ tgt) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B) return;
-  const uint32_t nr = min(*rec_count, rec_cap);
-  float in[kGIn];
-  float4 t = make_float4(0.f, 0.f, 1.f, 0.f);
-  if (nr == 0) {
-    for (int d = 0; d < kGIn; d++) in[d] = 0.f;
-  } else {
-    const uint32_t r = uint32_t((uint64_t(hash32(seed * 0x9E3779B97F4A7C15ull + i)) * nr) >> 32);
-    const float4 a = rec[4 * size_t(r)], b = rec[4 * size_t(r) + 1], c = rec[4 * size_t(r) + 2],
-                 d = rec[4 * size_t(r) + 3];
-    in[0] = a.x; in[1] = a.y; in[2] = a.z; in[3] = a.w; in[4] = b.x; in[5] = b.y;
-    t = make_float4(c.x, c.y, c.z, d.z);
-  }
-  for (int f = 0; f < kGFeatPad; f++) X0[size_t(i) * kGFeatPad + f] = bfround(g_feature(f, in));
-  tgt[i] = t;
-}
-
-// training batch from host rows (in[6], ω[3], w)
-__global__ void k_guide_feat_rows(const float *__restrict__ rows, uint32_t B, float *__restrict__ X0,
-                                  float4 *__restrict__ tgt) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B) return;
-  const float *r = rows + size_t(i) * 10;
-  float in[kGIn];
-  for (int d = 0; d < kGIn; d++) in[d] = r[d];
-  for (int f = 0; f < kGFeatPad; f++) X0[size_t(i) * kGFeatPad + f] = bfround(g_feature(f, in));
-  tgt[i] = make_float4(r[6], r[7], r[8], r[9]);
-}
-
-// C[M x N] (+)= A[M x K] B[K x N] on v_mfma_f32_16x16x32_bf16: A and B are
-// read as fp32, rounded to bf16 (nearest even) into LDS, and their products
-// summed in fp32.  A(m,k) = TA ? A[k lda + m] : A[m lda + k],
-// B(k,n) = TB ? B[n ldb + k] : B[k ldb + n].  64 x 64 block tiles, four waves
-// of 32 x 32 (2 x 2 MFMA tiles), K steps of 32; blockIdx.z splits K into
-// chunks of `kchunk` (a multiple of 32) whose partial sums are added with
-// atomics.  The operands of the network's layers are bf16 already
-// (weights, features, activations: exact); the backward's dZ is rounded.
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm_mfma(uint32_t M, uint32_t N, uint32_t K, const float *__restrict__ A,
-                                                   uint32_t lda, const float *__restrict__ Bm, uint32_t ldb,
-                                                   float *__restrict__ C, uint32_t ldc, uint32_t kchunk, int atomic) {
-  // [row][k] bf16; rows of 40 halves (80 B) keep every 8-half group 16-byte aligned
-  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];
-  __shared__ __attribute__((aligned(16))) uint16_t Bt[64][40];
-  const uint32_t tid = threadIdx.x, wave = tid / 64, lane = tid % 64, ln = lane % 16, lg = lane / 16;
-  const uint32_t wm = wave / 2, wn = wave % 2;
-  const uint32_t m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const uint32_t k_begin = blockIdx.z * kchunk, k_end = min(K, k_begin + kchunk);
-  f32x4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (uint32_t k0 = k_begin; k0 < k_end; k0 += 32) {
-    // 64 x 32 elements of each operand, 8 per thread; the global index that
-    // varies fastest follows the operand's storage order
-    for (uint32_t e = tid; e < 64u * 32u; e += 256u) {
-      const uint32_t am = TA ? e % 64u : e / 32u, ak = TA ? e / 64u : e % 32u;
-      const uint32_t gm = m0 + am, gk = k0 + ak;
-      float av = 0.f;
-      if (gm < M && gk < k_end) av = TA ? A[size_t(gk) * lda + gm] : A[size_t(gm) * lda + gk];
-      As[am][ak] = f2bf(av);
-      const uint32_t bn = TB ? e / 32u : e % 64u, bk = TB ? e % 32u : e / 64u;
-      const uint32_t gn = n0 + bn, gk2 = k0 + bk;
-      float bv = 0.f;
-      if (gn < N && gk2 < k_end) bv = TB ? Bm[size_t(gn) * ldb + gk2] : Bm[size_t(gk2) * ldb + gn];
-      Bt[bn][bk] = f2bf(bv);
+// Batch row i of a training step drawn from the records (uniform, with
+// replacement), and the batch's Σw per block of 256 rows (a fixed-order tree
+// in LDS).  Every thread of a block must call it.
+struct Gather {
+  const float4 *rec;
+  const uint32_t *rec_count;
+  uint32_t rec_cap, B;
+  uint64_t seed;
+  float *rows, *wpart;
+};
+__device__ inline void gather_row(const Gather &ga, uint32_t i, float *red) {
+  float w = 0.f;
+  if (i < ga.B) {
+    const uint32_t nr = min(*ga.rec_count, ga.rec_cap);
+    float v[kGRow] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f};
+    if (nr > 0) {
+      const uint32_t r = uint32_t((uint64_t(hash32(ga.seed * 0x9E3779B97F4A7C15ull + i)) * nr) >> 32);
+      const float4 a = ga.rec[4 * size_t(r)], b = ga.rec[4 * size_t(r) + 1], c = ga.rec[4 * size_t(r) + 2],
+                   d = ga.rec[4 * size_t(r) + 3];
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+      v[6] = c.x; v[7] = c.y; v[8] = c.z; v[9] = d.z;
     }
-    __syncthreads();
-    // lane (n, g) supplies A[n][8g .. 8g+7] and B[8g .. 8g+7][n]
-#pragma unroll
-    for (int mi = 0; mi < 2; mi++) {
-      const uint4 a = *reinterpret_cast<const uint4 *>(&As[32 * wm + 16 * mi + ln][8 * lg]);
-#pragma unroll
-      for (int ni = 0; ni < 2; ni++) {
-        const uint4 b = *reinterpret_cast<const uint4 *>(&Bt[32 * wn + 16 * ni + ln][8 * lg]);
-        acc[mi][ni] = mfma16(a, b, acc[mi][ni]);
-      }
-    }
-    __syncthreads();
+    for (int k = 0; k < kGRow; k++) ga.rows[size_t(i) * kGRow + k] = v[k];
+    w = v[9];
   }
-  // lane (n, g) holds D[4g + i][n] of each tile
-#pragma unroll
-  for (int mi = 0; mi < 2; mi++)
-#pragma unroll
-    for (int ni = 0; ni < 2; ni++)
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const uint32_t m = m0 + 32 * wm + 16 * mi + 4 * lg + i, n = n0 + 32 * wn + 16 * ni + ln;
-        if (m < M && n < N) {
-          if (atomic) atomicAdd(C + size_t(m) * ldc + n, acc[mi][ni][i]);
-          else C[size_t(m) * ldc + n] = acc[mi][ni][i];
-        }
-      }
-}
-
-// Z[b][j] += bias[j]; X = bf16(relu(Z)) (hidden layers) or X = Z (output)
-__global__ void k_bias_act(float *__restrict__ Z, const float *__restrict__ bias, uint32_t B, uint32_t N,
-                           float *__restrict__ X, int relu) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= B * N) return;
-  const float z = Z[e] + bias[e % N];
-  Z[e] = z;
-  if (X) X[e] = relu ? bfround(fmaxf(z, 0.f)) : z;
-}
-
-// dZ = dX where Z > 0, else 0
-__global__ void k_relu_grad(const float *__restrict__ dX, const float *__restrict__ Z, uint32_t n,
-                            float *__restrict__ dZ) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) dZ[e] = Z[e] > 0.f ? dX[e] : 0.f;
-}
-
-// db[j] += Σ_b dZ[b][j]
-__global__ void k_colsum(const float *__restrict__ dZ, uint32_t B, uint32_t N, float *__restrict__ db) {
-  const uint32_t j = blockIdx.x, t = threadIdx.x;
-  float s = 0.f;
-  for (uint32_t b = blockIdx.y * 256 + t; b < B; b += 256 * gridDim.y) s += dZ[size_t(b) * N + j];
-  __shared__ float red[256];
-  red[t] = s;
-  __syncthreads();
-  for (uint32_t w = 128; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
-  }
-  if (t == 0) atomicAdd(db + j, red[0]);
-}
-
-__global__ void k_sum_w(const float4 *__restrict__ tgt, uint32_t B, float *__restrict__ out) {
-  __shared__ float red[256];
   const uint32_t t = threadIdx.x;
-  float s = 0.f;
-  for (uint32_t b = blockIdx.x * 256 + t; b < B; b += 256 * gridDim.x) s += tgt[b].w;
-  red[t] = s;
+  red[t] = w;
   __syncthreads();
-  for (uint32_t w = 128; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
     __syncthreads();
   }
-  if (t == 0) atomicAdd(out, red[0]);
+  if (t == 0 && blockIdx.x * 256 < ga.B) ga.wpart[blockIdx.x] = red[0];
 }
 
-// per sample: loss term -w/Σw log p_g(ω) and its gradient with respect to the
-// 40 raw outputs.  scal[0] = Σw (input), scal[1] += loss.
-__global__ void k_guide_loss(const float *__restrict__ raw, const float4 *__restrict__ tgt, uint32_t B,
-                             float *__restrict__ scal, float *__restrict__ dZ) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B) return;
-  const float *r = raw + size_t(i) * kGOut;
-  float *d = dZ + size_t(i) * kGOut;
-  const float4 t = tgt[i];
-  const float sw = scal[0];
-  const float wn = sw > 0.f ? t.w / sw : 0.f;
-  if (wn == 0.f) {
-    for (int k = 0; k < kGOut; k++) d[k] = 0.f;
-    return;
+__global__ __launch_bounds__(256) void k_guide_gather(Gather ga) {
+  __shared__ float red[256];
+  gather_row(ga, blockIdx.x * 256 + threadIdx.x, red);
+}
+
+// Σw per block of 256 rows of a host batch
+__global__ __launch_bounds__(256) void k_guide_rows_w(const float *__restrict__ rows, uint32_t B,
+                                                      float *__restrict__ wpart) {
+  __shared__ float red[256];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x, t = threadIdx.x;
+  red[t] = i < B ? rows[size_t(i) * kGRow + 9] : 0.f;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
   }
-  const f3 om = mk(t.x, t.y, t.z);
+  if (t == 0) wpart[blockIdx.x] = red[0];
+}
+
+// bf16 half j (0..7) of a packed fragment word
+__device__ inline uint16_t half_of(const uint4 &v, int j) {
+  const uint32_t w = j < 2 ? v.x : (j < 4 ? v.y : (j < 6 ? v.z : v.w));
+  return uint16_t((j & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+// Loss term of one row and its gradient with respect to the 40 raw outputs
+// (r in, d out), scaled by wn = w / Σw; returns -wn log p.
+__device__ inline float row_loss(const float *r, float wn, f3 om, float *d) {
   float a_max = -3.4e38f;
+#pragma unroll
   for (int L = 0; L < kGLobes; L++) a_max = fmaxf(a_max, r[5 * L + 4]);
   float ws[kGLobes], lv[kGLobes], kap[kGLobes], md[kGLobes], vlen[kGLobes];
   f3 mu[kGLobes];
   float wsum = 0.f;
+#pragma unroll
   for (int L = 0; L < kGLobes; L++) {
     ws[L] = expf(r[5 * L + 4] - a_max);
     wsum += ws[L];
@@ -294,14 +250,16 @@ __global__ void k_guide_loss(const float *__restrict__ raw, const float4 *__rest
   }
   // log p = logsumexp(log w_L + log vMF_L); responsibilities
   float zmax = -3.4e38f, z[kGLobes];
+#pragma unroll
   for (int L = 0; L < kGLobes; L++) {
     z[L] = logf(ws[L] / wsum) + lv[L];
     zmax = fmaxf(zmax, z[L]);
   }
   float zs = 0.f;
+#pragma unroll
   for (int L = 0; L < kGLobes; L++) zs += expf(z[L] - zmax);
   const float logp = zmax + logf(zs);
-  atomicAdd(scal + 1, -wn * logp);
+#pragma unroll
   for (int L = 0; L < kGLobes; L++) {
     const float resp = expf(z[L] - logp), wl = ws[L] / wsum;
     const float k = kap[L];
@@ -318,101 +276,422 @@ __global__ void k_guide_loss(const float *__restrict__ raw, const float4 *__rest
     d[5 * L + 3] = -wn * g_k;
     d[5 * L + 4] = -wn * (resp - wl);
   }
+  return -wn * logp;
 }
 
-__global__ void k_adam(float *__restrict__ th, float *__restrict__ m, float *__restrict__ v,
-                       const float *__restrict__ g, uint32_t n, float lr, float b1, float b2, float eps, float c1,
-                       float c2) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  th[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+// Store (the block's first chunk) or add a batch of gradient sums at
+// bp[idx[u]] (idx < 0: none); a batch's loads are issued before its adds (a
+// block is one wave, with nothing else to hide their latency).
+template <int N>
+__device__ inline void put_n(float *__restrict__ bp, bool first, const int (&idx)[N], const float (&val)[N]) {
+  float old[N];
+#pragma unroll
+  for (int u = 0; u < N; u++) old[u] = (!first && idx[u] >= 0) ? bp[idx[u]] : 0.f;
+#pragma unroll
+  for (int u = 0; u < N; u++)
+    if (idx[u] >= 0) bp[idx[u]] = first ? val[u] : old[u] + val[u];
+}
+
+// Forward and backward of the batch `rows` (B rows, kGRow floats each), 64
+// rows per block of four waves: wave G carries rows 16G .. 16G+15 of the
+// chunk through the forward and the dX products (its 16 MFMA columns), and
+// output row tile G of every dW product (over all 64 rows, from LDS).
+// Block b takes chunks b, b + grid, ... and writes its gradient sums to
+// part[b][·] (stored by its first chunk, added by the others) and its loss
+// sum to lpart[b].  Σw is the fixed-order sum of wpart[0 .. nw-1]; block 0
+// stores it in scal[0].  raw (optional): the forward's outputs, B x 40.
+__global__ __launch_bounds__(256) void k_guide_fwdbwd(const uint4 *__restrict__ pack, const float *__restrict__ rows,
+                                                      uint32_t B, const float *__restrict__ wpart, uint32_t nw,
+                                                      float *__restrict__ part, float *__restrict__ lpart,
+                                                      float *__restrict__ scal, float *__restrict__ raw) {
+  __shared__ uint4 sfrag[kGFrags * 64];
+  __shared__ uint4 sbfrag[kGBFrags * 64];
+  __shared__ float sbias[kGBiasWords];
+  // X1ᵀ..X4ᵀ as [neuron][row] (bf16); X0ᵀ (160 features) once the others are used
+  __shared__ __attribute__((aligned(16))) uint16_t xt[4 * 64 * kXS];
+  __shared__ __attribute__((aligned(16))) uint16_t dzt[64 * kXS];  // bf16(dZ)ᵀ of the current layer
+  __shared__ float zs[64 * kZS];                                    // layer-5 outputs, then dZ5, per row
+  __shared__ float dbs[4 * 64];                                     // per-wave bias-gradient sums
+  __shared__ float s_sw;
+  const uint32_t tid = threadIdx.x, G = tid >> 6, lane = tid & 63u, n = lane & 15u, g = lane >> 4;
+  {
+    const float *gb = reinterpret_cast<const float *>(pack + kGFrags * 64);
+    const uint4 *bf = reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pack) + kGPackFwdBytes);
+    auto stage = [&](const uint4 *src, uint4 *dst, uint32_t cnt) {  // 8 loads in flight per thread
+      for (uint32_t i0 = 0; i0 < cnt; i0 += 8 * 256) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const uint32_t i = i0 + 256 * u + tid;
+          v[u] = i < cnt ? src[i] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const uint32_t i = i0 + 256 * u + tid;
+          if (i < cnt) dst[i] = v[u];
+        }
+      }
+    };
+    stage(pack, sfrag, uint32_t(kGFrags) * 64);
+    stage(bf, sbfrag, uint32_t(kGBFrags) * 64);
+    for (uint32_t i = tid; i < uint32_t(kGBiasWords); i += 256) sbias[i] = gb[i];
+    if (tid == 0) {
+      float sw = 0.f;  // in index order; the loads 16 at a time
+      for (uint32_t k0 = 0; k0 < nw; k0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = k0 + u < nw ? wpart[k0 + u] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; u++) sw += v[u];
+      }
+      s_sw = sw;
+      if (blockIdx.x == 0) scal[0] = sw;
+    }
+  }
+  __syncthreads();
+  const float sw = s_sw;
+  float *bp = part + size_t(blockIdx.x) * kGParamsPad;
+  float lsum = 0.f;
+  bool first = true;
+  for (uint32_t chunk = blockIdx.x; chunk * 64u < B; chunk += gridDim.x, first = false) {
+    const uint32_t rq = 16 * G + n;  // this lane's row in the chunk (shared by the lanes g)
+    const uint32_t qi = chunk * 64u + rq;
+    float in[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    f3 om = mk(0.f, 0.f, 1.f);
+    float w = 0.f;
+    if (qi < B) {
+      const float *r = rows + size_t(qi) * kGRow;
+#pragma unroll
+      for (int d = 0; d < kGIn; d++) in[d] = r[d];
+      om = mk(r[6], r[7], r[8]);
+      w = r[9];
+    }
+    const GIn q{in[0], in[1], in[2], in[3], in[4], in[5]};
+    // ---- forward (mlp_wave for one column group, keeping masks and activations) ----
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) acc[t][i] = sbias[16 * t + 4 * g + i];
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const uint4 xb = feature_chunk(c, g, q);
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc[t] = mfma16(sfrag[(kGFragL1 + t * 5 + c) * 64 + lane], xb, acc[t]);
+    }
+    uint32_t msk[4];  // Z > 0 of hidden layers 1..4, bit 4 t + i
+    f32x4 o[3];
+#pragma unroll
+    for (int L = 0; L < 4; L++) {
+      uint32_t mb = 0;
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (acc[t][i] > 0.f) mb |= 1u << (4 * t + i);
+      msk[L] = mb;
+      uint4 xb[2];
+      relu_pack(acc, xb);
+      // X_{L+1}ᵀ: K position 32c + 8g + j of xb[c] is neuron g_perm of it
+      uint16_t *X = xt + L * 64 * kXS;
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) X[(32 * c + 16 * (j >> 2) + 4 * int(g) + (j & 3)) * kXS + rq] = half_of(xb[c], j);
+      if (L < 3) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) acc[t][i] = sbias[kGHid * (L + 1) + 16 * t + 4 * g + i];
+#pragma unroll
+          for (int c = 0; c < 2; c++) acc[t] = mfma16(sfrag[(kGFragL2 + 8 * L + t * 2 + c) * 64 + lane], xb[c], acc[t]);
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) o[t][i] = sbias[4 * kGHid + 16 * t + 4 * g + i];
+#pragma unroll
+          for (int c = 0; c < 2; c++) o[t] = mfma16(sfrag[(kGFragL5 + t * 2 + c) * 64 + lane], xb[c], o[t]);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = 16 * t + 4 * int(g) + i;
+        if (row < kGOut) zs[rq * kZS + row] = o[t][i];
+      }
+    __syncthreads();
+    // ---- loss, one row per lane g = 0; dZ5 = ∂loss/∂raw ----
+    if (g == 0) {
+      float r[kGOut], d[kGOut];
+#pragma unroll
+      for (int k = 0; k < kGOut; k++) r[k] = zs[rq * kZS + k];
+      if (raw && qi < B)
+#pragma unroll
+        for (int k = 0; k < kGOut; k++) raw[size_t(qi) * kGOut + k] = r[k];
+      const float wn = sw > 0.f ? w / sw : 0.f;
+      if (wn != 0.f) {
+        lsum += row_loss(r, wn, om, d);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kGOut; k++) d[k] = 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < kGOut; k++) zs[rq * kZS + k] = d[k];
+      // bf16(dZ5)ᵀ, rows 40..47 zero
+#pragma unroll
+      for (int k = 0; k < kGOutPad; k += 2) {
+        const uint32_t wd = pack_bf2(k < kGOut ? d[k] : 0.f, k + 1 < kGOut ? d[k + 1] : 0.f);
+        dzt[k * kXS + rq] = uint16_t(wd & 0xFFFFu);
+        dzt[(k + 1) * kXS + rq] = uint16_t(wd >> 16);
+      }
+    }
+    __syncthreads();
+    if (tid < uint32_t(kGOut)) {  // db5 from the unrounded dZ5
+      float s = 0.f;
+      for (int r = 0; r < 64; r++) s += zs[r * kZS + tid];
+      put_n<1>(bp, first, {kGB5 + int(tid)}, {s});
+    }
+    if (G < 3) {  // dW5 = dZ5ᵀ X4, output rows 16G .. 16G+15 (< 40)
+      f32x4 a4[4];
+#pragma unroll
+      for (int ti = 0; ti < 4; ti++) a4[ti] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(&dzt[(16 * G + n) * kXS + 32 * c + 8 * g]);
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++) {
+          const uint4 b = *reinterpret_cast<const uint4 *>(&xt[(3 * 64 + 16 * ti + n) * kXS + 32 * c + 8 * g]);
+          a4[ti] = mfma16(a, b, a4[ti]);
+        }
+      }
+      int idx[16];
+      float val[16];
+#pragma unroll
+      for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int row = 16 * int(G) + 4 * int(g) + i;
+          idx[4 * ti + i] = row < kGOut ? kGW5 + row * kGHid + 16 * ti + int(n) : -1;
+          val[4 * ti + i] = a4[ti][i];
+        }
+      put_n(bp, first, idx, val);
+    }
+    // dX4ᵀ = W5ᵀ dZ5ᵀ: B operand bf16(dZ5) of this wave's rows, outputs 32c + 8g + j
+    f32x4 dz[4];
+    {
+      uint4 zb[2];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int k = 32 * c + 8 * int(g) + j;
+          v[j] = k < kGOut ? zs[rq * kZS + k] : 0.f;
+        }
+        zb[c] = make_uint4(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]), pack_bf2(v[6], v[7]));
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        dz[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; c++) dz[t] = mfma16(sbfrag[(kGBFragL5 + 2 * t + c) * 64 + lane], zb[c], dz[t]);
+      }
+    }
+    // ---- hidden layers l = 4 .. 1: dz holds dX_l; through the ReLU to dZ_l ----
+#pragma unroll 1
+    for (int l = 4; l >= 1; l--) {
+      const uint32_t mb = msk[l - 1];
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!((mb >> (4 * t + i)) & 1u)) dz[t][i] = 0.f;
+      // db_l (unrounded): this wave's 16 rows (lanes n), then the waves in order
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          float s = dz[t][i];
+          s += __shfl_xor(s, 8, 64);
+          s += __shfl_xor(s, 4, 64);
+          s += __shfl_xor(s, 2, 64);
+          s += __shfl_xor(s, 1, 64);
+          if (n == 0) dbs[64 * G + 16 * t + 4 * g + i] = s;
+        }
+      // bf16(dZ_l): the B operand of dX_{l-1} (K permuted as the forward's)
+      // and its transpose for dW_l
+      uint4 zb[2];
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const f32x4 a = dz[2 * c], b = dz[2 * c + 1];
+        zb[c] = make_uint4(pack_bf2(a[0], a[1]), pack_bf2(a[2], a[3]), pack_bf2(b[0], b[1]), pack_bf2(b[2], b[3]));
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          dzt[(32 * c + 16 * (j >> 2) + 4 * int(g) + (j & 3)) * kXS + rq] = half_of(zb[c], j);
+      }
+      if (l == 1) {
+        // X0ᵀ (the forward's features) over the activations, which are no
+        // longer needed
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+          const uint4 f = feature_chunk(c, g, q);
+#pragma unroll
+          for (int j = 0; j < 8; j++) xt[(32 * c + 8 * int(g) + j) * kXS + rq] = half_of(f, j);
+        }
+      }
+      __syncthreads();
+      if (tid < 64) {
+        const int bb = l == 1 ? kGB1 : (l == 2 ? kGB2 : (l == 3 ? kGB3 : kGB4));
+        put_n<1>(bp, first, {bb + int(tid)}, {((dbs[tid] + dbs[64 + tid]) + dbs[128 + tid]) + dbs[192 + tid]});
+      }
+      // dW_l = dZ_lᵀ X_{l-1}, output rows 16G .. 16G+15
+      const uint16_t *Xp = l == 1 ? xt : xt + (l - 2) * 64 * kXS;
+      const int wbase = l == 1 ? kGW1 : (l == 2 ? kGW2 : (l == 3 ? kGW3 : kGW4));
+      const int nti = l == 1 ? 10 : 4, ld = l == 1 ? kGFeat : kGHid;
+      {
+        f32x4 aw[10];
+#pragma unroll
+        for (int ti = 0; ti < 10; ti++) aw[ti] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const uint4 a = *reinterpret_cast<const uint4 *>(&dzt[(16 * G + n) * kXS + 32 * c + 8 * g]);
+#pragma unroll
+          for (int ti = 0; ti < 10; ti++)
+            if (ti < nti) {
+              const uint4 b = *reinterpret_cast<const uint4 *>(&Xp[(16 * ti + n) * kXS + 32 * c + 8 * g]);
+              aw[ti] = mfma16(a, b, aw[ti]);
+            }
+        }
+        int idx[40];
+        float val[40];
+#pragma unroll
+        for (int ti = 0; ti < 10; ti++) {
+          const int col = 16 * ti + int(n);
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            idx[4 * ti + i] = ti < nti && col < ld ? wbase + (16 * int(G) + 4 * int(g) + i) * ld + col : -1;
+            val[4 * ti + i] = aw[ti][i];
+          }
+        }
+        put_n(bp, first, idx, val);
+      }
+      if (l > 1) {
+        // dX_{l-1}ᵀ = W_lᵀ dZ_lᵀ
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          f32x4 nx = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int c = 0; c < 2; c++) nx = mfma16(sbfrag[(kGBFragL2 + 8 * (l - 2) + 2 * t + c) * 64 + lane], zb[c], nx);
+          dz[t] = nx;
+        }
+      }
+      __syncthreads();  // dzt, dbs and xt are rewritten next
+    }
+  }
+  // the block's loss sum: a fixed-order tree over each wave, then the waves in order
+  lsum += __shfl_xor(lsum, 32, 64);
+  lsum += __shfl_xor(lsum, 16, 64);
+  lsum += __shfl_xor(lsum, 8, 64);
+  lsum += __shfl_xor(lsum, 4, 64);
+  lsum += __shfl_xor(lsum, 2, 64);
+  lsum += __shfl_xor(lsum, 1, 64);
+  if (lane == 0) dbs[G] = lsum;
+  __syncthreads();
+  if (tid == 0) lpart[blockIdx.x] = ((dbs[0] + dbs[1]) + dbs[2]) + dbs[3];
+}
+
+// The step's gradient (the partials summed in block order) into grad
+// (optional) and, with adam, the Adam update of parameter tid written to th
+// and into the pack; thread 0 stores the loss in scal[1].  Threads also draw
+// the next step's batch (ga.B > 0).
+__global__ __launch_bounds__(256) void k_guide_update(const float *__restrict__ part, uint32_t nblk,
+                                                      const float *__restrict__ lpart, float *__restrict__ grad,
+                                                      float *__restrict__ th, float *__restrict__ m,
+                                                      float *__restrict__ v, uint4 *__restrict__ pack, int adam,
+                                                      float lr, float b1, float b2, float eps, float c1, float c2,
+                                                      float *__restrict__ scal, Gather ga) {
+  __shared__ float red[256];
+  const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
+  if (tid < uint32_t(kGParams)) {
+    float gs = 0.f;  // in block order; the loads 16 at a time
+    for (uint32_t b0 = 0; b0 < nblk; b0 += 16) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = b0 + u < nblk ? part[size_t(b0 + u) * kGParamsPad + tid] : 0.f;
+#pragma unroll
+      for (int u = 0; u < 16; u++) gs += v[u];
+    }
+    if (grad) grad[tid] = gs;
+    if (adam) {
+      const float mi = b1 * m[tid] + (1.f - b1) * gs;
+      const float vi = b2 * v[tid] + (1.f - b2) * gs * gs;
+      m[tid] = mi;
+      v[tid] = vi;
+      const float t = th[tid] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+      th[tid] = t;
+      pack_param(pack, int(tid), t);
+    }
+  }
+  if (tid == 0) {
+    float s = 0.f;
+    for (uint32_t b = 0; b < nblk; b++) s += lpart[b];
+    scal[1] = s;
+  }
+  if (ga.B > 0 && blockIdx.x * 256 < ga.B) gather_row(ga, tid, red);
 }
 
 // ---- host-side drivers ------------------------------------------------------------------
 static inline uint32_t cdiv(size_t a, uint32_t b) { return uint32_t((a + b - 1) / b); }
 
-template <bool TA, bool TB>
-static void gemm(uint32_t M, uint32_t N, uint32_t K, const float *A, uint32_t lda, const float *B, uint32_t ldb,
-                 float *C, uint32_t ldc, bool split, hipStream_t st) {
-  uint32_t zs = 1, kchunk = (K + 31) / 32 * 32;
-  if (split) {  // K = batch: spread over about 256 blocks with atomics
-    const uint32_t tiles = cdiv(M, 64) * cdiv(N, 64);
-    zs = std::max(1u, std::min(cdiv(K, 256), 256u / std::max(tiles, 1u)));
-    kchunk = (cdiv(K, zs) + 31) / 32 * 32;
-    zs = cdiv(K, kchunk);
-  }
-  hipLaunchKernelGGL((k_gemm_mfma<TA, TB>), dim3(cdiv(N, 64), cdiv(M, 64), zs), dim3(256), 0, st, M, N, K, A, lda,
-                     B, ldb, C, ldc, kchunk, split ? 1 : 0);
-}
-
-const float *guide_train_raw(const float *ws, uint32_t B) {
-  return ws + size_t(B) * (kGFeatPad + 8 * kGHid);
+// workspace: batch rows, Σw per 256 rows, gradient and loss partials
+struct TrainWs {
+  float *rows, *wpart, *part, *lpart;
+};
+static TrainWs train_ws(float *ws, uint32_t B) {
+  TrainWs t;
+  t.rows = ws;
+  t.wpart = t.rows + (size_t(B) * kGRow + 63) / 64 * 64;
+  t.part = t.wpart + (cdiv(B, 256) + 63) / 64 * 64;
+  t.lpart = t.part + size_t(kGTrainBlocks) * kGParamsPad;
+  return t;
 }
 
 size_t guide_train_bytes(uint32_t B) {
-  return size_t(B) * (kGFeatPad + 9 * kGHid + 2 * kGOut + 4) * 4 + 64;
+  return (size_t(B) * kGRow + 64 + cdiv(B, 256) + 64 + size_t(kGTrainBlocks) * kGParamsPad + kGTrainBlocks) * 4 +
+         64;
 }
 
-// One forward + backward over a prepared batch (X0, tgt in ws): grad
-// (kGParams, zeroed here) and scal[1] (loss) are written.  Weights come from
-// thq (bf16-rounded), biases from th (fp32), as in the MFMA network.
-static void guide_fwd_bwd(const float *th, const float *thq, float *ws, uint32_t B, float *grad, float *scal,
-                          hipStream_t st) {
-  float *X0 = ws;
-  float *Z[5], *X[5];
-  float *p = X0 + size_t(B) * kGFeatPad;
-  for (int l = 0; l < 4; l++) {
-    Z[l] = p; p += size_t(B) * kGHid;
-    X[l] = p; p += size_t(B) * kGHid;
-  }
-  Z[4] = p; p += size_t(B) * kGOut;
-  float *dZo = p; p += size_t(B) * kGOut;
-  float *dA = p; p += size_t(B) * kGHid;
-  float4 *tgt = reinterpret_cast<float4 *>(p);
-  const int Wo[5] = {kGW1, kGW2, kGW3, kGW4, kGW5}, Bo[5] = {kGB1, kGB2, kGB3, kGB4, kGB5};
-  const uint32_t In[5] = {kGFeat, kGHid, kGHid, kGHid, kGHid}, Out[5] = {kGHid, kGHid, kGHid, kGHid, kGOut};
-  const uint32_t ldx[5] = {kGFeatPad, kGHid, kGHid, kGHid, kGHid};
-  // forward: Z_l = X_{l-1} W_lᵀ + b_l
-  for (int l = 0; l < 5; l++) {
-    const float *Xin = l == 0 ? X0 : X[l - 1];
-    gemm<false, true>(B, Out[l], In[l], Xin, ldx[l], thq + Wo[l], In[l], Z[l], Out[l], false, st);
-    hipLaunchKernelGGL(k_bias_act, dim3(cdiv(size_t(B) * Out[l], 256)), dim3(256), 0, st, Z[l], th + Bo[l], B,
-                       Out[l], l < 4 ? X[l] : nullptr, l < 4 ? 1 : 0);
-  }
-  (void)launch_zero(scal, 16, st);  // whole 16-byte words
-  (void)launch_zero(grad, size_t(kGParams) * 4, st);
-  hipLaunchKernelGGL(k_sum_w, dim3(std::min(cdiv(B, 256), 64u)), dim3(256), 0, st, tgt, B, scal);
-  hipLaunchKernelGGL(k_guide_loss, dim3(cdiv(B, 256)), dim3(256), 0, st, Z[4], tgt, B, scal, dZo);
-  // backward
-  const float *dZ = dZo;
-  for (int l = 4; l >= 0; l--) {
-    const float *Xin = l == 0 ? X0 : X[l - 1];
-    // dW_l = dZ_lᵀ X_{l-1}
-    gemm<true, false>(Out[l], In[l], B, dZ, Out[l], Xin, ldx[l], grad + Wo[l], In[l], true, st);
-    hipLaunchKernelGGL(k_colsum, dim3(Out[l], std::min(cdiv(B, 256), 16u)), dim3(256), 0, st, dZ, B, Out[l],
-                       grad + Bo[l]);
-    if (l > 0) {
-      // dX_{l-1} = dZ_l W_l, then through the ReLU of layer l-1
-      gemm<false, false>(B, In[l], Out[l], dZ, Out[l], thq + Wo[l], In[l], dA, kGHid, false, st);
-      // X_{l-1} is no longer needed once dW_l is formed: it takes dZ_{l-1}
-      float *dZn = X[l - 1];
-      hipLaunchKernelGGL(k_relu_grad, dim3(cdiv(size_t(B) * kGHid, 256)), dim3(256), 0, st, dA, Z[l - 1],
-                         B * kGHid, dZn);
-      dZ = dZn;
-    }
-  }
+static hipError_t fwdbwd(const GuideTrain &gt, const float *rows, uint32_t B, const float *wpart, float *part,
+                         float *lpart, float *raw, hipStream_t st, uint32_t &nblk) {
+  nblk = std::min(cdiv(B, 64), kGTrainBlocks);
+  hipLaunchKernelGGL(k_guide_fwdbwd, dim3(nblk), dim3(256), 0, st, static_cast<const uint4 *>(gt.pack), rows, B,
+                     wpart, cdiv(B, 256), part, lpart, gt.scal, raw);
+  return hipGetLastError();
 }
 
-hipError_t guide_pack(const float *th, void *pack, float *thq, hipStream_t st) {
-  uint4 *frag = static_cast<uint4 *>(pack);
-  float *bias = reinterpret_cast<float *>(frag + kGFrags * 64);
-  const uint32_t n = std::max<uint32_t>(kGFrags * 64, kGParams);
-  hipLaunchKernelGGL(k_guide_pack, dim3(cdiv(n, 256)), dim3(256), 0, st, th, frag, bias, thq);
+static hipError_t update(GuideTrain &gt, uint32_t nblk, const TrainWs &t, float *grad, bool adam, const Gather &ga,
+                         hipStream_t st) {
+  float c1 = 1.f, c2 = 1.f;
+  if (adam) {
+    gt.step++;
+    c1 = 1.f - powf(gt.b1, float(gt.step));
+    c2 = 1.f - powf(gt.b2, float(gt.step));
+  }
+  const uint32_t threads = std::max<uint32_t>(kGParams, ga.B);
+  hipLaunchKernelGGL(k_guide_update, dim3(cdiv(threads, 256)), dim3(256), 0, st, t.part, nblk, t.lpart, grad,
+                     gt.th, gt.m, gt.v, static_cast<uint4 *>(gt.pack), adam ? 1 : 0, gt.lr, gt.b1, gt.b2, gt.eps,
+                     c1, c2, gt.scal, ga);
+  return hipGetLastError();
+}
+
+hipError_t guide_pack(const float *th, void *pack, hipStream_t st) {
+  const uint32_t n = std::max<uint32_t>((kGFrags + kGBFrags) * 64, kGBiasWords);
+  hipLaunchKernelGGL(k_guide_pack, dim3(cdiv(n, 256)), dim3(256), 0, st, th, static_cast<uint4 *>(pack));
   return hipGetLastError();
 }
 
@@ -421,39 +700,43 @@ hipError_t guide_infer(const void *pack, const float *in, uint32_t n, float *raw
   return hipGetLastError();
 }
 
-hipError_t guide_grad_rows(const float *th, const float *thq, const float *rows, uint32_t B, float *ws,
-                           float *grad, float *scal, hipStream_t st) {
-  float4 *tgt = reinterpret_cast<float4 *>(ws + size_t(B) * (kGFeatPad + 8 * kGHid + 2 * kGOut + kGHid));
-  hipLaunchKernelGGL(k_guide_feat_rows, dim3(cdiv(B, 256)), dim3(256), 0, st, rows, B, ws, tgt);
-  guide_fwd_bwd(th, thq, ws, B, grad, scal, st);
+hipError_t guide_loss_grad_rows(GuideTrain &gt, const float *rows, uint32_t B, float *grad, float *raw,
+                                hipStream_t st) {
+  const TrainWs t = train_ws(gt.ws, B);
+  hipLaunchKernelGGL(k_guide_rows_w, dim3(cdiv(B, 256)), dim3(256), 0, st, rows, B, t.wpart);
+  uint32_t nblk = 0;
+  hipError_t e = fwdbwd(gt, rows, B, t.wpart, t.part, t.lpart, raw, st, nblk);
+  if (e != hipSuccess) return e;
+  return update(gt, nblk, t, grad, false, Gather{}, st);
+}
+
+hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t B, uint32_t steps, hipStream_t st) {
+  const TrainWs t = train_ws(gt.ws, B);
+  hipLaunchKernelGGL(k_guide_rows_w, dim3(cdiv(B, 256)), dim3(256), 0, st, rows, B, t.wpart);
+  for (uint32_t s = 0; s < steps; s++) {
+    uint32_t nblk = 0;
+    hipError_t e = fwdbwd(gt, rows, B, t.wpart, t.part, t.lpart, nullptr, st, nblk);
+    if (e == hipSuccess) e = update(gt, nblk, t, nullptr, true, Gather{}, st);
+    if (e != hipSuccess) return e;
+  }
   return hipGetLastError();
 }
 
 hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t *rec_count, uint32_t rec_cap,
                                uint32_t steps, hipStream_t st) {
   const uint32_t B = gt.batch;
-  float4 *tgt = reinterpret_cast<float4 *>(gt.ws + size_t(B) * (kGFeatPad + 8 * kGHid + 2 * kGOut + kGHid));
+  const TrainWs t = train_ws(gt.ws, B);
+  auto batch = [&](uint32_t step) {
+    return Gather{rec, rec_count, rec_cap, B, gt.seed ^ (uint64_t(step) << 20), t.rows, t.wpart};
+  };
+  if (steps == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_guide_gather, dim3(cdiv(B, 256)), dim3(256), 0, st, batch(gt.step + 1));
   for (uint32_t s = 0; s < steps; s++) {
-    gt.step++;
-    hipLaunchKernelGGL(k_guide_gather, dim3(cdiv(B, 256)), dim3(256), 0, st, rec, rec_count, rec_cap, B,
-                       gt.seed ^ (uint64_t(gt.step) << 20), gt.ws, tgt);
-    guide_fwd_bwd(gt.th, gt.thq, gt.ws, B, gt.grad, gt.scal, st);
-    const float c1 = 1.f - powf(gt.b1, float(gt.step)), c2 = 1.f - powf(gt.b2, float(gt.step));
-    hipLaunchKernelGGL(k_adam, dim3(cdiv(kGParams, 256)), dim3(256), 0, st, gt.th, gt.m, gt.v, gt.grad,
-                       uint32_t(kGParams), gt.lr, gt.b1, gt.b2, gt.eps, c1, c2);
-    (void)guide_pack(gt.th, gt.pack, gt.thq, st);
-  }
-  return hipGetLastError();
-}
-
-hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t B, uint32_t steps, hipStream_t st) {
-  for (uint32_t s = 0; s < steps; s++) {
-    gt.step++;
-    (void)guide_grad_rows(gt.th, gt.thq, rows, B, gt.ws, gt.grad, gt.scal, st);
-    const float c1 = 1.f - powf(gt.b1, float(gt.step)), c2 = 1.f - powf(gt.b2, float(gt.step));
-    hipLaunchKernelGGL(k_adam, dim3(cdiv(kGParams, 256)), dim3(256), 0, st, gt.th, gt.m, gt.v, gt.grad,
-                       uint32_t(kGParams), gt.lr, gt.b1, gt.b2, gt.eps, c1, c2);
-    (void)guide_pack(gt.th, gt.pack, gt.thq, st);
+    uint32_t nblk = 0;
+    hipError_t e = fwdbwd(gt, t.rows, B, t.wpart, t.part, t.lpart, nullptr, st, nblk);
+    // the update kernel draws the batch of the step after it
+    if (e == hipSuccess) e = update(gt, nblk, t, nullptr, true, s + 1 < steps ? batch(gt.step + 2) : Gather{}, st);
+    if (e != hipSuccess) return e;
   }
   return hipGetLastError();
 }

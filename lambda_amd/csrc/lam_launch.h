@@ -102,21 +102,23 @@ struct GuideDev {
   uint32_t rec_cap;
   uint32_t *rec_of_slot;  // record written by each path slot at the current vertex
 };
-// Training state (fp32 master parameters and Adam moments on the device)
+// Training state: fp32 master parameters and Adam moments, the packed
+// network (updated in place), scal[0] = Σw and scal[1] = loss of the last
+// step's batch, and a workspace of guide_train_bytes(batch)
 struct GuideTrain {
-  float *th, *m, *v, *grad, *thq, *scal, *ws;
+  float *th, *m, *v, *scal, *ws;
   void *pack;
   uint32_t batch, step;
   float lr, b1, b2, eps;
   uint64_t seed;
 };
 size_t guide_train_bytes(uint32_t batch);
-// the training forward's raw outputs (batch x 40) inside the workspace
-const float *guide_train_raw(const float *ws, uint32_t batch);
-hipError_t guide_pack(const float *th, void *pack, float *thq, hipStream_t st);
+hipError_t guide_pack(const float *th, void *pack, hipStream_t st);
 hipError_t guide_infer(const void *pack, const float *in, uint32_t n, float *raw, uint32_t blocks, hipStream_t st);
-hipError_t guide_grad_rows(const float *th, const float *thq, const float *rows, uint32_t batch, float *ws,
-                           float *grad, float *scal, hipStream_t st);
+// loss and gradient of host rows (batch x 10: in[6], ω[3], w) without an
+// update: grad (kGParams floats) and raw (optional, batch x 40) on the device
+hipError_t guide_loss_grad_rows(GuideTrain &gt, const float *rows, uint32_t batch, float *grad, float *raw,
+                                hipStream_t st);
 hipError_t guide_train_rows(GuideTrain &gt, const float *rows, uint32_t batch, uint32_t steps, hipStream_t st);
 hipError_t guide_train_records(GuideTrain &gt, const float4 *rec, const uint32_t *rec_count, uint32_t rec_cap,
                                uint32_t steps, hipStream_t st);

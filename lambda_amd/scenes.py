@@ -29,9 +29,11 @@ OBJ + CSV files in the reference's scene schema (lib/numli/cg.hpp:1991-2474).
                      shell (Lambertian), 200 instances of a rounded box
                      (subdivision 14: 2,352 tris; ~470 k in total) with Blinn
                      lobes (α uniform 50-500, no transmission), and one small
-                     light under the ceiling hidden by a plate below it, so
-                     the room is lit by the plate's and the ceiling's bounce;
-                     1024x1024.  The workload of the learned proposal.
+                     light on a rimmed cabinet top in the far corner, facing
+                     the ceiling, hidden from the room, which is lit by the
+                     ceiling corner it lights (layout "plate": under the
+                     ceiling above a plate instead); 1024x1024.  The workload
+                     of the learned proposal.
 
 Each function returns the path of the .nls file.  Same seed, same bytes.
 """
@@ -456,8 +458,20 @@ def rounded_box(k, p=8.0):
     return V, F, N
 
 
-def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14, plate=1.0, plate_gap=0.35,
-                    box_ks=0.35):
+def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv=14, layout="cabinet", plate=1.0,
+                    plate_gap=0.35, box_ks=0.35):
+    """The learned-proposal workload: a 10 x 10 x 4 m Lambertian room shell,
+    200 instanced Blinn rounded boxes (exponent 50-500) and one small quad
+    light hidden behind an occluder, so that the room is lit indirectly.
+
+    layout "cabinet": the light sits on top of a 3 m cabinet in the far
+    corner, facing the ceiling, inside a 0.4 m rim; no point of the room
+    below the rim sees it, and the room is lit by the ceiling corner and the
+    upper walls it lights (a bright patch seen by the whole room).
+    layout "plate": the light hangs under the ceiling facing down onto a 2 m
+    plate `plate_gap` below it; the room is lit through the slit between
+    plate and ceiling and then by several diffuse bounces (broad light that
+    leaves a learned proposal little to gain)."""
     os.makedirs(out_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     lam_ref = np.arange(355.0, 836.0, 5.0)
@@ -491,27 +505,47 @@ def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv
         write_obj(os.path.join(out_dir, f"{nm}.obj"), V, F, N)
         objects.append({"name": nm, "type": "trimesh", "transform": {}, "material": f"mat_{nm}",
                         "source": os.path.join(out_dir, f"{nm}.obj")})
-    # the light hangs under the ceiling at (lx, ly), facing down; a plate
-    # (half-width `plate`, 2 m by default) `plate_gap` below the ceiling
-    # hides it from the room, which is lit by the plate's and then the
-    # ceiling's bounce.  (The exponent is even: with an odd one the
-    # reference's quad "dir" light gives negative radiance to points behind
-    # it, cg.hpp:1625-1668, and the ceiling is.)
-    lx, ly, ph = 2.5, 2.5, plate
-    V, F, N = _mesh_from_quads(_block(lx - ph, ly - ph, lx + ph, ly + ph, 0.05))
-    write_obj(os.path.join(out_dir, "plate.obj"), V, F, N)
     write_csv(os.path.join(out_dir, "albedo_plate.csv"), lam_ref, 0.8 * np.ones_like(lam_ref))
     materials.append({"name": "mat_plate", "type": "lambertian",
                       "albedo": {"val": os.path.join(out_dir, "albedo_plate.csv")}})
-    objects.append({"name": "plate", "type": "trimesh", "material": "mat_plate",
-                    "source": os.path.join(out_dir, "plate.obj"), "transform": {"translate": [0, 0, H - plate_gap]}})
+
+    def block(name, x0, y0, x1, y1, h, z):
+        V, F, N = _mesh_from_quads(_block(x0, y0, x1, y1, h))
+        write_obj(os.path.join(out_dir, f"{name}.obj"), V, F, N)
+        objects.append({"name": name, "type": "trimesh", "material": "mat_plate",
+                        "source": os.path.join(out_dir, f"{name}.obj"), "transform": {"translate": [0, 0, z]}})
+
+    # (The light's exponent is even: with an odd one the reference's quad
+    # "dir" light gives negative radiance to points behind it,
+    # cg.hpp:1625-1668.  Its back emits as its front.)
+    if layout == "cabinet":
+        # cabinet 1.2 m square, 3 m tall, 1 cm off the corner walls; a 5 cm
+        # rim, 0.4 m tall, on its two open sides; the light 0.25 m above the
+        # top, facing up
+        top, rim, c0 = 3.0, 0.4, 1.2
+        x0, y0, x1, y1 = X - c0, Y - c0, X - 0.01, Y - 0.01
+        block("cabinet", x0, y0, x1, y1, top, 0.0)
+        block("rim_s", x0, y0, x1, y0 + 0.05, rim, top)
+        block("rim_w", x0, y0 + 0.05, x0 + 0.05, y1, rim, top)
+        light_tf = {"scale": [0.15, 0.15, 1], "translate": [0.5 * (x0 + x1), 0.5 * (y0 + y1), top + 0.25]}
+        keep_out = lambda x, y: x > x0 - 0.6 and y > y0 - 0.6
+    elif layout == "plate":
+        # the light hangs under the ceiling at (lx, ly), facing down; a plate
+        # (half-width `plate`) `plate_gap` below the ceiling hides it
+        lx, ly = 2.5, 2.5
+        block("plate", lx - plate, ly - plate, lx + plate, ly + plate, 0.05, H - plate_gap)
+        light_tf = {"scale": [0.15, 0.15, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
+                    "translate": [lx, ly, H - 0.01]}
+        keep_out = lambda x, y: False
+    else:
+        raise ValueError(f"unknown layout {layout!r}")
     # glossy boxes
     V, F, N = rounded_box(box_subdiv)
     write_obj(os.path.join(out_dir, "rbox.obj"), V, F, N)
     placed = 0
     while placed < n_boxes:
         x, y = rng.uniform(-4.4, 4.4, 2)
-        if (x + 4.5) ** 2 + (y + 4.5) ** 2 < 2.0 ** 2:  # keep the camera's corner clear
+        if (x + 4.5) ** 2 + (y + 4.5) ** 2 < 2.0 ** 2 or keep_out(x, y):  # the camera's corner, the cabinet
             continue
         sc = float(rng.uniform(0.15, 0.45))
         z = float(rng.uniform(sc, 3.0))
@@ -531,9 +565,7 @@ def glossy_interior(out_dir, seed=4, width=1024, hero=4, n_boxes=200, box_subdiv
         "hero_samples": hero,
         "camera": {"pos": [-4.5, -4.5, 1.7], "look_at": [1.0, 1.0, 1.2], "up": [0, 0, 1], "fov": 70,
                    "ar": 1.0, "width": width},
-        "lights": [{"name": "hidden", "type": "quaddir", "specular": 2,
-                    "transform": {"scale": [0.15, 0.15, 1], "rotate": {"axis": [1, 0, 0], "degrees": 180},
-                                  "translate": [lx, ly, H - 0.01]},
+        "lights": [{"name": "hidden", "type": "quaddir", "specular": 2, "transform": light_tf,
                     "radiance": {"val": os.path.join(out_dir, "blackbody4500.csv"), "mult": 400.0}}],
         "objects": objects,
         "materials": materials,

@@ -1,6 +1,7 @@
 """Writes glossy_interior_64_fp64.npz: the FP64 oracle's render of the
 guiding workload's scene (the glossy interior of lambda_amd.scenes with its
-seed 4: 200 Blinn rounded boxes, hidden light) at 64x64, 4096 samples per
+seed 4 and its default cabinet layout: 200 Blinn rounded boxes, the light
+hidden on a rimmed cabinet top) at 64x64, 4096 samples per
 pixel, 32 bounces, p = 0.8, hero 4 — the reference image against which the
 guided and plain GPU renders are judged for bias (tests/test_guide.py).
 

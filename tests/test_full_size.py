@@ -94,7 +94,8 @@ def test_full_size_paths(ctx, full, name):
     smp = rng.integers(0, 1024, 16384).astype(np.uint32)
     pr = la.params(spp=1, max_scatterings=bounces, sample_p=0.8, hero=scene.hero, seed=9)
     g, o = ctx.trace_paths(pr, pix, smp), orc.trace_paths(pr, pix, smp)
-    assert np.isfinite(o).all() and (o > 0).mean() > 0.05
+    # (the glossy interior's light is hidden: about 3% of its path values are non-zero)
+    assert np.isfinite(o).all() and (o > 0).mean() > 0.01
     _check_paths(g, o)
 
 

@@ -282,7 +282,7 @@ def test_deep_bvh_uploads_and_renders(ctx, tmp_path):
     stack held): it uploads, its hits are bit-exact and every traversal
     kernel (trace queue, finish, megakernel, path hook) agrees"""
     from lambda_amd import scenes
-    s = la.load_nls(scenes.glossy_interior(str(tmp_path), width=48, plate=0.5, plate_gap=0.2))
+    s = la.load_nls(scenes.glossy_interior(str(tmp_path), width=48, layout="plate", plate=0.5, plate_gap=0.2))
     ctx.upload(s)
     assert ctx.stats().stack_bound > 64
     orc = ob.Oracle(s)

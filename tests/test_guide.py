@@ -4,9 +4,13 @@ Adam), and the guided render (unbiased against the plain render loop).
 CPU:  the numpy oracle's hand-written backward against torch float64
       autograd of the same forward; the vMF mixture integrates to 1.
 GPU:  lam_guide_infer vs the oracle forward (same bf16 roundings);
-      lam_guide_loss_grad vs the oracle gradient; Adam's first step;
-      training fits a known distribution; lam_render_guided with α = 0 is
-      lam_render; the guided image has the plain image's expectation.
+      lam_guide_loss_grad vs the oracle gradient (one chunk per block,
+      several, a ragged last chunk); the training forward is the inference
+      bit for bit; Adam rewrites the packed network in place as a fresh pack
+      would, deterministically; Adam's first step; training fits a known
+      distribution; lam_render_guided with α = 0 is lam_render; the guided
+      image has the plain image's and the FP64 oracle's expectation; the
+      trained network lowers the variance.
 """
 import numpy as np
 import pytest
@@ -166,6 +170,50 @@ def test_loss_grad_matches_oracle(ctx):
 
 
 @pytest.mark.gpu
+def test_loss_grad_multi_chunk_and_ragged(ctx):
+    """a batch larger than one pass of the training grid (blocks that sum
+    several 64-row chunks) and a ragged one (a partial last chunk) against
+    the oracle"""
+    th = _theta(16)
+    ctx.guide_set_params(th)
+    for n in (65536, 333):
+        rows = _rows(np.random.default_rng(n), n)
+        loss, g = ctx.guide_loss_grad(rows)
+        lo, go = og.loss_grad(th, rows, bf16_dz=True)
+        assert abs(loss - lo) <= 1e-4 * abs(lo), n
+        assert np.linalg.norm(g - go) <= 5e-3 * np.linalg.norm(go), n
+
+
+@pytest.mark.gpu
+def test_training_forward_is_the_inference(ctx):
+    """the training forward is the inference network's evaluation: its raw
+    outputs equal lam_guide_infer's bit for bit"""
+    th = _theta(17)
+    ctx.guide_set_params(th)
+    rows = _rows(np.random.default_rng(18), 1000)
+    _, _, raw = ctx.guide_loss_grad(rows, want_raw=True)
+    assert np.array_equal(raw, ctx.guide_infer(np.ascontiguousarray(rows[:, :6])))
+
+
+@pytest.mark.gpu
+def test_training_updates_the_pack_in_place_and_deterministically(ctx):
+    """Adam steps rewrite each parameter's packed bf16 entries in place: the
+    network afterwards evaluates exactly as one packed afresh from the same
+    parameters; and two identical training runs give identical parameters"""
+    rows = _rows(np.random.default_rng(19), 4096)
+    x = np.ascontiguousarray(_rows(np.random.default_rng(20), 512)[:, :6])
+    out = []
+    for _ in range(2):
+        ctx.guide_reset(21)
+        ctx.guide_train(rows, 5, 2e-3)
+        out.append((ctx.guide_get_params(), ctx.guide_infer(x)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    ctx.guide_set_params(out[0][0])  # a fresh pack of the same parameters
+    assert np.array_equal(ctx.guide_infer(x), out[0][1])
+
+
+@pytest.mark.gpu
 def test_adam_first_step(ctx):
     """Adam's first step moves every parameter by lr · g/(|g| + ε)"""
     th = _theta(8)
@@ -285,9 +333,9 @@ def test_guided_render_unbiased(ctx, glossy):
 def test_trained_network_lowers_variance(ctx, tmp_path):
     """What the learned proposal gains: on the hidden-light room (Lambertian
     boxes), per-pixel variance over 8 independent renders with the network
-    trained on 64 passes is below the plain sampler's (measured ratio
-    0.83) and well below guiding with the untrained network (2.6x plain),
-    at equal samples per pixel."""
+    trained on 64 passes is well below the plain sampler's (measured ratio
+    0.44) and below guiding with the untrained network (2.0x plain), at
+    equal samples per pixel."""
     from lambda_amd import scenes
     sc = la.load_nls(scenes.glossy_interior(str(tmp_path), seed=4, width=64, n_boxes=24, box_subdiv=3,
                                             box_ks=0.0))
@@ -311,7 +359,7 @@ def test_trained_network_lowers_variance(ctx, tmp_path):
     assert ctx.timing().guide_records > 100000
     v_trained = rel_var([ctx.render_guided(mk(50 + k), frozen) for k in range(K)])
     print(f"relative variance: plain {v_plain:.5f}, untrained {v_untrained:.5f}, trained {v_trained:.5f}")
-    assert v_trained < 0.93 * v_plain, (v_plain, v_trained)
+    assert v_trained < 0.7 * v_plain, (v_plain, v_trained)
     assert v_trained < 0.5 * v_untrained, (v_untrained, v_trained)
 
 
@@ -343,7 +391,10 @@ def test_guided_and_plain_match_the_fp64_reference(ctx, tmp_path):
     first 32 passes) both converge to the FP64 oracle's 4096-spp image
     (tests/golden/glossy_interior_64_fp64.npz, written by
     tests/golden/make_glossy_fp64.py): image sums within 4 standard errors,
-    per-pixel |z| of median below 1.2, relMSE against the reference printed"""
+    per-pixel |z| of median below 1.2, relMSE against the reference printed.
+    (A check of bias only: at 64x64 the network learns from 256x fewer
+    records than at the workload's 1024x1024, and the guided estimate is the
+    noisier one here: measured relMSE 0.43 against plain 0.16.)"""
     import os
     from lambda_amd import scenes
     ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "glossy_interior_64_fp64.npz"))
@@ -354,9 +405,8 @@ def test_guided_and_plain_match_the_fp64_reference(ctx, tmp_path):
                                        spp_offset=off)
     plain = [ctx.render(mk(3, k * spp // 2)).astype(np.float64) for k in range(2)]
     ctx.guide_reset(5)
-    guided = [ctx.render_guided(mk(5 + k), la.guide_params(alpha=0.5, train_passes=32 if k == 0 else 0,
-                                                           steps_per_pass=8, batch=4096, seed=5, max_vertex=4,
-                                                           lobes=la.GUIDE_LAMBERTIAN)).astype(np.float64)
+    guided = [ctx.render_guided(mk(5 + k), la.guide_params(train_passes=32 if k == 0 else 0, steps_per_pass=32,
+                                                           batch=4096, seed=5)).astype(np.float64)
               for k in range(2)]
     r, vr = ref["mean"].astype(np.float64), ref["var_of_mean"].astype(np.float64)
     w = np.array([0.2126, 0.7152, 0.0722])
