@@ -250,6 +250,9 @@ def render_kernel_roofline(ctx, bounces, p, hero, spp=4):
     return out
 
 
+_CPU_INFO = {}
+
+
 def _cpu_info():
     """CPU model, sockets, physical cores and logical CPUs from lscpu (or
     /proc/cpuinfo), and the CPUs this process may run on"""
@@ -288,7 +291,7 @@ def cpu_baseline(scene, spp, bounces, p, hero, budget_s=20.0):
     the reference's per-object linear loop on scenes of many instances."""
     from oracle.binding import Oracle
     orc = Oracle(scene)
-    info = _cpu_info()
+    info = dict(_CPU_INFO) or _cpu_info()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info.get("usable_cpus") or 1
     h = scene.height
     w = scene.width
@@ -553,12 +556,14 @@ def main():
     ap.add_argument("--samples-per-pass", type=int, default=0,
                     help="samples of every pixel per wavefront pass (0 = the library's default)")
     ap.add_argument("--extend-only", action="store_true",
-                    help="profiling helper: only the extend microbenchmark plus a 1 GiB device copy "
-                         "(the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
+                    help="profiling helper: only the extend and shadow ray-batch microbenchmarks plus a 1 GiB "
+                         "device copy (the copy calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     # the CPU baseline's OpenMP threads spread over the cores (read by the
     # OpenMP runtime when the oracle library loads)
     os.environ.setdefault("OMP_PROC_BIND", "spread")
+    # the host's CPUs before anything pins this process's threads
+    _CPU_INFO.update(_cpu_info())
     if args.gpus != 1:
         print(json.dumps({"error": "this renderer runs on one GPU; --gpus > 1 is not supported "
                                    "(samples would shard by pixel tiles with one film all-reduce)"}))
@@ -602,17 +607,22 @@ def main():
     w, h = scene.width, scene.height
     if args.extend_only:
         rl = extend_roofline(ctx, scene, reps=3)
+        sl = shadow_roofline(ctx, scene, reps=3)
         nbytes = 1 << 30
         a, b = la.DeviceBuffer(nbytes), la.DeviceBuffer(nbytes)
         for _ in range(3):  # device to device: the counters' scale reference
             assert la._hip().hipMemcpy(b.ptr, a.ptr, nbytes, 3) == 0
         a.free()
         b.free()
-        print(json.dumps({"extend": rl, "copy_bytes": nbytes}), flush=True)
+        print(json.dumps({"extend": rl, "shadow": sl, "copy_bytes": nbytes}), flush=True)
         ctx.close()
         return
-    if args.warmup:
-        ctx.render(la.params(spp=args.warmup, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
+    # the warm-up renders at least one full wavefront pass (samples_per_pass
+    # samples of every pixel), so that every lane's path state and graph
+    # exist before the timed render, whatever W is
+    warm_spp = max(args.warmup, ctx.wave_config()["samples_per_pass"]) if args.warmup else 0
+    if warm_spp:
+        ctx.render(la.params(spp=warm_spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
                              spp_offset=1 << 20, flags=args.flags))
     pr = la.params(spp=args.steps, max_scatterings=bounces, sample_p=p, hero=hero, seed=1, flags=args.flags)
     ctx.reset_timing()
@@ -630,7 +640,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": "icosphere field: 64 x 15,680-tri instances (1,003,520 tris), ground plane, "
                                "2 area lights, 1920x1080, 32 bounces, p=0.8, hero 4 on 360-830 nm",
-                   "width": w, "height": h, "spp_per_step": 1, "bounces": bounces, "bounce_prob": p,
+                   "width": w, "height": h, "spp_per_step": 1, "warmup_spp": warm_spp, "bounces": bounces,
+                   "bounce_prob": p,
                    "triangles": scene.n_tris},
         "render_rays": {"rays_per_s": round((tm.extend_rays + tm.shadow_rays) / wall, 1),
                         "extend_rays_per_sample": round(tm.extend_rays / samples, 4),

@@ -294,8 +294,8 @@ def test_guided_alpha0_is_plain(ctx, glossy, flags):
 def test_guided_render_unbiased(ctx, glossy):
     """guided and plain renders are independent estimates of one image: the
     per-pixel difference of their means is noise (z-test), with a trained
-    network and with an untrained one, and with only the first two vertices
-    guided"""
+    network and with an untrained one, with only the first two vertices
+    guided, and with the Blinn specular lobe under the mixture as well"""
     ctx.upload(glossy)
     spp = 256
     p = la.params(spp=spp, max_scatterings=8, sample_p=0.8, hero=4, seed=5)
@@ -303,10 +303,11 @@ def test_guided_render_unbiased(ctx, glossy):
     for half in range(2):  # two halves for a per-pixel variance estimate
         plain.append(ctx.render(la.params(spp=spp // 2, max_scatterings=8, sample_p=0.8, hero=4, seed=5,
                                           spp_offset=half * (spp // 2))).astype(np.float64))
-    for trained, mv in ((False, 0), (True, 0), (True, 2)):
+    every = la.GUIDE_LAMBERTIAN | la.GUIDE_BLINN_KD | la.GUIDE_BLINN_SPEC
+    for trained, mv, lobes in ((False, 0, 0), (True, 0, 0), (True, 2, 0), (True, 0, every)):
         ctx.guide_reset(2)
         gp = la.guide_params(alpha=0.5, train_passes=8 if trained else 0, steps_per_pass=8, batch=4096,
-                             lr=5e-3, seed=7, max_vertex=mv)
+                             lr=5e-3, seed=7, max_vertex=mv, lobes=lobes)
         guided = []
         for half in range(2):
             # pure wavefront: every vertex is shaded by the guided kernel
@@ -314,7 +315,7 @@ def test_guided_render_unbiased(ctx, glossy):
                            spp_offset=0, flags=la.LAM_FLAG_NO_TAIL)
             guided.append(ctx.render_guided(pg, gp if half == 0 else
                                             la.guide_params(alpha=0.5, train_passes=0, seed=7,
-                                                            max_vertex=mv)).astype(np.float64))
+                                                            max_vertex=mv, lobes=lobes)).astype(np.float64))
             if trained and half == 0:
                 assert ctx.timing().guide_records > 1000
         mp, mg = (plain[0] + plain[1]) / 2, (guided[0] + guided[1]) / 2
