@@ -9,21 +9,59 @@
 // term of vertex j).  With the random numbers of the path fixed:
 //   ∂L/∂θ = Σ_j (∂f_j/∂θ / f_j) ⊙ L_after(j) + Σ_j β_j ⊙ w_L,j mult_j ∂coef_j/∂θ / p_L,j
 // where L_after(j) is the radiance collected after vertex j's NEE term.
-// The forward pass (the render's wavefront pass) leaves each path's total
-// L; the replay walks the path again subtracting each collected term, so
-// nothing is stored per vertex.
 // None of the differentiated parameters changes a sampled direction, pdf or
 // Russian-roulette decision (Lambertian sampling is cosine, thin-film
 // sampling is the Blinn half-vector of α, cg.hpp:1755-1931).
+//
+// The replay is a wavefront pass over the same path slots as the forward
+// pass before it (the render's pass without a film, which leaves each path's
+// total radiance L in the path state), and shares its extend kernel and
+// queues.  L then holds the radiance a path still collects: every collected
+// term is subtracted as the replay meets it, so nothing is stored per vertex.
+// Per iteration:
+//   k_extend       closest hits of the queue (the render's kernel)
+//   k_adj_pending  at the path's next vertex, the pending terms of the
+//                  previous one: its sampled factor, times the L left now
+//                  (= L_after), and its NEE term if the light was visible
+//   k_adj_shade    the vertex itself: light sample, BSDF sample, MIS
+//                  weights as in k_shade, the NEE gradient held pending
+//                  with its shadow ray, the sampled factor's ∂f/f held
+//                  pending for the next vertex, β and the continuation
+//   k_adj_shadow   any hit; a visible light subtracts the NEE term from L
+//                  and marks the pending NEE gradient as due
+// After the last iteration k_adj_final adds the NEE terms of paths that
+// ended at a visible light sample.  The gradient terms gather in LDS per
+// block (float atomics) and each block adds its sums into a partial row of
+// its own; k_adj_sum_rows adds the rows into the gradient at the end.
 #include "lam_launch.h"
 #include "lam_shade.h"
+#include "lam_wave.h"
 
 namespace lam {
 
 struct DiffDev {
   const int32_t *mat_d, *mat_ior, *mat_albedo;
   uint32_t n_params;
+  // per-block partial gradients of the replay kernels: block b of any of
+  // them adds its LDS sums into row b (n_params floats; rows of one lane,
+  // whose kernels run in stream order)
+  float *part;
 };
+
+// the block's gradient sums in LDS (n_params floats), zeroed at the start
+__device__ __forceinline__ void grad_lds_init(float *gl, const DiffDev &dv) {
+  for (uint32_t j = threadIdx.x; j < dv.n_params; j += blockDim.x) gl[j] = 0.f;
+  __syncthreads();
+}
+// ... and added into the block's partial row at the end
+__device__ __forceinline__ void grad_lds_flush(const float *gl, const DiffDev &dv) {
+  __syncthreads();
+  float *row = dv.part + size_t(blockIdx.x) * dv.n_params;
+  for (uint32_t j = threadIdx.x; j < dv.n_params; j += blockDim.x) {
+    const float g = gl[j];
+    if (g != 0.f) row[j] += g;
+  }
+}
 
 // thinfilm::BRDFcosθ (cg.hpp:1031-1051) with its derivatives with respect to
 // the film thickness d and the film IOR n1 at each hero wavelength.  The
@@ -84,51 +122,6 @@ __device__ void thinfilm_f_grad(const DevScene &sc, const lam_material &m, const
   }
 }
 
-// gradient of one BSDF factor of the material at a vertex, scaled per
-// wavelength by w[k], accumulated into the block's gradient in LDS.
-// ratio: the factor value divides w (sampled-factor form ∂f/f · L_after)
-template <int H>
-__device__ void accumulate(const DevScene &sc, const DiffDev &dv, const lam_material &m, uint32_t mat,
-                           const Lam<H> &L, f3 i, f3 o, f3 n, bool front, const float *w, bool ratio,
-                           float *gl) {
-  if (m.type == LAM_MAT_LAMBERTIAN) {
-    const int32_t base = dv.mat_albedo[mat];
-    if (base < 0) return;
-    // albedo(λ) = (1-t) v_i + t v_{i+1}; f = albedo · cos/π
-    float rho[H];
-    spec_eval<H>(sc, m.spec[0], L, rho);
-    const float cs = fmax_ref(0.f, dot(i, n)) * kInvPi;
-#pragma unroll
-    for (int k = 0; k < H; k++) {
-      float g;
-      if (ratio) g = rho[k] != 0.f ? w[k] / rho[k] : 0.f;
-      else g = w[k] * cs;
-      if (g == 0.f) continue;
-      atomicAdd(gl + base + L.i[k], g * (1.f - L.t[k]));
-      atomicAdd(gl + base + L.i[k] + 1, g * L.t[k]);
-    }
-  } else if (m.type == LAM_MAT_THINFILM) {
-    const int32_t bd = dv.mat_d[mat], bn = dv.mat_ior[mat];
-    if (bd < 0 && bn < 0) return;
-    float val[H], dd[H], dn[H];
-    thinfilm_f_grad<H>(sc, m, L, i, o, n, front, val, dd, dn);
-    float gd = 0.f;
-#pragma unroll
-    for (int k = 0; k < H; k++) {
-      float s = w[k];
-      if (ratio) s = val[k] != 0.f ? w[k] / val[k] : 0.f;
-      if (s == 0.f) continue;
-      gd += s * dd[k];
-      if (bn >= 0) {
-        const float gn = s * dn[k];
-        atomicAdd(gl + bn + L.i[k], gn * (1.f - L.t[k]));
-        atomicAdd(gl + bn + L.i[k] + 1, gn * L.t[k]);
-      }
-    }
-    if (bd >= 0 && gd != 0.f) atomicAdd(gl + bd, gd);
-  }
-}
-
 template <int H>
 __device__ bool differentiable(const DiffDev &dv, const lam_material &m, uint32_t mat) {
   if (m.type == LAM_MAT_LAMBERTIAN) return dv.mat_albedo[mat] >= 0;
@@ -136,185 +129,453 @@ __device__ bool differentiable(const DiffDev &dv, const lam_material &m, uint32_
   return false;
 }
 
-// One vertex of the replay, with the operations of path_vertex (lam_shade.h):
-// L holds the radiance the path still collects; each collected term is
-// subtracted, and the parameter gradients of the vertex are accumulated
-// into gl, weighted by a[k] = ∂loss/∂L_k.  Returns false at the path's end.
-template <int H, class Stack>
-__device__ __forceinline__ bool replay_vertex(const DevScene &sc, const DiffDev &dv, uint32_t max_scat,
-                                              float sample_p, uint64_t key, const Lam<H> &lam, uint32_t s, Ray &ray,
-                                              float *beta, float *L, const float *a, float *gl, const Stack &stk,
-                                              bool &ovf) {
-  TravCounts tc{0, 0};
-  const Hit hit = trace<false, false>(sc, ray, stk, tc, ovf);
-  if (hit.obj == LAM_NONE || ovf) return false;
-  const Surf sf = surface(sc, hit, ray.o, ray.d);
-  const f3 o = neg(normalized(ray.d));
-  const f3 n = sf.n();
-  const lam_material &mat = sc.materials[sf.mat];
-  // an emitter's term is the path's last: it is what L still holds
-  if (mat.type == LAM_MAT_EMITTER || mat.type == LAM_MAT_DIREMITTER) return false;
-  if (s > max_scat) return false;
-  const uint32_t nl = sc.n_lights;
-  const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
-  const float pl = 1.f / float(nl);
-  const lam_light &lt = sc.lights[li];
-  LightSample<H> ls;
-  sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
-                  lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
-  float coef[H];
-  bxdf<H>(sc, mat, lam, ls.dir, o, n, sf.front, coef);
-  bool vis = true;
-  if (ls.shadow) {
-    const Hit sh = trace<true, false>(sc, ls.sray, stk, tc, ovf);
-    vis = sh.obj == LAM_NONE;
-  }
-  float lis[H];
-#pragma unroll
-  for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
-  MatSample<H> ms;
-  sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
-                     lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)), lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)),
-                     sample_p, ms);
-  const bool diff = differentiable<H>(dv, mat, sf.mat);
-  float w_L = 1.f, w_mat = 0.f;
-  if (ms.ok) {
-    const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
-    const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], sample_p);
-    const float p_L_i = ls.prob * pl;
-    w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
-    w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
-  }
-  if (vis) {
-    float w[H];
+// ---- pending gradient terms and their records ------------------------------------------
+// A term of one vertex is (material, u[k], v[k]) per hero wavelength k, for
+// a Lambertian material u = the weight of the albedo at λ_k (shared by the
+// two grid coefficients the lookup interpolates, (1-t) and t); for thin film
+// u = the weight of the thickness d (summed over k) and v = the weight of
+// the film IOR at λ_k.  Its records multiply by m[k] (L_after for a sampled
+// factor, 1 for an NEE term).
+constexpr uint32_t kPendMark = 0x80000000u;  // pn_mat: NEE light not (yet) seen visible
+
+// a term's contributions, times m[k], added into the block's LDS sums
+template <int H>
+__device__ __forceinline__ void accumulate_term(float *gl, const DiffDev &dv, const lam_material &m, uint32_t mat,
+                                                const Lam<H> &L, const float *u, const float *v, const float *mult) {
+  if (m.type == LAM_MAT_LAMBERTIAN) {
+    const int32_t b = dv.mat_albedo[mat];
+    if (b < 0) return;
 #pragma unroll
     for (int k = 0; k < H; k++) {
-      L[k] -= ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
-      // ∂term/∂coef = β w_L mult / (p_L p_l)
-      w[k] = a[k] * (beta[k] * w_L * ls.mult[k] / (ls.prob * pl));
+      const float g = u[k] * mult[k];
+      if (g == 0.f) continue;
+      atomicAdd(gl + b + L.i[k], g * (1.f - L.t[k]));
+      atomicAdd(gl + b + L.i[k] + 1, g * L.t[k]);
     }
-    if (diff) accumulate<H>(sc, dv, mat, sf.mat, lam, ls.dir, o, n, sf.front, w, false, gl);
-  }
-  if (!ms.ok) return false;
-  if (diff) {
-    float w[H];
+  } else if (m.type == LAM_MAT_THINFILM) {
+    const int32_t bd = dv.mat_d[mat], bn = dv.mat_ior[mat];
+    if (bd >= 0) {
+      float gd = 0.f;
 #pragma unroll
-    for (int k = 0; k < H; k++) w[k] = a[k] * L[k];
-    accumulate<H>(sc, dv, mat, sf.mat, lam, ms.dir, o, n, sf.front, w, true, gl);
-  }
+      for (int k = 0; k < H; k++) gd += u[k] * mult[k];
+      if (gd != 0.f) atomicAdd(gl + bd, gd);
+    }
+    if (bn >= 0)
 #pragma unroll
-  for (int k = 0; k < H; k++) beta[k] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
-  ray.o = sf.p;
-  ray.d = ms.dir;
-  ray.tmax = kFltMax;
-  ray.skip = LAM_NONE;
-  return true;
+      for (int k = 0; k < H; k++) {
+        const float g = v[k] * mult[k];
+        if (g == 0.f) continue;
+        atomicAdd(gl + bn + L.i[k], g * (1.f - L.t[k]));
+        atomicAdd(gl + bn + L.i[k] + 1, g * L.t[k]);
+      }
+  }
 }
 
-// camera ray of a path (renderer.cpp:33-37 with the counter RNG)
-__device__ __forceinline__ Ray adjoint_camera_ray(const DevScene &sc, uint64_t key, uint32_t pixel) {
+// The term of the BSDF value at (i, o): ratio = false: ∂f/∂θ scaled by w[k]
+// (an NEE term); ratio = true: (∂f/∂θ)/f scaled by w[k] (a sampled factor).
+template <int H>
+__device__ __forceinline__ void bsdf_term(const DevScene &sc, const lam_material &m, const Lam<H> &L, f3 i, f3 o,
+                                          f3 n, bool front, const float *w, bool ratio, float *u, float *v) {
+  if (m.type == LAM_MAT_LAMBERTIAN) {
+    // albedo(λ) = (1-t) v_i + t v_{i+1}; f = albedo · cos/π
+    float rho[H];
+    spec_eval<H>(sc, m.spec[0], L, rho);
+    const float cs = fmax_ref(0.f, dot(i, n)) * kInvPi;
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      u[k] = ratio ? (rho[k] != 0.f ? w[k] / rho[k] : 0.f) : w[k] * cs;
+      v[k] = 0.f;
+    }
+  } else {
+    float val[H], dd[H], dn[H];
+    thinfilm_f_grad<H>(sc, m, L, i, o, n, front, val, dd, dn);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      const float s = ratio ? (val[k] != 0.f ? w[k] / val[k] : 0.f) : w[k];
+      u[k] = s * dd[k];
+      v[k] = s * dn[k];
+    }
+  }
+}
+
+// Replay state beyond the path state (per slot, [k][slot] for per-wavelength
+// values): the adjoint weights and the pending sampled factor and NEE term.
+struct ReplayDev {
+  float *a;
+  uint32_t *pf_mat;  // LAM_NONE: no pending sampled factor
+  float *pf_u, *pf_v;
+  uint32_t *pn_mat;  // LAM_NONE: none; with kPendMark: the light not seen visible
+  float *pn_u, *pn_v;
+};
+
+// adjoint weights of a path's hero wavelengths: ∂loss/∂L_k through the
+// film's linear map (cg.hpp:144-148, 306-341)
+template <int H>
+__device__ __forceinline__ void adjoint_weights(const DevScene &sc, const float *__restrict__ dloss, uint32_t pixel,
+                                                const Lam<H> &lam, float prob_l, float inv_n, float *a) {
+  const uint32_t w = sc.cam.width, h = sc.cam.height;
+  const uint32_t row = pixel / w, colj = pixel - row * w;
+  const size_t img = (size_t(h - row - 1) * w + colj) * 3;
+  const float g0 = dloss[img], g1 = dloss[img + 1], g2 = dloss[img + 2];
+  const float gx = 3.2404542f * g0 - 0.9692660f * g1 + 0.0556434f * g2;
+  const float gy = -1.5371385f * g0 + 1.8760108f * g1 - 0.2040259f * g2;
+  const float gz = -0.4985314f * g0 + 0.0415560f * g1 + 1.0572252f * g2;
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    splat_xyz(sc, lam.l[k], 1.f, X, Y, Z);
+    a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
+  }
+}
+
+// ---- replay kernels -------------------------------------------------------------------
+// the camera rays of the pass's slots (as k_raygen); L keeps the forward total
+template <int H>
+__global__ __launch_bounds__(256) void k_adj_raygen(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                    ReplayDev rd, uint32_t n, uint32_t *__restrict__ ctrl,
+                                                    const float *__restrict__ dloss, float inv_n) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    ctrl[CTRL_COUNT0] = n;
+    ctrl[CTRL_FETCH_EXT] = 0;
+    ctrl[CTRL_TAIL] = 0;
+  }
+  if (p >= n) return;
+  uint32_t pixel, sample;
+  path_id(ids, ctrl, p, pixel, sample);
+  const uint64_t key = lam_path_key(pa.seed, pixel, sample);
   const uint32_t w = sc.cam.width;
   const uint32_t row = pixel / w, colj = pixel - row * w;
-  const float u = float(colj) + lam_draw_f(key, LAM_DIM_U);
-  const float v = float(row) + lam_draw_f(key, LAM_DIM_V);
-  Ray ray;
-  camera_ray(sc.cam, u, v, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), ray.o, ray.d);
-  ray.tmax = kFltMax;
-  ray.skip = LAM_NONE;
-  return ray;
+  const float uu = float(colj) + lam_draw_f(key, LAM_DIM_U);
+  const float vv = float(row) + lam_draw_f(key, LAM_DIM_V);
+  f3 o, d;
+  camera_ray(sc.cam, uu, vv, lam_draw_f(key, LAM_DIM_LENS_X), lam_draw_f(key, LAM_DIM_LENS_Y), o, d);
+  float4 *q = reinterpret_cast<float4 *>(ws.ray[0]) + 2 * size_t(p);
+  q[0] = make_float4(o.x, o.y, o.z, kFltMax);
+  q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+  Lam<H> lam;
+  const float prob_l = lam_for<H>(sc, key, lam);
+  float a[H];
+  adjoint_weights<H>(sc, dloss, pixel, lam, prob_l, inv_n, a);
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    ws.beta[size_t(k) * ws.cap + p] = 1.f;
+    rd.a[size_t(k) * ws.cap + p] = a[k];
+  }
+  rd.pf_mat[p] = LAM_NONE;
+  rd.pn_mat[p] = LAM_NONE;
 }
 
-// The replay of one sample per pixel for the n paths of a chunk (pixels
-// pixel_offset .. pixel_offset + n - 1).  The forward pass is the render's
-// wavefront pass, run just before on the same paths: Lfwd[k·lcap + i] is the
-// total radiance of path i at wavelength k (the same value the vertex
-// operations of the replay add up to).  Adjoint weights come from
-// ∂loss/∂rgb of the path's pixel (the film's linear map, cg.hpp:144-148 and
-// 306-341).  Each lane replays its own path one vertex per step and takes
-// the next path when its replay ends (the queue pool of the traversal
-// kernels: a block starts with paths 64b..64b+63, then reserves runs from
-// *ctr, which starts at 0), so a wave never waits for its longest path.
-// Gradients gather in LDS and leave with one atomic per parameter per block.
+// At the path's next vertex (every entry of the extend queue, hit or miss),
+// before k_adj_shade: the previous vertex's pending terms, its sampled
+// factor times the radiance the path still collects (L_after) and its NEE
+// term if the light was visible
 template <int H>
-__global__ __launch_bounds__(64) void k_adjoint(DevScene sc, PathArgs pa, DiffDev dv, uint32_t sample, uint32_t n,
-                                                uint32_t pixel_offset, const float *__restrict__ Lfwd, size_t lcap,
-                                                float inv_n, const float *__restrict__ dloss, float *__restrict__ grad,
-                                                uint32_t *__restrict__ ctr, uint32_t *__restrict__ err,
-                                                uint32_t *__restrict__ spill) {
-  __shared__ uint32_t stack[kLdsStack * 64];
+__global__ __launch_bounds__(kShadeBlock) void k_adj_pending(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                             ReplayDev rd, DiffDev dv, uint32_t qsel,
+                                                             const uint32_t *__restrict__ ctrl) {
   extern __shared__ float gl[];
-  const uint32_t lane = threadIdx.x;
-  const SplitStack stk = SplitStack::of(stack, spill, blockIdx.x, gridDim.x);
-  for (uint32_t j = lane; j < dv.n_params; j += 64) gl[j] = 0.f;
-  __syncthreads();
-  const uint32_t w = sc.cam.width, h = sc.cam.height;
-  QueuePool qp;
-  qp.init(n);
-  bool active = false, ovf = false;
-  uint32_t s = 0;
-  uint64_t key = 0;
-  Lam<H> lam;
-  float a[H], beta[H], L[H];
-  Ray ray;
-  while (true) {
-    const uint64_t idle = __ballot(!active);
-    const uint32_t nidle = __popcll(idle);
-    if (qp.done(nidle)) break;
-    if (qp.can_refill(nidle)) {
-      const uint32_t i = qp.take(idle, nidle, !active, ctr);
-      if (!active && i < n) {
-        const uint32_t pixel = pixel_offset + i;
-        key = lam_path_key(pa.seed, pixel, sample);
-        ray = adjoint_camera_ray(sc, key, pixel);
-        const float prob_l = hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), lam);
-        // adjoint weight of each wavelength: ∂loss/∂L_k
-        const uint32_t row = pixel / w, colj = pixel - row * w;
-        const size_t img = (size_t(h - row - 1) * w + colj) * 3;
-        const float g0 = dloss[img], g1 = dloss[img + 1], g2 = dloss[img + 2];
-        const float gx = 3.2404542f * g0 - 0.9692660f * g1 + 0.0556434f * g2;
-        const float gy = -1.5371385f * g0 + 1.8760108f * g1 - 0.2040259f * g2;
-        const float gz = -0.4985314f * g0 + 0.0415560f * g1 + 1.0572252f * g2;
-        bool any = false;
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  if (blockIdx.x * kShadeBlock >= n) return;
+  grad_lds_init(gl, dv);
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  const size_t cap = ws.cap;
+  for (uint32_t q = blockIdx.x * kShadeBlock + threadIdx.x; q < n; q += gridDim.x * kShadeBlock) {
+    const uint32_t p = __float_as_uint(rin[2 * size_t(q) + 1].w);
+    const uint32_t mf = rd.pf_mat[p], mn = rd.pn_mat[p];
+    if (mf == LAM_NONE && mn == LAM_NONE) continue;
+    uint32_t pixel, sample;
+    path_id(ids, ctrl, p, pixel, sample);
+    Lam<H> lam;
+    lam_for<H>(sc, lam_path_key(pa.seed, pixel, sample), lam);
+    if (mf != LAM_NONE) {
+      float uf[H], vf[H], Lr[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        uf[k] = rd.pf_u[size_t(k) * cap + p];
+        vf[k] = rd.pf_v[size_t(k) * cap + p];
+        Lr[k] = ws.L[size_t(k) * cap + p];
+      }
+      accumulate_term<H>(gl, dv, sc.materials[mf], mf, lam, uf, vf, Lr);
+      rd.pf_mat[p] = LAM_NONE;
+    }
+    if (mn != LAM_NONE) {
+      if (!(mn & kPendMark)) {
+        float un[H], vn[H], ones[H];
 #pragma unroll
         for (int k = 0; k < H; k++) {
-          float X = 0.f, Y = 0.f, Z = 0.f;
-          splat_xyz(sc, lam.l[k], 1.f, X, Y, Z);
-          a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
-          any = any || a[k] != 0.f;
-          beta[k] = 1.f;
-          L[k] = Lfwd[size_t(k) * lcap + i];
+          un[k] = rd.pn_u[size_t(k) * cap + p];
+          vn[k] = rd.pn_v[size_t(k) * cap + p];
+          ones[k] = 1.f;
         }
-        active = any;
-        s = 0;
+        accumulate_term<H>(gl, dv, sc.materials[mn], mn, lam, un, vn, ones);
       }
-    }
-    if (active) {
-      active = replay_vertex<H>(sc, dv, pa.max_scat, pa.sample_p, key, lam, s, ray, beta, L, a, gl, stk, ovf);
-      s++;
-      if (ovf) {
-        atomicOr(err, 1u);
-        active = false;
-      }
+      rd.pn_mat[p] = LAM_NONE;
     }
   }
-  __syncthreads();
-  for (uint32_t j = lane; j < dv.n_params; j += 64)
-    if (gl[j] != 0.f) atomicAdd(grad + j, gl[j]);
+  grad_lds_flush(gl, dv);
 }
 
-hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
-                          const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
-                          uint32_t n, uint32_t pixel_offset, const float *Lfwd, size_t lcap, float inv_n,
-                          const float *dloss, float *grad, uint32_t *ctr, uint32_t *err, uint32_t *spill,
-                          uint32_t blocks, hipStream_t st) {
-  DiffDev dv{mat_d, mat_ior, mat_albedo, n_params};
-  const size_t shmem = size_t(n_params) * 4;
-  if (hero == 16)
-    hipLaunchKernelGGL((k_adjoint<16>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, n, pixel_offset, Lfwd,
-                       lcap, inv_n, dloss, grad, ctr, err, spill);
-  else
-    hipLaunchKernelGGL((k_adjoint<4>), dim3(blocks), dim3(64), shmem, st, sc, pa, dv, sample, n, pixel_offset, Lfwd,
-                       lcap, inv_n, dloss, grad, ctr, err, spill);
+template <int H>
+__global__ __launch_bounds__(kShadeBlock) void k_adj_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                           ReplayDev rd, DiffDev dv, uint32_t s, uint32_t qsel,
+                                                           uint32_t *__restrict__ ctrl) {
+  const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
+  float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
+  const size_t cap = ws.cap;
+  const uint32_t stride = gridDim.x * kShadeBlock;
+  __shared__ BlockQueues<kShadeBlock / 64, 2> sq;
+  extern __shared__ float gl[];
+  if (blockIdx.x * kShadeBlock >= n) return;
+  grad_lds_init(gl, dv);
+  uint32_t par = 0;
+  for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
+    const uint32_t q = bbase + threadIdx.x;
+    bool push_next = false, push_shadow = false;
+    uint32_t p = 0;
+    Ray nxt, shd;
+    Lam<H> lam;
+    if (q < n) {
+      const float4 ra = rin[2 * size_t(q)], rb = rin[2 * size_t(q) + 1];
+      const f3 ro = mk(ra.x, ra.y, ra.z), rdir = mk(rb.x, rb.y, rb.z);
+      p = __float_as_uint(rb.w);
+      uint32_t pixel, sample;
+      path_id(ids, ctrl, p, pixel, sample);
+      const uint64_t key = lam_path_key(pa.seed, pixel, sample);
+      lam_for<H>(sc, key, lam);
+      const Hit hit = ws.hits[q];
+      if (hit.obj != LAM_NONE && s <= pa.max_scat) {
+        const Surf sf = surface(sc, hit, ro, rdir);
+        const f3 o = neg(normalized(rdir));
+        const f3 nn = sf.n();
+        const lam_material &mat = sc.materials[sf.mat];
+        if (mat.type != LAM_MAT_EMITTER && mat.type != LAM_MAT_DIREMITTER) {
+          float beta[H], a[H];
+#pragma unroll
+          for (int k = 0; k < H; k++) {
+            beta[k] = ws.beta[size_t(k) * cap + p];
+            a[k] = rd.a[size_t(k) * cap + p];
+          }
+          const uint32_t nl = sc.n_lights;
+          const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
+          const float pl = 1.f / float(nl);
+          const lam_light &lt = sc.lights[li];
+          LightSample<H> ls;
+          sample_light<H>(sc, lt, lam, sf, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_A)),
+                          lam_draw_f(key, lam_vdim(s, LAM_VDIM_LIGHT_B)), ls);
+          float coef[H];
+          bxdf<H>(sc, mat, lam, ls.dir, o, nn, sf.front, coef);
+          float lis[H];
+#pragma unroll
+          for (int k = 0; k < H; k++) lis[k] = ls.mult[k] * coef[k] / (ls.prob * pl);
+          MatSample<H> ms;
+          sample_material<H>(sc, mat, lam, sf, o, lam_draw_f(key, lam_vdim(s, LAM_VDIM_LOBE)),
+                             lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_A)),
+                             lam_draw_f(key, lam_vdim(s, LAM_VDIM_DIR_B)), pa.sample_p, ms);
+          const bool diff = differentiable<H>(dv, mat, sf.mat);
+          float w_L = 1.f, w_mat = 0.f;
+          if (ms.ok) {
+            const float p_L_mati = prob_light(sc, lt, sf.p, ms.dir) * pl;
+            const float p_mat_Li = prob_material(sc, mat, sf, ls.dir, o, lam.l[0], pa.sample_p);
+            const float p_L_i = ls.prob * pl;
+            w_L = p_L_i * p_L_i / (p_L_i * p_L_i + p_mat_Li * p_mat_Li);
+            w_mat = ms.prob * ms.prob / (ms.prob * ms.prob + p_L_mati * p_L_mati);
+          }
+          // NEE: the term, and ∂term/∂coef = β w_L mult / (p_L p_l)
+          float pend[H], wn[H];
+#pragma unroll
+          for (int k = 0; k < H; k++) {
+            pend[k] = ms.ok ? beta[k] * (lis[k] * w_L) : beta[k] * lis[k];
+            wn[k] = a[k] * (beta[k] * w_L * ls.mult[k] / (ls.prob * pl));
+          }
+          float un[H], vn[H];
+          if (diff) bsdf_term<H>(sc, mat, lam, ls.dir, o, nn, sf.front, wn, false, un, vn);
+          if (ls.shadow) {
+#pragma unroll
+            for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + p] = pend[k];
+            if (diff) {
+              rd.pn_mat[p] = sf.mat | kPendMark;
+#pragma unroll
+              for (int k = 0; k < H; k++) {
+                rd.pn_u[size_t(k) * cap + p] = un[k];
+                rd.pn_v[size_t(k) * cap + p] = vn[k];
+              }
+            }
+            push_shadow = true;
+            shd = ls.sray;
+          } else {
+#pragma unroll
+            for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] -= pend[k];
+            if (diff) {  // no shadow ray decides this NEE term: it is due now
+              float ones[H];
+#pragma unroll
+              for (int k = 0; k < H; k++) ones[k] = 1.f;
+              accumulate_term<H>(gl, dv, mat, sf.mat, lam, un, vn, ones);
+            }
+          }
+          if (ms.ok) {
+            if (diff) {  // the sampled factor, (∂f/∂θ)/f a, waits for L_after
+              float uf[H], vf[H];
+              bsdf_term<H>(sc, mat, lam, ms.dir, o, nn, sf.front, a, true, uf, vf);
+              rd.pf_mat[p] = sf.mat;
+#pragma unroll
+              for (int k = 0; k < H; k++) {
+                rd.pf_u[size_t(k) * cap + p] = uf[k];
+                rd.pf_v[size_t(k) * cap + p] = vf[k];
+              }
+            }
+#pragma unroll
+            for (int k = 0; k < H; k++) ws.beta[size_t(k) * cap + p] = beta[k] * (ms.mult[k] / ms.prob * w_mat);
+            push_next = true;
+            nxt.o = sf.p;
+            nxt.d = ms.dir;
+          }
+        }
+      }
+    }
+    uint32_t slot[2];
+    block_append(sq, par, {push_next, push_shadow}, {ctrl + CTRL_COUNT0 + (qsel ^ 1u), ctrl + CTRL_SHADOW}, slot);
+    if (push_next) {
+      rout[2 * size_t(slot[0])] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
+      rout[2 * size_t(slot[0]) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
+    }
+    if (push_shadow) {
+      float4 *sr = reinterpret_cast<float4 *>(ws.sray);
+      sr[2 * size_t(slot[1])] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
+      sr[2 * size_t(slot[1]) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
+      ws.spath[slot[1]] = p;
+    }
+  }
+  grad_lds_flush(gl, dv);
+}
+
+// any hit; a visible light subtracts the NEE term from L and makes the
+// pending NEE gradient due
+template <int H>
+__global__ __launch_bounds__(64) void k_adj_shadow(DevScene sc, WaveState ws, ReplayDev rd,
+                                                   uint32_t *__restrict__ ctrl) {
+  __shared__ uint32_t stack[kLdsStack * 64];
+  const uint32_t n = ctrl[CTRL_SHADOW];
+  const float4 *__restrict__ sr = reinterpret_cast<const float4 *>(ws.sray);
+  const size_t cap = ws.cap;
+  auto load = [&](uint32_t i) {
+    const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
+    Ray r;
+    r.o = mk(a.x, a.y, a.z);
+    r.tmax = a.w;
+    r.d = mk(b.x, b.y, b.z);
+    r.skip = __float_as_uint(b.w);
+    return r;
+  };
+  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &) {
+    if (ovf) atomicOr(ctrl + 15, 1u);
+    if (h.obj == LAM_NONE && !ovf) {
+      const uint32_t p = ws.spath[i];
+#pragma unroll
+      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] -= ws.pend[size_t(k) * cap + p];
+      const uint32_t m = rd.pn_mat[p];
+      if (m != LAM_NONE) rd.pn_mat[p] = m & ~kPendMark;
+    }
+  };
+  const SplitStack stk = SplitStack::of(stack, ws.spill, blockIdx.x, gridDim.x);
+  trace_queue<true, false>(sc, n, ctrl + CTRL_FETCH_SHD, stk, load, done);
+}
+
+// after the last iteration: the NEE terms of paths that ended with a visible
+// light sample (the slots of the pass, grid-stride)
+template <int H>
+__global__ __launch_bounds__(kShadeBlock) void k_adj_final(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
+                                                           ReplayDev rd, DiffDev dv, uint32_t n,
+                                                           const uint32_t *__restrict__ ctrl) {
+  extern __shared__ float gl[];
+  if (blockIdx.x * kShadeBlock >= n) return;
+  grad_lds_init(gl, dv);
+  for (uint32_t p = blockIdx.x * kShadeBlock + threadIdx.x; p < n; p += gridDim.x * kShadeBlock) {
+    const uint32_t m = rd.pn_mat[p];
+    if (m == LAM_NONE) continue;
+    if (!(m & kPendMark)) {
+      uint32_t pixel, sample;
+      path_id(ids, ctrl, p, pixel, sample);
+      Lam<H> lam;
+      lam_for<H>(sc, lam_path_key(pa.seed, pixel, sample), lam);
+      float u[H], v[H], ones[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        u[k] = rd.pn_u[size_t(k) * ws.cap + p];
+        v[k] = rd.pn_v[size_t(k) * ws.cap + p];
+        ones[k] = 1.f;
+      }
+      accumulate_term<H>(gl, dv, sc.materials[m], m, lam, u, v, ones);
+    }
+    rd.pn_mat[p] = LAM_NONE;
+  }
+  grad_lds_flush(gl, dv);
+}
+
+// grad[j] += Σ_r part[r][j] over `rows` partial rows (one thread per parameter)
+__global__ void k_adj_sum_rows(const float *__restrict__ part, uint32_t rows, uint32_t n_params,
+                               float *__restrict__ grad) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_params) return;
+  float s = 0.f;
+  for (uint32_t r0 = 0; r0 < rows; r0 += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = r0 + u < rows ? part[size_t(r0 + u) * n_params + j] : 0.f;
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  grad[j] += s;
+}
+
+// ---- host side ---------------------------------------------------------------------------
+template <int H>
+static hipError_t enqueue_replay_t(const DevScene &sc, const PathArgs &pa, const PathIds &ids, const WaveState &ws,
+                                   const ReplayDev &rd, const DiffDev &dv, uint32_t n, uint32_t *ctrl,
+                                   const float *dloss, float inv_n, const WaveLaunch &wl, hipStream_t st) {
+  const uint32_t b256 = (n + 255) / 256;
+  const size_t gl_bytes = size_t(dv.n_params) * 4;
+  hipLaunchKernelGGL((k_adj_raygen<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, rd, n, ctrl, dloss, inv_n);
+  uint32_t q = 0;
+  const uint32_t iters = pa.max_scat + 2;
+  hipError_t e = hipSuccess;
+  for (uint32_t s = 0; s < iters && e == hipSuccess; s++) {
+    e = enqueue_iter_extend(sc, ws, ctrl, q, wl, st);
+    hipLaunchKernelGGL((k_adj_pending<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), gl_bytes, st, sc, pa, ids, ws,
+                       rd, dv, q, ctrl);
+    hipLaunchKernelGGL((k_adj_shade<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), gl_bytes, st, sc, pa, ids, ws, rd,
+                       dv, s, q, ctrl);
+    hipLaunchKernelGGL((k_adj_shadow<H>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, rd, ctrl);
+    q ^= 1u;
+  }
+  hipLaunchKernelGGL((k_adj_final<H>), dim3(std::min(b256, wl.shade_blocks)), dim3(kShadeBlock), gl_bytes, st, sc, pa,
+                     ids, ws, rd, dv, n, ctrl);
+  return e != hipSuccess ? e : hipGetLastError();
+}
+
+hipError_t enqueue_replay(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset, uint32_t m,
+                          const WaveState &ws, const ReplayBufs &rb, const int32_t *mat_d, const int32_t *mat_ior,
+                          const int32_t *mat_albedo, uint32_t n_params, float *part, uint32_t n, uint32_t *ctrl,
+                          const float *dloss, float inv_n, const WaveLaunch &wl, hipStream_t st) {
+  const PathIds ids{nullptr, nullptr, pixel_offset, m > 1 ? n / m : 0u};
+  const DiffDev dv{mat_d, mat_ior, mat_albedo, n_params, part};
+  const size_t cap = ws.cap;
+  const size_t hf = size_t(hero) * cap;
+  ReplayDev rd;
+  rd.a = rb.f;
+  rd.pf_u = rb.f + hf;
+  rd.pf_v = rb.f + 2 * hf;
+  rd.pn_u = rb.f + 3 * hf;
+  rd.pn_v = rb.f + 4 * hf;
+  rd.pf_mat = rb.u;
+  rd.pn_mat = rb.u + cap;
+  if (hero == 16) return enqueue_replay_t<16>(sc, pa, ids, ws, rd, dv, n, ctrl, dloss, inv_n, wl, st);
+  return enqueue_replay_t<4>(sc, pa, ids, ws, rd, dv, n, ctrl, dloss, inv_n, wl, st);
+}
+
+hipError_t enqueue_sum_rows(const float *part, uint32_t rows, uint32_t n_params, float *grad, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_sum_rows, dim3((n_params + 255) / 256), dim3(256), 0, st, part, rows, n_params, grad);
   return hipGetLastError();
 }
 

@@ -90,6 +90,11 @@ struct lam_ctx {
     uint32_t *spill = nullptr;  // traversal-stack spill columns (WaveState::spill)
     uint32_t spill_lanes = 0;
     unsigned long long *rays = nullptr;  // ray counters of the lane's passes (WaveState::rays)
+    // adjoint replay state (lam::ReplayBufs) for rep_cap slots of rep_hero wavelengths
+    void *rep_mem = nullptr;
+    size_t rep_cap = 0;
+    int rep_hero = 0;
+    lam::ReplayBufs rep{};
   } lane[kMaxLanes];
   // block counts are set on the first render (they depend on the CU count);
   // the finish kernel takes over below 131,072 paths or after 12 bounces
@@ -430,6 +435,25 @@ int ensure_wave(lam_ctx *ctx, size_t cap, int hero, int l = 0) {
 }
 
 // guiding-network output buffer of lane l (80 B per path slot)
+// the adjoint's replay state for the lane's path slots (its rows are
+// ws.cap apart, so it follows the wave state's capacity)
+int ensure_replay(lam_ctx *ctx, int l, int hero) {
+  lam_ctx::Lane &ln = ctx->lane[l];
+  const size_t cap = ln.ws.cap;
+  if (ln.rep_mem && ln.rep_cap >= cap && ln.rep_hero >= hero) return LAM_OK;
+  if (ln.rep_mem) HIPC(hipFree(ln.rep_mem));
+  ln.rep_mem = nullptr;
+  ln.rep_cap = 0;
+  const size_t fb = (5 * size_t(hero) * cap * 4 + 255) / 256 * 256;
+  HIPC(hipMalloc(&ln.rep_mem, fb + 2 * cap * 4));
+  char *b = static_cast<char *>(ln.rep_mem);
+  ln.rep.f = reinterpret_cast<float *>(b);
+  ln.rep.u = reinterpret_cast<uint32_t *>(b + fb);
+  ln.rep_cap = cap;
+  ln.rep_hero = hero;
+  return LAM_OK;
+}
+
 int ensure_mix(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
   if (ln.mix_cap >= ln.ws.cap) return LAM_OK;
@@ -836,6 +860,7 @@ void lam_destroy(lam_ctx *ctx) {
     if (ln.ctrl) (void)hipFree(ln.ctrl);
     if (ln.rays) (void)hipFree(ln.rays);
     if (ln.spill) (void)hipFree(ln.spill);
+    if (ln.rep_mem) (void)hipFree(ln.rep_mem);
     for (hipEvent_t e : {ln.ev_film, ln.ev_join})
       if (e) (void)hipEventDestroy(e);
     if (ln.stream) (void)hipStreamDestroy(ln.stream);
@@ -1296,60 +1321,71 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
   const size_t npix = size_t(w) * h;
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
-  // scratch: [0] parameter maps, [1] dloss, [2] gradient + one fetch counter per lane
+  if ((rc = ensure_launch(ctx))) return rc;
+  // scratch: [0] parameter maps, [1] dloss, [2] gradient, then the replay
+  // kernels' partial rows (shade_blocks per lane)
   const size_t maps = size_t(nm) * 3 * 4;
+  const size_t gbytes = (size_t(dp->n_params) * 4 + 255) / 256 * 256;
+  const uint32_t part_rows = ctx->wl.shade_blocks;
+  const size_t part_bytes = size_t(part_rows) * dp->n_params * 4;
   if ((rc = ensure_scratch(ctx, 0, maps))) return rc;
   if ((rc = ensure_scratch(ctx, 1, npix * 12))) return rc;
-  if ((rc = ensure_scratch(ctx, 2, size_t(dp->n_params) * 4 + 16 * kMaxLanes + 16))) return rc;
+  const size_t sbytes = (gbytes + kMaxLanes * part_bytes + 15) / 16 * 16;
+  if ((rc = ensure_scratch(ctx, 2, sbytes))) return rc;
   int32_t *dmaps = static_cast<int32_t *>(ctx->scratch[0]);
   HIPC(hipMemcpyAsync(dmaps, dp->mat_d, nm * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(dmaps + nm, dp->mat_ior, nm * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(dmaps + 2 * nm, dp->mat_albedo, nm * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemcpyAsync(ctx->scratch[1], dloss_drgb, npix * 12, hipMemcpyHostToDevice, ctx->stream));
   float *dgrad = static_cast<float *>(ctx->scratch[2]);
-  const size_t gbytes = (size_t(dp->n_params) * 4 + 15) / 16 * 16;
-  uint32_t *ctr0 = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(ctx->scratch[2]) + gbytes);
-  HIPC(lam::launch_zero(dgrad, gbytes, ctx->stream));
+  float *dpart = dgrad + gbytes / 4;
+  HIPC(lam::launch_zero(dgrad, sbytes, ctx->stream));
   if ((rc = clear_err_flag(ctx))) return rc;
-  // Forward passes: the render's wavefront pass without a film, which
-  // leaves each path's total radiance in its lane's path state; the replay
-  // of the pass follows on the same lane.  Consecutive passes go round-robin
-  // to the pass lanes, so one lane's forward pass runs beside the others'
-  // replays; every replay adds into the one gradient with atomics.
-  const size_t cap = std::min(npix, kMaxWaveCap);
-  const uint32_t lanes = (p->flags & LAM_FLAG_NO_PIPE) ? 1u : std::min(ctx->lanes, p->spp);
+  // Passes of m samples of every pixel of a chunk, as lam_render's: the
+  // forward pass (the render's wavefront pass without a film, which leaves
+  // each path's total radiance in the lane's path state), then the replay
+  // pass of the same slots on the same lane.  Consecutive passes go
+  // round-robin to the pass lanes, so one lane's forward pass runs beside
+  // another's replay; every replay adds into the one gradient with atomics.
+  const size_t cpix = std::min(npix, kMaxWaveCap);
+  const uint32_t m_cfg =
+      npix <= kMaxWaveCap
+          ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass, kMaxPassSlots / npix)))
+          : 1u;
+  const uint32_t m = std::min(m_cfg, p->spp);
+  const size_t cap = cpix * m;
+  const uint32_t npass = (p->spp + m - 1) / m;
+  const uint32_t lanes = (p->flags & LAM_FLAG_NO_PIPE) ? 1u : std::min(ctx->lanes, npass);
   hipStream_t st[kMaxLanes];
   for (uint32_t l = 0; l < lanes; l++) {
     if ((rc = ensure_wave(ctx, cap, int(p->hero), int(l)))) return rc;
+    if ((rc = ensure_replay(ctx, int(l), int(p->hero)))) return rc;
     st[l] = l == 0 ? ctx->stream : ctx->lane[l].stream;
     HIPC(lam::launch_zero(ctx->lane[l].ctrl, lam::kCtrlWords * 4, ctx->stream));
   }
   const lam::PathArgs pa = path_args(p);
   lam::WaveLaunch wl = launch_for(ctx, p);
-  // 8 blocks per CU (within the finish grid the spill columns are sized for)
-  const uint32_t blocks = std::min(uint32_t(ctx->cus) * 8u, ctx->wl.finish_blocks);
   const float inv_n = 1.f / float(uint64_t(p->spp) * p->hero);
   HIPC(hipEventRecord(ctx->ev[0], ctx->stream));
   HIPC(hipEventRecord(ctx->lane[0].ev_join, ctx->stream));
   for (uint32_t l = 1; l < lanes; l++) HIPC(hipStreamWaitEvent(st[l], ctx->lane[0].ev_join, 0));
   hipError_t e = hipSuccess;
-  for (size_t off = 0; off < npix && e == hipSuccess; off += cap) {
-    const uint32_t n = uint32_t(std::min(cap, npix - off));
-    wl.tail_threshold = std::min(launch_for(ctx, p).tail_threshold, n / ctx->tail_div);
-    for (uint32_t s = 0; s < p->spp && e == hipSuccess; s++) {
-      const uint32_t l = s % lanes;
+  uint32_t gi = 0;
+  for (size_t off = 0; off < npix && e == hipSuccess; off += cpix) {
+    const uint32_t n = uint32_t(std::min(cpix, npix - off));
+    for (uint32_t first = 0; first < p->spp && e == hipSuccess; first += m, gi++) {
+      const uint32_t mg = std::min(m, p->spp - first);
+      const uint32_t l = gi % lanes;
       lam_ctx::Lane &ln = ctx->lane[l];
-      uint32_t *ctr = ctr0 + 4 * l;
-      e = lam::enqueue_pass_set(ln.ctrl, p->spp_offset + s, st[l]);
+      wl.tail_threshold = std::min(launch_for(ctx, p).tail_threshold, n * mg / ctx->tail_div);
+      e = lam::enqueue_pass_set(ln.ctrl, p->spp_offset + first, st[l]);
       if (e == hipSuccess)
-        e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ln.ws, n, ln.ctrl, nullptr,
-                              nullptr, wl, st[l], nullptr, nullptr, false);
-      if (e == hipSuccess) e = lam::launch_zero(ctr, 16, st[l]);
+        e = lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ln.ws, n * mg, ln.ctrl,
+                              nullptr, nullptr, wl, st[l], nullptr, nullptr, false, mg);
       if (e == hipSuccess)
-        e = lam::launch_adjoint(int(p->hero), ctx->sc, pa, dmaps, dmaps + nm, dmaps + 2 * nm, dp->n_params,
-                                p->spp_offset + s, n, uint32_t(off), ln.ws.L, ln.ws.cap, inv_n,
-                                static_cast<float *>(ctx->scratch[1]), dgrad, ctr, ctx->d_err, ln.spill, blocks,
-                                st[l]);
+        e = lam::enqueue_replay(int(p->hero), ctx->sc, pa, uint32_t(off), mg, ln.ws, ln.rep, dmaps, dmaps + nm,
+                                dmaps + 2 * nm, dp->n_params, dpart + size_t(l) * part_rows * dp->n_params, n * mg,
+                                ln.ctrl, static_cast<float *>(ctx->scratch[1]), inv_n, wl, st[l]);
     }
   }
   // join: stream 0 waits for the other lanes
@@ -1359,6 +1395,7 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
     if (e == hipSuccess) e = ej;
   }
   HIPC(e);
+  HIPC(lam::enqueue_sum_rows(dpart, lanes * part_rows, dp->n_params, dgrad, ctx->stream));
   HIPC(hipEventRecord(ctx->ev[1], ctx->stream));
   HIPC(hipMemcpyAsync(grad, dgrad, size_t(dp->n_params) * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));

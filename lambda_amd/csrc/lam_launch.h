@@ -155,11 +155,27 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
 
 // ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------
 constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block
-hipError_t launch_adjoint(int hero, const DevScene &sc, const PathArgs &pa, const int32_t *mat_d,
-                          const int32_t *mat_ior, const int32_t *mat_albedo, uint32_t n_params, uint32_t sample,
-                          uint32_t n, uint32_t pixel_offset, const float *Lfwd, size_t lcap, float inv_n,
-                          const float *dloss, float *grad, uint32_t *ctr, uint32_t *err, uint32_t *spill,
-                          uint32_t blocks, hipStream_t st);
+// replay state of one pass lane: f = 5 hero x cap floats (adjoint weights,
+// pending sampled factor u/v, pending NEE term u/v), u = 2 cap words (their
+// materials)
+struct ReplayBufs {
+  float *f;
+  uint32_t *u;
+};
+// The replay pass of the n slots of the forward pass just traced on the
+// same lane (m samples of n/m pixels from pixel_offset, ws.L = each path's
+// total radiance).  Its kernels add their blocks' gradient sums into the
+// lane's partial rows `part` (wl.shade_blocks rows of n_params floats).
+hipError_t enqueue_replay(int hero, const DevScene &sc, const PathArgs &pa, uint32_t pixel_offset, uint32_t m,
+                          const WaveState &ws, const ReplayBufs &rb, const int32_t *mat_d, const int32_t *mat_ior,
+                          const int32_t *mat_albedo, uint32_t n_params, float *part, uint32_t n, uint32_t *ctrl,
+                          const float *dloss, float inv_n, const WaveLaunch &wl, hipStream_t st);
+// grad += the sum of `rows` partial rows
+hipError_t enqueue_sum_rows(const float *part, uint32_t rows, uint32_t n_params, float *grad, hipStream_t st);
+// one bounce iteration's start in a pass without a tail: counters reset and
+// the closest hits of extend queue q (the render's k_iter_begin and k_extend)
+hipError_t enqueue_iter_extend(const DevScene &sc, const WaveState &ws, uint32_t *ctrl, uint32_t q,
+                               const WaveLaunch &wl, hipStream_t st);
 hipError_t enqueue_pass_next(uint32_t *ctrl, uint32_t stride, hipStream_t st);
 
 }  // namespace lam

@@ -1,0 +1,108 @@
+// Pieces of the wavefront integrator shared by its kernel files
+// (lam_wavefront.hip: the render passes; lam_adjoint.hip: the adjoint's
+// replay passes): the control-block words, path identities and the
+// block-aggregated queue appends.
+#pragma once
+#include "lam_launch.h"
+#include "lam_shade.h"
+
+namespace lam {
+
+// control block (uint32 words)
+enum : uint32_t {
+  CTRL_COUNT0 = 0,  // extend queue ping
+  CTRL_COUNT1 = 1,  // extend queue pong
+  CTRL_SHADOW = 2,  // shadow queue count
+  CTRL_FETCH_EXT = 3,
+  CTRL_FETCH_SHD = 4,
+  CTRL_TAIL = 5,    // 1: the current queue is finished by k_finish
+  CTRL_FETCH_FIN = 6,
+  CTRL_PASS = 8,    // current pass (sample) index
+  CTRL_GUIDE_DROP = 12,  // guided continuations dropped (weight not finite)
+  CTRL_GUIDE_ZERO = 13,  // guide-branch samples with a zero mixture density
+  CTRL_WORDS = 16
+};
+
+// threads per shade block (the block appends aggregate this many entries)
+constexpr uint32_t kShadeBlock = 256;
+// largest spectra table the shade kernel stages in LDS (3 blocks per CU)
+constexpr uint32_t kSpecLdsBytes = 48 * 1024;
+
+// Identity of a path slot.  From lists (test hooks), or a pass of m samples
+// of npix pixels: slot = q npix + i is pixel pixel_offset + i, sample
+// ctrl[CTRL_PASS] + q (npix = 0: one sample, slot = i).
+struct PathIds {
+  const uint32_t *pixel;   // nullable
+  const uint32_t *sample;  // nullable
+  uint32_t pixel_offset;
+  uint32_t npix;
+};
+
+__device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
+                                        uint32_t &sample) {
+  if (ids.pixel) {
+    pixel = ids.pixel[slot];
+    sample = ids.sample[slot];
+    return;
+  }
+  uint32_t q = 0;
+  if (ids.npix)
+    while (slot >= ids.npix) {  // a few samples per pass at most
+      slot -= ids.npix;
+      q++;
+    }
+  pixel = slot + ids.pixel_offset;
+  sample = ctrl[CTRL_PASS] + q;
+}
+
+template <int H>
+__device__ __forceinline__ float lam_for(const DevScene &sc, uint64_t key, Lam<H> &L) {
+  return hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), L);
+}
+
+// Block-aggregated append to NQ queues: one atomic per queue per block
+// iteration instead of one per wave.  Same-address atomics on the queue
+// counts are device-scope and serialise at the memory side across the 8
+// XCDs; per-wave appends cost the shade kernel more than half its time.
+// Every thread of the block calls this (two barriers).  The per-wave counts
+// are double-buffered by iteration parity `par`, so the next iteration may
+// start writing before every wave has read this one's offsets.  Within a
+// block, queue order is wave order then lane order.
+template <int WAVES, int NQ>
+struct BlockQueues {
+  uint32_t cnt[2][NQ][WAVES];
+  uint32_t base[2][NQ];
+};
+template <int WAVES, int NQ>
+__device__ __forceinline__ void block_append(BlockQueues<WAVES, NQ> &sm, uint32_t par, const bool (&want)[NQ],
+                                             uint32_t *const (&count)[NQ], uint32_t (&slot)[NQ]) {
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  uint64_t m[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) {
+    m[k] = __ballot(want[k]);
+    if (lane == 0) sm.cnt[par][k][wave] = __popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const uint32_t k = threadIdx.x;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; w++) {
+      const uint32_t c = sm.cnt[par][k][w];
+      sm.cnt[par][k][w] = tot;
+      tot += c;
+    }
+    uint32_t *ck = count[0];
+#pragma unroll
+    for (int j = 1; j < NQ; j++)
+      if (k == uint32_t(j)) ck = count[j];
+    sm.base[par][k] = tot ? atomicAdd(ck, tot) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NQ; k++) slot[k] = sm.base[par][k] + sm.cnt[par][k][wave] + __popcll(m[k] & below);
+}
+
+}  // namespace lam

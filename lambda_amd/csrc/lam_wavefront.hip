@@ -26,60 +26,9 @@
 #include "lam_guide.h"
 #include "lam_launch.h"
 #include "lam_shade.h"
+#include "lam_wave.h"
 
 namespace lam {
-
-// control block (uint32 words)
-enum : uint32_t {
-  CTRL_COUNT0 = 0,  // extend queue ping
-  CTRL_COUNT1 = 1,  // extend queue pong
-  CTRL_SHADOW = 2,  // shadow queue count
-  CTRL_FETCH_EXT = 3,
-  CTRL_FETCH_SHD = 4,
-  CTRL_TAIL = 5,    // 1: the current queue is finished by k_finish
-  CTRL_FETCH_FIN = 6,
-  CTRL_PASS = 8,    // current pass (sample) index
-  CTRL_GUIDE_DROP = 12,  // guided continuations dropped (weight not finite)
-  CTRL_GUIDE_ZERO = 13,  // guide-branch samples with a zero mixture density
-  CTRL_WORDS = 16
-};
-
-// threads per shade block (the block appends aggregate this many entries)
-constexpr uint32_t kShadeBlock = 256;
-// largest spectra table the shade kernel stages in LDS (3 blocks per CU)
-constexpr uint32_t kSpecLdsBytes = 48 * 1024;
-
-// Identity of a path slot.  From lists (test hooks), or a pass of m samples
-// of npix pixels: slot = q npix + i is pixel pixel_offset + i, sample
-// ctrl[CTRL_PASS] + q (npix = 0: one sample, slot = i).
-struct PathIds {
-  const uint32_t *pixel;   // nullable
-  const uint32_t *sample;  // nullable
-  uint32_t pixel_offset;
-  uint32_t npix;
-};
-
-__device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
-                                        uint32_t &sample) {
-  if (ids.pixel) {
-    pixel = ids.pixel[slot];
-    sample = ids.sample[slot];
-    return;
-  }
-  uint32_t q = 0;
-  if (ids.npix)
-    while (slot >= ids.npix) {  // a few samples per pass at most
-      slot -= ids.npix;
-      q++;
-    }
-  pixel = slot + ids.pixel_offset;
-  sample = ctrl[CTRL_PASS] + q;
-}
-
-template <int H>
-__device__ __forceinline__ float lam_for(const DevScene &sc, uint64_t key, Lam<H> &L) {
-  return hero_lambdas<H>(sc, lam_draw_f(key, LAM_DIM_HERO), L);
-}
 
 // ---- ray generation: render (src/renderer.cpp:35-40) ------------------------------
 template <int H>
@@ -179,51 +128,6 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
 }
 
 // ---- shade ------------------------------------------------------------------------------
-// Block-aggregated append to NQ queues: one atomic per queue per block
-// iteration instead of one per wave.  Same-address atomics on the queue
-// counts are device-scope and serialise at the memory side across the 8
-// XCDs; per-wave appends cost the shade kernel more than half its time.
-// Every thread of the block calls this (two barriers).  The per-wave counts
-// are double-buffered by iteration parity `par`, so the next iteration may
-// start writing before every wave has read this one's offsets.  Within a
-// block, queue order is wave order then lane order.
-template <int WAVES, int NQ>
-struct BlockQueues {
-  uint32_t cnt[2][NQ][WAVES];
-  uint32_t base[2][NQ];
-};
-template <int WAVES, int NQ>
-__device__ __forceinline__ void block_append(BlockQueues<WAVES, NQ> &sm, uint32_t par, const bool (&want)[NQ],
-                                             uint32_t *const (&count)[NQ], uint32_t (&slot)[NQ]) {
-  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const uint64_t below = (uint64_t(1) << lane) - 1;
-  uint64_t m[NQ];
-#pragma unroll
-  for (int k = 0; k < NQ; k++) {
-    m[k] = __ballot(want[k]);
-    if (lane == 0) sm.cnt[par][k][wave] = __popcll(m[k]);
-  }
-  __syncthreads();
-  if (threadIdx.x < NQ) {
-    const uint32_t k = threadIdx.x;
-    uint32_t tot = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; w++) {
-      const uint32_t c = sm.cnt[par][k][w];
-      sm.cnt[par][k][w] = tot;
-      tot += c;
-    }
-    uint32_t *ck = count[0];
-#pragma unroll
-    for (int j = 1; j < NQ; j++)
-      if (k == uint32_t(j)) ck = count[j];
-    sm.base[par][k] = tot ? atomicAdd(ck, tot) : 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NQ; k++) slot[k] = sm.base[par][k] + sm.cnt[par][k][wave] + __popcll(m[k] & below);
-}
-
 // H = 4: at least 3 waves per SIMD; the register budget then stays under
 // 168 VGPRs without spills (2 waves at the default allocation), and the
 // latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
@@ -1013,6 +917,14 @@ hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32
     hipLaunchKernelGGL((k_film<16>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
   else
     hipLaunchKernelGGL((k_film<4>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_iter_extend(const DevScene &sc, const WaveState &ws, uint32_t *ctrl, uint32_t q,
+                               const WaveLaunch &wl, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, st, ctrl, q ^ 1u, 0u, 0u, nullptr);
+  hipLaunchKernelGGL((k_extend<false>), dim3(wl.trace_blocks), dim3(64), 0, st, sc,
+                     reinterpret_cast<const float4 *>(ws.ray[q]), ws.hits, ctrl, q, ws.spill, nullptr);
   return hipGetLastError();
 }
 
