@@ -553,6 +553,8 @@ def main():
                     help="where the glossy interior hides its light (scenes.glossy_interior)")
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0, help="pass lanes (0 = the library's default)")
+    ap.add_argument("--max-iters", type=int, default=0,
+                    help="bounce iterations of a pass before its tail goes to the finish kernel (0 = the library's)")
     ap.add_argument("--samples-per-pass", type=int, default=0,
                     help="samples of every pixel per wavefront pass (0 = the library's default)")
     ap.add_argument("--extend-only", action="store_true",
@@ -601,6 +603,8 @@ def main():
         ctx.set_wave_config(samples_per_pass=args.samples_per_pass)
     if args.lanes:
         ctx.set_wave_config(lanes=args.lanes)
+    if args.max_iters:
+        ctx.set_wave_config(max_iters=args.max_iters)
     st = ctx.stats()
     log(f"scene: {scene.n_tris} tris, {st.n_nodes} BVH8 nodes, depth {st.depth}, stack bound "
         f"{st.stack_bound}, setup {time.perf_counter() - t0:.1f}s")
@@ -617,10 +621,12 @@ def main():
         print(json.dumps({"extend": rl, "shadow": sl, "copy_bytes": nbytes}), flush=True)
         ctx.close()
         return
-    # the warm-up renders at least one full wavefront pass (samples_per_pass
-    # samples of every pixel), so that every lane's path state and graph
-    # exist before the timed render, whatever W is
-    warm_spp = max(args.warmup, ctx.wave_config()["samples_per_pass"]) if args.warmup else 0
+    # the warm-up renders at least two full wavefront passes of the timed
+    # render's shape (min(samples_per_pass, K/2) samples of every pixel
+    # each), so that the lanes' path state exists before the timed render,
+    # whatever W is
+    m_pass = max(1, min(ctx.wave_config()["samples_per_pass"], args.steps // 2))
+    warm_spp = max(args.warmup, 2 * m_pass) if args.warmup else 0
     if warm_spp:
         ctx.render(la.params(spp=warm_spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
                              spp_offset=1 << 20, flags=args.flags))

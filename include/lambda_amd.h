@@ -418,9 +418,10 @@ typedef struct {
   uint32_t batch_per_cu;   /* ... for ray batches (lam_extend, lam_occluded, lam_extend_dev): 1..32 (default 24) */
   uint32_t finish_per_cu;  /* finish-kernel blocks (64 lanes) per CU: 1..32 (default 10) */
   uint32_t shade_per_cu;   /* shade-kernel blocks (256 lanes) per CU: 1..8 (default 3) */
-  uint32_t samples_per_pass; /* samples of every pixel one plain wavefront pass traces together: 1..8
-                                (default 8; frames of at most 4 Mi pixels, at most 16 Mi paths per pass; the film still adds
-                                each pixel's samples in sample order) */
+  uint32_t samples_per_pass; /* at most this many samples of every pixel one wavefront pass traces together:
+                                1..32 (default 32); a render of spp samples takes min(this, spp / 2) per pass, at
+                                most 64 Mi paths per pass (frames of at most 64 Mi pixels); the film still adds each
+                                pixel's samples in sample order */
 } lam_wave_config;
 int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out);
 int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg);

@@ -119,7 +119,7 @@ struct lam_ctx {
   uint32_t *trace_ctr = nullptr;  // fetch counters of the batch trace kernel (one per launch)
   uint32_t lanes = 3;     // passes in flight
   uint32_t tail_div = 16;  // the finish kernel's threshold is at most 1/tail_div of a chunk
-  uint32_t samples_per_pass = 8;  // samples of every pixel traced together in one plain wavefront pass
+  uint32_t samples_per_pass = 32;  // at most this many samples of every pixel traced together in one pass
   std::vector<hipEvent_t> prof_ev;  // profile mode: 4 + 4 per iteration
   float box_lo[3] = {0.f, 0.f, 0.f}, box_hi[3] = {1.f, 1.f, 1.f};  // scene box (guiding inputs)
   // guiding network: fp32 master parameters, Adam moments, gradient,
@@ -140,9 +140,10 @@ struct lam_ctx {
 // Path slots per wavefront pass.  The frame is processed in chunks of this
 // many paths; 4 Mi slots with 16 hero wavelengths is ~0.9 GB of path state.
 static constexpr size_t kMaxWaveCap = size_t(1) << 22;
-// path slots of one plain wavefront pass of several samples per pixel
-// (samples_per_pass): ~3.6 GB of path state per lane at 16 hero wavelengths
-static constexpr size_t kMaxPassSlots = size_t(1) << 24;
+// path slots of one wavefront pass of several samples per pixel
+// (samples_per_pass): up to ~11 GB of path state per lane at 4 hero
+// wavelengths (HBM is 288 GB)
+static constexpr size_t kMaxPassSlots = size_t(1) << 26;
 
 namespace {
 
@@ -454,6 +455,20 @@ int ensure_replay(lam_ctx *ctx, int l, int hero) {
   return LAM_OK;
 }
 
+// Shape of the wavefront passes of a frame of npix pixels: a frame of at
+// most kMaxPassSlots pixels is one chunk, each pass tracing m samples of
+// every pixel, m = min(samples_per_pass, spp / 2, kMaxPassSlots / npix)
+// (at least 1): longer queues and one tail per m samples, and still two
+// passes or more for the lanes to overlap (measured best at 1080p for 16 to
+// 256 spp); a larger frame goes in chunks of kMaxWaveCap pixels, one sample
+// per pass.
+size_t pass_pixels(size_t npix) { return npix <= kMaxPassSlots ? npix : kMaxWaveCap; }
+uint32_t pass_samples(const lam_ctx *ctx, size_t npix, uint32_t spp) {
+  if (npix > kMaxPassSlots) return 1u;
+  const size_t m = std::min<size_t>({size_t(ctx->samples_per_pass), size_t(spp / 2), kMaxPassSlots / npix});
+  return uint32_t(std::max<size_t>(1, m));
+}
+
 int ensure_mix(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
   if (ln.mix_cap >= ln.ws.cap) return LAM_OK;
@@ -607,16 +622,10 @@ struct GuideRun {
 // LAM_FLAG_NO_GRAPH launches without a graph.
 int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size_t npix,
                      const GuideRun *gr = nullptr) {
-  // pixels per chunk, and samples per wavefront pass: a render of a frame
-  // that fits takes up to samples_per_pass samples of every pixel into one
-  // pass (longer queues, one tail per several samples); a guided render's
+  // pixels per chunk and samples per pass (pass_samples); a guided render's
   // training passes keep one sample each
-  const size_t cpix = std::min(npix, kMaxWaveCap);
-  const uint32_t m_cfg =
-      npix <= kMaxWaveCap
-          ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass, kMaxPassSlots / npix)))
-          : 1u;
-  const size_t cap = cpix * std::min(m_cfg, std::max(p->spp, 1u));
+  const size_t cpix = pass_pixels(npix);
+  const size_t cap = cpix * pass_samples(ctx, npix, p->spp);
   const bool profile = p->flags & LAM_FLAG_PROFILE;
   const bool graph = !(p->flags & (LAM_FLAG_NO_GRAPH | LAM_FLAG_PROFILE));
   // every configured lane gets its state on the first render, also when
@@ -712,7 +721,7 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
     if (first >= p->spp) continue;
     // passes of m samples of every pixel of the chunk (the last may hold fewer)
     const uint32_t left = p->spp - first;
-    const uint32_t m = std::min(m_cfg, left);
+    const uint32_t m = pass_samples(ctx, npix, left);
     const uint32_t ng = (left + m - 1) / m, nfull = left / m;
     const uint32_t nl = std::min(lanes, ng);
     const uint32_t npass = n * m;
@@ -1347,12 +1356,8 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
   // pass of the same slots on the same lane.  Consecutive passes go
   // round-robin to the pass lanes, so one lane's forward pass runs beside
   // another's replay; every replay adds into the one gradient with atomics.
-  const size_t cpix = std::min(npix, kMaxWaveCap);
-  const uint32_t m_cfg =
-      npix <= kMaxWaveCap
-          ? uint32_t(std::max<size_t>(1, std::min<size_t>(ctx->samples_per_pass, kMaxPassSlots / npix)))
-          : 1u;
-  const uint32_t m = std::min(m_cfg, p->spp);
+  const size_t cpix = pass_pixels(npix);
+  const uint32_t m = pass_samples(ctx, npix, p->spp);
   const size_t cap = cpix * m;
   const uint32_t npass = (p->spp + m - 1) / m;
   const uint32_t lanes = (p->flags & LAM_FLAG_NO_PIPE) ? 1u : std::min(ctx->lanes, npass);
@@ -1517,8 +1522,8 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
   const size_t npix = size_t(w) * h;
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
-  const size_t cap = std::min(npix, kMaxWaveCap);
-  if ((rc = ensure_wave(ctx, cap, int(p->hero)))) return rc;
+  // the training passes trace one sample of every pixel of a chunk
+  if ((rc = ensure_wave(ctx, pass_pixels(npix), int(p->hero)))) return rc;
   const size_t wcap = ctx->lane[0].ws.cap;
   if (g->train_passes) {
     if (ctx->g_rec_slot_cap < wcap) {
@@ -1709,8 +1714,8 @@ int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg) {
   if (cfg->finish_per_cu < 1 || cfg->finish_per_cu > 32)
     return fail(ctx, LAM_E_INVALID, "finish_per_cu must be 1..32");
   if (cfg->shade_per_cu < 1 || cfg->shade_per_cu > 8) return fail(ctx, LAM_E_INVALID, "shade_per_cu must be 1..8");
-  if (cfg->samples_per_pass < 1 || cfg->samples_per_pass > 8)
-    return fail(ctx, LAM_E_INVALID, "samples_per_pass must be 1..8");
+  if (cfg->samples_per_pass < 1 || cfg->samples_per_pass > 32)
+    return fail(ctx, LAM_E_INVALID, "samples_per_pass must be 1..32");
   ctx->lanes = cfg->lanes;
   ctx->wl.tail_threshold = cfg->tail_threshold;
   ctx->tail_div = cfg->tail_div;

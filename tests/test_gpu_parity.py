@@ -348,7 +348,7 @@ def test_wave_config_keeps_the_image(ctx, box_scene):
     hits = ctx.extend(rays)
     default = ctx.wave_config()
     assert default == {"lanes": 3, "tail_threshold": 131072, "tail_div": 16, "max_iters": 12, "trace_per_cu": 16,
-                       "batch_per_cu": 24, "finish_per_cu": 10, "shade_per_cu": 3, "samples_per_pass": 8}
+                       "batch_per_cu": 24, "finish_per_cu": 10, "shade_per_cu": 3, "samples_per_pass": 32}
     try:
         for cfg in [dict(lanes=1), dict(lanes=2, tail_div=2, max_iters=3, trace_per_cu=1, batch_per_cu=1),
                     dict(lanes=4, tail_threshold=0, trace_per_cu=32, batch_per_cu=32, finish_per_cu=1,
@@ -362,7 +362,7 @@ def test_wave_config_keeps_the_image(ctx, box_scene):
             assert np.array_equal(_bits(ctx.extend(rays)), _bits(hits)), cfg
         for bad in [dict(lanes=0), dict(lanes=5), dict(tail_div=0), dict(max_iters=0), dict(trace_per_cu=0),
                     dict(trace_per_cu=33), dict(batch_per_cu=0), dict(batch_per_cu=33), dict(finish_per_cu=0),
-                    dict(shade_per_cu=9), dict(samples_per_pass=0), dict(samples_per_pass=9)]:
+                    dict(shade_per_cu=9), dict(samples_per_pass=0), dict(samples_per_pass=33)]:
             with pytest.raises(la.LambdaError) as e:
                 ctx.set_wave_config(**{**default, **bad})
             assert e.value.code == la.LAM_E_INVALID
