@@ -294,8 +294,9 @@ int lam_adjoint(lam_ctx *ctx, const lam_render_params *p, const lam_diff_params 
  * so the image converges to lam_render's. */
 typedef struct {
   float alpha;             /* guide-branch probability, [0, 1); 0 = lam_render */
-  uint32_t train_passes;   /* the first passes train the network online */
-  uint32_t steps_per_pass; /* Adam steps after each training pass */
+  uint32_t train_passes;   /* the first train_passes samples per pixel train the network online, traced in
+                              passes of m = min(8, train_passes / 4) samples (at least 1) */
+  uint32_t steps_per_pass; /* Adam steps per training sample: m x steps_per_pass after each training pass */
   uint32_t batch;          /* training records per step, multiple of 256, <= 65536 */
   float lr;                /* Adam step size */
   uint64_t seed;           /* network initialisation (if none yet) and minibatches */

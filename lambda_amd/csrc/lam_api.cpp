@@ -469,6 +469,13 @@ uint32_t pass_samples(const lam_ctx *ctx, size_t npix, uint32_t spp) {
   return uint32_t(std::max<size_t>(1, m));
 }
 
+// samples per training pass of a guided render: at least four training
+// passes (the network is updated after each), at most 8 samples each
+uint32_t train_pass_samples(const lam_ctx *ctx, size_t npix, uint32_t spp, uint32_t train_passes) {
+  const uint32_t tp = std::min(train_passes, spp);
+  return std::max(1u, std::min({8u, tp / 4u, pass_samples(ctx, npix, spp)}));
+}
+
 int ensure_mix(lam_ctx *ctx, int l) {
   lam_ctx::Lane &ln = ctx->lane[l];
   if (ln.mix_cap >= ln.ws.cap) return LAM_OK;
@@ -697,13 +704,17 @@ int render_wavefront(lam_ctx *ctx, const lam_render_params *p, float *film, size
       gtr.mix = gdl[0].mix;
       ctx->timing.guide_records = 0;
       const auto t0 = std::chrono::steady_clock::now();
-      for (; first < tp; first++) {
+      // passes of mt samples of every pixel, each followed by mt x
+      // steps_per_pass Adam steps on its records
+      const uint32_t mt = train_pass_samples(ctx, npix, p->spp, gp->train_passes);
+      for (; first < tp; first += std::min(mt, tp - first)) {
+        const uint32_t mg = std::min(mt, tp - first);
         HIPC(lam::launch_zero(gtr.rec_count, 16, st[0]));
         HIPC(lam::enqueue_pass_set(ctrl[0], p->spp_offset + first, st[0]));
-        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[0], n, ctrl[0],
-                               film, nullptr, wl, st[0], nullptr, &gtr, true));
+        HIPC(lam::enqueue_pass(int(p->hero), ctx->sc, pa, nullptr, nullptr, uint32_t(off), ws[0], n * mg, ctrl[0],
+                               film, nullptr, wl, st[0], nullptr, &gtr, true, mg));
         HIPC(lam::guide_weights(int(p->hero), ws[0], gtr, st[0]));
-        HIPC(lam::guide_train_records(gt, gtr.rec, gtr.rec_count, gtr.rec_cap, gp->steps_per_pass, st[0]));
+        HIPC(lam::guide_train_records(gt, gtr.rec, gtr.rec_count, gtr.rec_cap, gp->steps_per_pass * mg, st[0]));
         uint32_t cnt = 0;
         float sc[2] = {0.f, 0.f};
         HIPC(hipMemcpyAsync(&cnt, gtr.rec_count, 4, hipMemcpyDeviceToHost, st[0]));
@@ -1522,8 +1533,9 @@ int lam_render_guided(lam_ctx *ctx, const lam_render_params *p, const lam_guide_
   const uint32_t w = ctx->sc.cam.width, h = ctx->sc.cam.height;
   const size_t npix = size_t(w) * h;
   if (npix > (size_t(1) << 30)) return fail(ctx, LAM_E_LIMIT, "image too large");
-  // the training passes trace one sample of every pixel of a chunk
-  if ((rc = ensure_wave(ctx, pass_pixels(npix), int(p->hero)))) return rc;
+  // the training passes trace train_pass_samples samples of every pixel
+  const size_t train_slots = pass_pixels(npix) * train_pass_samples(ctx, npix, p->spp, g->train_passes);
+  if ((rc = ensure_wave(ctx, train_slots, int(p->hero)))) return rc;
   const size_t wcap = ctx->lane[0].ws.cap;
   if (g->train_passes) {
     if (ctx->g_rec_slot_cap < wcap) {

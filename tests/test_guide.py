@@ -395,7 +395,7 @@ def test_guided_and_plain_match_the_fp64_reference(ctx, tmp_path):
     per-pixel |z| of median below 1.2, relMSE against the reference printed.
     (A check of bias only: at 64x64 the network learns from 256x fewer
     records than at the workload's 1024x1024, and the guided estimate is the
-    noisier one here: measured relMSE 0.43 against plain 0.16.)"""
+    noisier one here: measured relMSE 0.26 against plain 0.16.)"""
     import os
     from lambda_amd import scenes
     ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "glossy_interior_64_fp64.npz"))
@@ -406,7 +406,7 @@ def test_guided_and_plain_match_the_fp64_reference(ctx, tmp_path):
                                        spp_offset=off)
     plain = [ctx.render(mk(3, k * spp // 2)).astype(np.float64) for k in range(2)]
     ctx.guide_reset(5)
-    guided = [ctx.render_guided(mk(5 + k), la.guide_params(train_passes=32 if k == 0 else 0, steps_per_pass=32,
+    guided = [ctx.render_guided(mk(5 + k), la.guide_params(train_passes=32 if k == 0 else 0, steps_per_pass=8,
                                                            batch=4096, seed=5)).astype(np.float64)
               for k in range(2)]
     r, vr = ref["mean"].astype(np.float64), ref["var_of_mean"].astype(np.float64)
