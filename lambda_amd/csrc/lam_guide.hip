@@ -605,10 +605,12 @@ __global__ __launch_bounds__(256) void k_guide_fwdbwd(const uint4 *__restrict__ 
   if (tid == 0) lpart[blockIdx.x] = ((dbs[0] + dbs[1]) + dbs[2]) + dbs[3];
 }
 
-// The step's gradient (the partials summed in block order) into grad
-// (optional) and, with adam, the Adam update of parameter tid written to th
-// and into the pack; thread 0 stores the loss in scal[1].  Threads also draw
-// the next step's batch (ga.B > 0).
+// The step's gradient (the partials summed in a fixed order) into grad
+// (optional) and, with adam, the Adam update of each parameter written to
+// th and into the pack; thread 0 stores the loss in scal[1].  A block takes
+// 64 parameters, its four waves a quarter of the partial rows each (16 loads
+// in flight per thread), then one sum over the quarters in LDS.  Threads
+// also draw the next step's batch (ga.B > 0).
 __global__ __launch_bounds__(256) void k_guide_update(const float *__restrict__ part, uint32_t nblk,
                                                       const float *__restrict__ lpart, float *__restrict__ grad,
                                                       float *__restrict__ th, float *__restrict__ m,
@@ -616,33 +618,40 @@ __global__ __launch_bounds__(256) void k_guide_update(const float *__restrict__ 
                                                       float lr, float b1, float b2, float eps, float c1, float c2,
                                                       float *__restrict__ scal, Gather ga) {
   __shared__ float red[256];
-  const uint32_t tid = blockIdx.x * 256 + threadIdx.x;
-  if (tid < uint32_t(kGParams)) {
-    float gs = 0.f;  // in block order; the loads 16 at a time
-    for (uint32_t b0 = 0; b0 < nblk; b0 += 16) {
-      float v[16];
+  const uint32_t pl = threadIdx.x % 64, quarter = threadIdx.x / 64;
+  const uint32_t j = blockIdx.x * 64 + pl;
+  const uint32_t r_begin = quarter * ((nblk + 3) / 4), r_end = min(nblk, r_begin + (nblk + 3) / 4);
+  float gs = 0.f;
+  if (j < uint32_t(kGParams))
+    for (uint32_t b0 = r_begin; b0 < r_end; b0 += 16) {
+      float x[16];
 #pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = b0 + u < nblk ? part[size_t(b0 + u) * kGParamsPad + tid] : 0.f;
+      for (int u = 0; u < 16; u++) x[u] = b0 + u < r_end ? part[size_t(b0 + u) * kGParamsPad + j] : 0.f;
 #pragma unroll
-      for (int u = 0; u < 16; u++) gs += v[u];
+      for (int u = 0; u < 16; u++) gs += x[u];
     }
-    if (grad) grad[tid] = gs;
+  red[threadIdx.x] = gs;
+  __syncthreads();
+  if (quarter == 0 && j < uint32_t(kGParams)) {
+    gs = ((red[pl] + red[64 + pl]) + red[128 + pl]) + red[192 + pl];
+    if (grad) grad[j] = gs;
     if (adam) {
-      const float mi = b1 * m[tid] + (1.f - b1) * gs;
-      const float vi = b2 * v[tid] + (1.f - b2) * gs * gs;
-      m[tid] = mi;
-      v[tid] = vi;
-      const float t = th[tid] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-      th[tid] = t;
-      pack_param(pack, int(tid), t);
+      const float mi = b1 * m[j] + (1.f - b1) * gs;
+      const float vi = b2 * v[j] + (1.f - b2) * gs * gs;
+      m[j] = mi;
+      v[j] = vi;
+      const float t = th[j] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+      th[j] = t;
+      pack_param(pack, int(j), t);
     }
   }
-  if (tid == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     float s = 0.f;
     for (uint32_t b = 0; b < nblk; b++) s += lpart[b];
     scal[1] = s;
   }
-  if (ga.B > 0 && blockIdx.x * 256 < ga.B) gather_row(ga, tid, red);
+  __syncthreads();  // red is reused by the batch draw
+  if (ga.B > 0 && blockIdx.x * 256 < ga.B) gather_row(ga, blockIdx.x * 256 + threadIdx.x, red);
 }
 
 // ---- host-side drivers ------------------------------------------------------------------
@@ -682,8 +691,8 @@ static hipError_t update(GuideTrain &gt, uint32_t nblk, const TrainWs &t, float 
     c1 = 1.f - powf(gt.b1, float(gt.step));
     c2 = 1.f - powf(gt.b2, float(gt.step));
   }
-  const uint32_t threads = std::max<uint32_t>(kGParams, ga.B);
-  hipLaunchKernelGGL(k_guide_update, dim3(cdiv(threads, 256)), dim3(256), 0, st, t.part, nblk, t.lpart, grad,
+  const uint32_t blocks = std::max<uint32_t>(cdiv(kGParams, 64), cdiv(ga.B, 256));
+  hipLaunchKernelGGL(k_guide_update, dim3(blocks), dim3(256), 0, st, t.part, nblk, t.lpart, grad,
                      gt.th, gt.m, gt.v, static_cast<uint4 *>(gt.pack), adam ? 1 : 0, gt.lr, gt.b1, gt.b2, gt.eps,
                      c1, c2, gt.scal, ga);
   return hipGetLastError();
