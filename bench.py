@@ -26,9 +26,10 @@ Also reported:
                 points to uniform points on the quad lights: 32 + 4 +
                 128 per node + 48 per triangle.
   render_kernels_roofline  the render's own extend and shadow kernels in a
-                profiled render (one lane, HIP events per launch): their
-                node and triangle counts (LAM_FLAG_PROFILE_FETCH) over their
-                time in a second render of the same samples.
+                profiled render of two passes of the timed render's shape
+                (one lane, HIP events per launch): their node and triangle
+                counts (LAM_FLAG_PROFILE_FETCH) over their time in a second
+                render of the same samples.
   cpu_baseline  the CPU oracle (C restatement of the reference render loop,
                 OpenMP over pixels) on a bounded row sample of the same
                 frame and settings, on this host's cores.
@@ -223,12 +224,14 @@ def shadow_roofline(ctx, scene, reps=10):
     return _batch_roofline(ctx, scene, sh, True, reps, "shadow_icosphere_field_batch")
 
 
-def render_kernel_roofline(ctx, bounces, p, hero, spp=4):
+def render_kernel_roofline(ctx, bounces, p, hero, spp):
     """the render's own extend and shadow kernels: one render with
     LAM_FLAG_PROFILE | LAM_FLAG_PROFILE_FETCH (one lane, every traversal
     launch between HIP events, the kernels counting nodes and triangles)
     gives the algorithmic bytes; a second render of the same samples with
-    LAM_FLAG_PROFILE alone gives the kernels' time without the counting"""
+    LAM_FLAG_PROFILE alone gives the kernels' time without the counting.
+    spp = two passes of the timed render's shape: the queue lengths, and
+    with them the kernels' occupancy, follow the samples per pass"""
     base = dict(spp=spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=11)
     ctx.reset_timing()
     ctx.render(la.params(flags=la.LAM_FLAG_PROFILE | la.LAM_FLAG_PROFILE_FETCH, **base))
@@ -246,7 +249,9 @@ def render_kernel_roofline(ctx, bounces, p, hero, spp=4):
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms": round(ms, 3), "rays": int(rays),
                      "nodes_per_ray": round(nodes / max(rays, 1), 3), "tris_per_ray": round(tris / max(rays, 1), 3),
                      "rays_per_s": round(rays / (ms * 1e-3), 1) if ms > 0 else 0.0}
-    out["workload"] = f"{spp} profiled passes of the render (one lane, no graph)"
+    m = ctx.wave_config()["samples_per_pass"]
+    out["workload"] = (f"{spp}-spp profiled render (one lane, no graph): passes of "
+                       f"{max(1, min(m, spp // 2))} samples of every pixel")
     return out
 
 
@@ -658,7 +663,7 @@ def main():
         log(f"extend: {out['roofline']}")
         out["shadow_roofline"] = shadow_roofline(ctx, scene)
         log(f"shadow: {out['shadow_roofline']}")
-        out["render_kernels_roofline"] = render_kernel_roofline(ctx, bounces, p, hero)
+        out["render_kernels_roofline"] = render_kernel_roofline(ctx, bounces, p, hero, 2 * m_pass)
         log(f"render kernels: {out['render_kernels_roofline']}")
     ctx.close()
     if "adjoint" in legs:
