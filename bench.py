@@ -4,9 +4,11 @@ Workload (the plain method at scale): a seeded procedural scene of ~1.0 M
 triangles (64 instances of a 15,680-triangle noise-displaced geodesic
 sphere over a ground plane, Lambertian / Blinn / thin-film materials, two
 area lights), 1920x1080, 32 bounces, bounce probability 0.8, 4 hero
-wavelengths on 360-830 nm, FP32.  One step = one sample per pixel over the
-whole frame (2,073,600 camera paths).  value = samples/s over the timed
-steps, scene resident in HBM.
+wavelengths on 360-830 nm, FP32, 64 samples per pixel.  One step = one
+render of that frame (Renderer::render: 64 x 2,073,600 camera paths, the
+linear RGB image read back to the host); K steps are K such frames with
+different seeds.  value = samples/s over the timed frames, scene resident
+in HBM.
 
 Also reported:
   render_rays   rays the timed render traced (extend + shadow, counted on the
@@ -535,9 +537,10 @@ def gpu_bvh_bench(bounces=32, p=0.8, spp=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64,
-                    help="passes (samples per pixel) in the timed render; 64 is the configuration's spp")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed frames (each one render of --spp samples per pixel)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed frames before the timed ones")
+    ap.add_argument("--spp", type=int, default=64, help="samples per pixel of one frame; 64 is the configuration's")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -626,37 +629,40 @@ def main():
         print(json.dumps({"extend": rl, "shadow": sl, "copy_bytes": nbytes}), flush=True)
         ctx.close()
         return
-    # the warm-up renders at least two full wavefront passes of the timed
-    # render's shape (min(samples_per_pass, K/2) samples of every pixel
-    # each), so that the lanes' path state exists before the timed render,
-    # whatever W is
-    m_pass = max(1, min(ctx.wave_config()["samples_per_pass"], args.steps // 2))
-    warm_spp = max(args.warmup, 2 * m_pass) if args.warmup else 0
-    if warm_spp:
-        ctx.render(la.params(spp=warm_spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=1,
-                             spp_offset=1 << 20, flags=args.flags))
-    pr = la.params(spp=args.steps, max_scatterings=bounces, sample_p=p, hero=hero, seed=1, flags=args.flags)
+    # warm-up: whole frames of the timed shape, so that the lanes' path
+    # state and graphs exist before the timed frames
+    spp = args.spp
+    warm = args.warmup
+    for fw in range(warm):
+        ctx.render(la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=1000 + fw,
+                             flags=args.flags))
     ctx.reset_timing()
+    stream_ms = 0.0
     t0 = time.perf_counter()
-    img = ctx.render(pr)
+    for f in range(args.steps):
+        img = ctx.render(la.params(spp=spp, max_scatterings=bounces, sample_p=p, hero=hero, seed=1 + f,
+                                   flags=args.flags))
+        stream_ms += ctx.timing().render_ms
     wall = time.perf_counter() - t0
     tm = ctx.timing()
-    samples = w * h * args.steps
-    log(f"render: {args.steps} spp in {wall:.3f}s wall, {tm.render_ms:.1f} ms on stream; "
-        f"mean {img.mean():.4g} finite {np.isfinite(img).all()}")
+    samples = w * h * spp * args.steps
+    log(f"render: {args.steps} frames of {spp} spp in {wall:.3f}s wall, {stream_ms:.1f} ms on stream; "
+        f"last frame mean {img.mean():.4g} finite {np.isfinite(img).all()}")
+    m_pass = max(1, min(ctx.wave_config()["samples_per_pass"], spp // 2))
     out = {
         "metric": "samples/s (plain method, 1 GPU)", "value": round(samples / wall, 1), "unit": "samples/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "n_gpus": 1, "steps": args.steps, "warmup": warm,
         "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": "icosphere field: 64 x 15,680-tri instances (1,003,520 tris), ground plane, "
-                               "2 area lights, 1920x1080, 32 bounces, p=0.8, hero 4 on 360-830 nm",
-                   "width": w, "height": h, "spp_per_step": 1, "warmup_spp": warm_spp, "bounces": bounces,
-                   "bounce_prob": p,
-                   "triangles": scene.n_tris},
+                               "2 area lights, 1920x1080, 64 spp, 32 bounces, p=0.8, hero 4 on 360-830 nm; "
+                               "one step = one frame",
+                   "width": w, "height": h, "spp_per_step": spp, "samples_per_pass": m_pass,
+                   "bounces": bounces, "bounce_prob": p, "triangles": scene.n_tris},
         "render_rays": {"rays_per_s": round((tm.extend_rays + tm.shadow_rays) / wall, 1),
                         "extend_rays_per_sample": round(tm.extend_rays / samples, 4),
                         "shadow_rays_per_sample": round(tm.shadow_rays / samples, 4)},
+        "stream_ms_per_step": round(stream_ms / args.steps, 3),
     }
     if "roofline" in legs:
         out["roofline"] = extend_roofline(ctx, scene)

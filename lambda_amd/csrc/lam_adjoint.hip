@@ -405,8 +405,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_adj_shade(DevScene sc, PathArgs
             push_shadow = true;
             shd = ls.sray;
           } else {
-#pragma unroll
-            for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] -= pend[k];
+            sub_from<H>(ws.L, cap, p, pend);
             if (diff) {  // no shadow ray decides this NEE term: it is due now
               float ones[H];
 #pragma unroll
@@ -472,8 +471,10 @@ __global__ __launch_bounds__(64) void k_adj_shadow(DevScene sc, WaveState ws, Re
     if (ovf) atomicOr(ctrl + 15, 1u);
     if (h.obj == LAM_NONE && !ovf) {
       const uint32_t p = ws.spath[i];
+      float pe[H];
 #pragma unroll
-      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] -= ws.pend[size_t(k) * cap + p];
+      for (int k = 0; k < H; k++) pe[k] = ws.pend[size_t(k) * cap + p];
+      sub_from<H>(ws.L, cap, p, pe);
       const uint32_t m = rd.pn_mat[p];
       if (m != LAM_NONE) rd.pn_mat[p] = m & ~kPendMark;
     }

@@ -126,13 +126,15 @@ __device__ __forceinline__ Surf surface(const DevScene &sc, const Hit &h, f3 o, 
   const lam_object &ob = sc.objects[h.obj];
   s.mat = ob.material;
   if (h.slot != LAM_NONE) {
+    // both records requested before either is used: one round trip
     const float4 a = sc.tris[3 * h.slot + 0];
     const float4 b = sc.tris[3 * h.slot + 1];
     const float4 c = sc.tris[3 * h.slot + 2];
-    const f3 bb = tri_bary(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), o, d);
     const float4 n0 = sc.shade[3 * h.slot + 0];
     const float4 n1 = sc.shade[3 * h.slot + 1];
     const float4 n2 = sc.shade[3 * h.slot + 2];
+    __builtin_amdgcn_sched_barrier(0);
+    const f3 bb = tri_bary(mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), mk(b.z, b.w, c.x), o, d);
     const f3 N0 = mk(n0.x, n0.y, n0.z), N1 = mk(n0.w, n1.x, n1.y), N2 = mk(n1.z, n1.w, n2.x);
     s.gn = mk(n2.y, n2.z, n2.w);
     s.p = add(o, mul(d, h.t));

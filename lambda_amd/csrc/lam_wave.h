@@ -27,6 +27,79 @@ enum : uint32_t {
 constexpr uint32_t kShadeBlock = 256;
 // largest spectra table the shade kernel stages in LDS (3 blocks per CU)
 constexpr uint32_t kSpecLdsBytes = 48 * 1024;
+// largest spectra + materials + lights + objects the shade kernel stages
+// in LDS together (3 blocks per CU within the CU's 160 KB)
+constexpr uint32_t kTabLdsBytes = 52 * 1024;
+
+// LDS footprint of the staged tables in float4 units: spectra; materials
+// (56 B each, rounded up as a whole); lights (48 B); objects (112 B)
+struct TabLds {
+  uint32_t spec4, mat4, light4, obj4;
+  __host__ __device__ uint32_t total4() const { return spec4 + mat4 + light4 + obj4; }
+};
+__host__ __device__ inline TabLds tab_lds(const DevScene &sc) {
+  static_assert(sizeof(lam_material) == 56 && sizeof(lam_light) == 48 && sizeof(lam_object) == 112,
+                "table layouts");
+  TabLds t;
+  t.spec4 = (sc.n_spectra * sc.grid_n + 3u) / 4u;
+  t.mat4 = (sc.n_materials * 56u + 15u) / 16u;
+  t.light4 = sc.n_lights * 3u;
+  t.obj4 = sc.n_objects * 7u;
+  return t;
+}
+
+// Copies the scene's read-only tables into the block's dynamic LDS and
+// points sc at the copies: the coefficient spectra (STAGE >= 1), then the
+// materials, lights and objects (STAGE 2), so that the shading of a hit
+// waits on one dependent global fetch (the triangle records) instead of a
+// chain of object -> material -> light -> light object fetches.  Every
+// thread of the block calls it (one barrier); NT threads per block.
+template <int STAGE, uint32_t NT>
+__device__ __forceinline__ void stage_tables(DevScene &sc) {
+  extern __shared__ float4 lds_tab[];
+  const TabLds t = tab_lds(sc);
+  const float4 *sp = reinterpret_cast<const float4 *>(sc.spectra);
+  for (uint32_t i = threadIdx.x; i < t.spec4; i += NT) lds_tab[i] = sp[i];
+  if constexpr (STAGE >= 2) {
+    float4 *mat = lds_tab + t.spec4;
+    float4 *lt = mat + t.mat4;
+    float4 *ob = lt + t.light4;
+    // materials are 8-byte aligned in HBM: copied as uint2
+    const uint2 *ms = reinterpret_cast<const uint2 *>(sc.materials);
+    uint2 *md = reinterpret_cast<uint2 *>(mat);
+    for (uint32_t i = threadIdx.x; i < sc.n_materials * 7u; i += NT) md[i] = ms[i];
+    const float4 *ls = reinterpret_cast<const float4 *>(sc.lights);
+    for (uint32_t i = threadIdx.x; i < t.light4; i += NT) lt[i] = ls[i];
+    const float4 *os = reinterpret_cast<const float4 *>(sc.objects);
+    for (uint32_t i = threadIdx.x; i < t.obj4; i += NT) ob[i] = os[i];
+    sc.materials = reinterpret_cast<const lam_material *>(mat);
+    sc.lights = reinterpret_cast<const lam_light *>(lt);
+    sc.objects = reinterpret_cast<const lam_object *>(ob);
+  }
+  __syncthreads();
+  sc.spectra = reinterpret_cast<const float *>(lds_tab);
+}
+
+// L[k] += v[k] over the H wavelengths of slot p, all loads first: the
+// path-state arrays may alias as far as the compiler knows, so a
+// wavelength-by-wavelength += would wait for each load in turn
+template <int H>
+__device__ __forceinline__ void add_to(float *__restrict__ L, size_t cap, uint32_t p, const float (&v)[H]) {
+  float cur[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) cur[k] = L[size_t(k) * cap + p];
+#pragma unroll
+  for (int k = 0; k < H; k++) L[size_t(k) * cap + p] = cur[k] + v[k];
+}
+// L[k] -= v[k], the same way
+template <int H>
+__device__ __forceinline__ void sub_from(float *__restrict__ L, size_t cap, uint32_t p, const float (&v)[H]) {
+  float cur[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) cur[k] = L[size_t(k) * cap + p];
+#pragma unroll
+  for (int k = 0; k < H; k++) L[size_t(k) * cap + p] = cur[k] - v[k];
+}
 
 // Identity of a path slot.  From lists (test hooks), or a pass of m samples
 // of npix pixels: slot = q npix + i is pixel pixel_offset + i, sample
@@ -38,8 +111,9 @@ struct PathIds {
   uint32_t npix;
 };
 
-__device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
-                                        uint32_t &sample) {
+// (pass = ctrl[CTRL_PASS], read once per kernel by the callers that loop)
+__device__ __forceinline__ void path_id_of(const PathIds &ids, uint32_t pass, uint32_t slot, uint32_t &pixel,
+                                           uint32_t &sample) {
   if (ids.pixel) {
     pixel = ids.pixel[slot];
     sample = ids.sample[slot];
@@ -52,7 +126,11 @@ __device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl
       q++;
     }
   pixel = slot + ids.pixel_offset;
-  sample = ctrl[CTRL_PASS] + q;
+  sample = pass + q;
+}
+__device__ __forceinline__ void path_id(const PathIds &ids, const uint32_t *ctrl, uint32_t slot, uint32_t &pixel,
+                                        uint32_t &sample) {
+  path_id_of(ids, ids.pixel ? 0u : ctrl[CTRL_PASS], slot, pixel, sample);
 }
 
 template <int H>

@@ -132,28 +132,37 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
 // 168 VGPRs without spills (2 waves at the default allocation), and the
 // latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
 // whole register file (1 wave).
-// SPEC_LDS: the coefficient spectra (n_spectra x grid_n floats, at most
-// kSpecLdsBytes) are staged in LDS at the start of the block and every
-// spectral lookup of the block reads them there.
-template <int H, bool SPEC_LDS>
+// STAGE: 1 = the coefficient spectra (n_spectra x grid_n floats, at most
+// kSpecLdsBytes), 2 = the spectra, materials, lights and objects (at most
+// kTabLdsBytes together) are staged in LDS at the start of the block and
+// every lookup of the block reads them there; 0 = none.
+// The ray and hit of the thread's next queue entry are loaded at the end of
+// the current one, so that their round trip overlaps its stores and the
+// block's barrier (loads and stores share one in-order wait counter, so a
+// load issued earlier would be waited for by the entry's own loads).
+template <int H, int STAGE>
 __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t s,
                                                uint32_t qsel, uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
-  if constexpr (SPEC_LDS) {
-    extern __shared__ float4 spec_lds[];
-    const uint32_t n4 = (sc.n_spectra * sc.grid_n + 3u) / 4u;
-    const float4 *src = reinterpret_cast<const float4 *>(sc.spectra);
-    for (uint32_t i = threadIdx.x; i < n4; i += kShadeBlock) spec_lds[i] = src[i];
-    __syncthreads();
-    sc.spectra = reinterpret_cast<const float *>(spec_lds);
-  }
+  if constexpr (STAGE > 0) stage_tables<STAGE, kShadeBlock>(sc);
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const uint32_t pass = ctrl[CTRL_PASS];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
   const size_t cap = ws.cap;
   const uint32_t stride = gridDim.x * kShadeBlock;
   __shared__ BlockQueues<kShadeBlock / 64, 2> sq;
   uint32_t par = 0;
+  float4 next_a = make_float4(0.f, 0.f, 0.f, 0.f), next_b = next_a;
+  Hit next_hit{0.f, LAM_NONE, LAM_NONE, LAM_NONE};
+  auto fetch = [&](uint32_t q) {
+    if (q < n) {
+      next_a = rin[2 * size_t(q)];
+      next_b = rin[2 * size_t(q) + 1];
+      next_hit = ws.hits[q];
+    }
+  };
+  fetch(blockIdx.x * kShadeBlock + threadIdx.x);
   // uniform trip count across the block: every thread takes part in the
   // block appends
   for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
@@ -161,14 +170,14 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
     bool push_next = false, push_shadow = false;
     uint32_t p = 0;
     Ray nxt, shd;
+    const float4 a = next_a, b = next_b;
+    const Hit hit = next_hit;
     if (q < n) {
-      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
       const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
       p = __float_as_uint(b.w);
-      const Hit hit = ws.hits[q];
       if (hit.obj != LAM_NONE) {
         uint32_t pixel, sample;
-        path_id(ids, ctrl, p, pixel, sample);
+        path_id_of(ids, pass, p, pixel, sample);
         const uint64_t key = lam_path_key(pa.seed, pixel, sample);
         Lam<H> lam;
         lam_for<H>(sc, key, lam);
@@ -184,9 +193,10 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
           spec_eval<H>(sc, mat.spec[0], lam, le);
           float f = 1.f;
           if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+          float add[H];
 #pragma unroll
-          for (int k = 0; k < H; k++)
-            ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+          for (int k = 0; k < H; k++) add[k] = beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+          add_to<H>(ws.L, cap, p, add);
         } else if (s <= pa.max_scat) {
           const uint32_t nl = sc.n_lights;
           const uint32_t li = lam_draw_u32(key, lam_vdim(s, LAM_VDIM_LIGHT_PICK)) % nl;
@@ -229,14 +239,16 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
             push_shadow = true;
             shd = ls.sray;
           } else {
-#pragma unroll
-            for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += pend[k];
+            add_to<H>(ws.L, cap, p, pend);
           }
         }
       }
     }
     uint32_t slot[2];
     block_append(sq, par, {push_next, push_shadow}, {ctrl + CTRL_COUNT0 + (qsel ^ 1u), ctrl + CTRL_SHADOW}, slot);
+    // the next entry's ray and hit: issued after this entry's last wait on
+    // memory (the block append's atomic), in flight across its stores
+    fetch(q + stride);
     if (push_next) {
       rout[2 * size_t(slot[0])] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
       rout[2 * size_t(slot[0]) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
@@ -451,9 +463,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
           spec_eval<H>(sc, mat.spec[0], lam, le);
           float f = 1.f;
           if (mat.type == LAM_MAT_DIREMITTER) f = lam_powf(fmax_ref(dot(o, sf.F.z), 0.f), mat.specular);
+          float add[H];
 #pragma unroll
-          for (int k = 0; k < H; k++)
-            ws.L[size_t(k) * cap + p] += beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+          for (int k = 0; k < H; k++) add[k] = beta[k] * (mat.type == LAM_MAT_DIREMITTER ? le[k] * f : le[k]);
+          add_to<H>(ws.L, cap, p, add);
         } else if (s <= pa.max_scat) {
           live = true;
           guided = guide_vertex(gd, mat, s);
@@ -611,8 +624,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
         push_shadow = true;
         shd = ls.sray;
       } else {
-#pragma unroll
-        for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += pend[k];
+        add_to<H>(ws.L, cap, p, pend);
       }
     }
     uint32_t slot[TRAIN ? 3 : 2];
@@ -699,8 +711,10 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
     if (ovf) atomicOr(ctrl + 15, 1u);
     if (h.obj == LAM_NONE && !ovf) {
       const uint32_t p = ws.spath[i];
+      float pe[H];
 #pragma unroll
-      for (int k = 0; k < H; k++) ws.L[size_t(k) * cap + p] += ws.pend[size_t(k) * cap + p];
+      for (int k = 0; k < H; k++) pe[k] = ws.pend[size_t(k) * cap + p];
+      add_to<H>(ws.L, cap, p, pe);
       if (REC) {  // the vertex's training record: its NEE term reached L
         const uint32_t r = rec_of_slot[p];
         if (r < rec_cap) rec[4 * size_t(r) + 3].y = 1.f;
@@ -833,6 +847,7 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
   const uint32_t b256 = (n + 255) / 256;
   // (rounded up to whole float4s, as the staging copies them)
   const uint32_t spec_bytes = (sc.n_spectra * sc.grid_n + 3u) / 4u * 16u;
+  const uint32_t tab_bytes = tab_lds(sc).total4() * 16u;
   auto ev = [&](int slot) {
     if (prof) (void)hipEventRecord(prof->ev[slot], st);
   };
@@ -867,11 +882,14 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     if (gv)
       hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gv, s, q,
                          ctrl);
-    if (!gv && spec_bytes <= kSpecLdsBytes)
-      hipLaunchKernelGGL((k_shade<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
+    if (!gv && tab_bytes <= kTabLdsBytes)
+      hipLaunchKernelGGL((k_shade<H, 2>), dim3(wl.shade_blocks), dim3(kShadeBlock), tab_bytes, st, sc, pa, ids,
+                         ws, s, q, ctrl);
+    else if (!gv && spec_bytes <= kSpecLdsBytes)
+      hipLaunchKernelGGL((k_shade<H, 1>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
                          ws, s, q, ctrl);
     else if (!gv)
-      hipLaunchKernelGGL((k_shade<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s,
+      hipLaunchKernelGGL((k_shade<H, 0>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s,
                          q, ctrl);
     else if (train)
       hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gv,
