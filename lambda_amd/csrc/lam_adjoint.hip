@@ -460,16 +460,16 @@ __global__ __launch_bounds__(64) void k_adj_shadow(DevScene sc, WaveState ws, Re
   const size_t cap = ws.cap;
   auto load = [&](uint32_t i) {
     const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
-    Ray r;
-    r.o = mk(a.x, a.y, a.z);
-    r.tmax = a.w;
-    r.d = mk(b.x, b.y, b.z);
-    r.skip = __float_as_uint(b.w);
-    return r;
+    RayItem it;
+    it.r.o = mk(a.x, a.y, a.z);
+    it.r.tmax = a.w;
+    it.r.d = mk(b.x, b.y, b.z);
+    it.r.skip = __float_as_uint(b.w);
+    return it;
   };
-  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &) {
+  auto done = [&](uint32_t i, const RayItem &, const Trav &tv, bool ovf, const TravCounts &) {
     if (ovf) atomicOr(ctrl + 15, 1u);
-    if (h.obj == LAM_NONE && !ovf) {
+    if (tv.best.obj == LAM_NONE && !ovf) {
       const uint32_t p = ws.spath[i];
       float pe[H];
 #pragma unroll

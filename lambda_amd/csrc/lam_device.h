@@ -261,6 +261,7 @@ struct Trav {
   float ix, iy, iz;
   f3 mr;
   Hit best;
+  uint32_t orig;  // input index of best's triangle (LAM_NONE otherwise)
   uint32_t cur;
   int sp;
 };
@@ -275,6 +276,7 @@ __device__ __forceinline__ bool trav_begin(const DevScene &sc, const Ray &r, Tra
   t.best.slot = LAM_NONE;
   t.best.obj = LAM_NONE;
   t.best.prim = LAM_NONE;
+  t.orig = LAM_NONE;
   for (uint32_t a = 0; a < sc.n_analytic; a++) {
     const uint32_t oi = sc.analytic[a];
     if (oi == r.skip) continue;
@@ -321,6 +323,17 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
     const uint32_t qlo[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
     const uint32_t qhi[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
     const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    // the near and far planes of each axis by the sign of the ray's inverse
+    // direction: for ix > 0, (lo - o) ix <= (hi - o) ix (rounding is
+    // monotone), so the slab's min / max are these two values exactly, and
+    // the test below computes the same tn and tf as with min / max of both
+    bool pos[3] = {t.ix > 0.f, t.iy > 0.f, t.iz > 0.f};
+    uint32_t qn[6], qf[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      qn[j] = pos[j >> 1] ? qlo[j] : qhi[j];
+      qf[j] = pos[j >> 1] ? qhi[j] : qlo[j];
+    }
     const float kInf = __builtin_inff();
     const f3 o = t.r.o;
     float tk[8];
@@ -330,17 +343,17 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
       // origin + q * 2^e: the product is exact (8-bit q, power-of-two
       // scale), so the fused form rounds once, exactly as the sum does
       const int w = k >> 2, sh = (k & 3) * 8;
-      const float lx = __builtin_fmaf(float((qlo[0 + w] >> sh) & 0xffu), sx, h0.x);
-      const float ly = __builtin_fmaf(float((qlo[2 + w] >> sh) & 0xffu), sy, h0.y);
-      const float lz = __builtin_fmaf(float((qlo[4 + w] >> sh) & 0xffu), sz, h0.z);
-      const float hx = __builtin_fmaf(float((qhi[0 + w] >> sh) & 0xffu), sx, h0.x);
-      const float hy = __builtin_fmaf(float((qhi[2 + w] >> sh) & 0xffu), sy, h0.y);
-      const float hz = __builtin_fmaf(float((qhi[4 + w] >> sh) & 0xffu), sz, h0.z);
-      const float tx0 = (lx - o.x) * t.ix, tx1 = (hx - o.x) * t.ix;
-      const float ty0 = (ly - o.y) * t.iy, ty1 = (hy - o.y) * t.iy;
-      const float tz0 = (lz - o.z) * t.iz, tz1 = (hz - o.z) * t.iz;
-      const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
-      const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), t.best.t));
+      const float nx = __builtin_fmaf(float((qn[0 + w] >> sh) & 0xffu), sx, h0.x);
+      const float ny = __builtin_fmaf(float((qn[2 + w] >> sh) & 0xffu), sy, h0.y);
+      const float nz = __builtin_fmaf(float((qn[4 + w] >> sh) & 0xffu), sz, h0.z);
+      const float fx = __builtin_fmaf(float((qf[0 + w] >> sh) & 0xffu), sx, h0.x);
+      const float fy = __builtin_fmaf(float((qf[2 + w] >> sh) & 0xffu), sy, h0.y);
+      const float fz = __builtin_fmaf(float((qf[4 + w] >> sh) & 0xffu), sz, h0.z);
+      const float tnx = (nx - o.x) * t.ix, tfx = (fx - o.x) * t.ix;
+      const float tny = (ny - o.y) * t.iy, tfy = (fy - o.y) * t.iy;
+      const float tnz = (nz - o.z) * t.iz, tfz = (fz - o.z) * t.iz;
+      const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
+      const float tf = fminf(fminf(tfx, tfy), fminf(tfz, t.best.t));
       const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobust);
       tk[k] = hit ? fminf(tn, kFltMax) : kInf;
       ck[k] = ch[k];
@@ -427,16 +440,17 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
       if (obj == t.r.skip) continue;
       const f3 v0 = mk(a.x, a.y, a.z), v1 = mk(a.w, b.x, b.y), v2 = mk(b.z, b.w, c.x);
       const float th = tri_t(v0, v1, v2, t.r.o, t.r.d, t.mr);
-      if (th >= kBias) {
-        const uint32_t prim = __float_as_uint(c.z);
-        if (better(th, obj, prim, t.best)) {
-          t.best.t = th;
-          t.best.slot = s;
-          t.best.obj = obj;
-          t.best.prim = prim;
-          if (ANY) return false;
-        }
-      }
+      const uint32_t prim = __float_as_uint(c.z);
+      // selects rather than a branch: hipcc for gfx950 lowered the branch
+      // form with the new input index lost on ties won by a lower id (the
+      // register it sat in was reset for every lane of the t >= best.t arm)
+      const bool bt = th >= kBias && better(th, obj, prim, t.best);
+      t.best.t = bt ? th : t.best.t;
+      t.best.slot = bt ? s : t.best.slot;
+      t.best.obj = bt ? obj : t.best.obj;
+      t.best.prim = bt ? prim : t.best.prim;
+      t.orig = bt ? __float_as_uint(c.w) : t.orig;
+      if (ANY && bt) return false;
     }
   }
   if (t.sp == 0) return false;
@@ -467,15 +481,26 @@ __device__ __forceinline__ Hit trace(const DevScene &sc, const Ray &r, const Sta
 // size gives each wave about kPoolRounds reservations of the queue's
 // remainder (32..kPool rays): large enough that the atomic's latency is
 // rare, small enough that the last reservations do not leave a few waves
-// with most of the work.  load(i) returns ray i; done(i, hit, overflow,
-// counts) consumes its result (counts: nodes and triangles of this ray
-// when COUNT).
+// with most of the work.  load(i) returns queue entry i (an item with the
+// ray as member r, and whatever done needs: loaded with the ray, in one
+// round trip); done(i, item, trav, overflow, counts) consumes its result
+// (trav.best, trav.orig; counts: nodes and triangles of this ray when
+// COUNT).
+// (Measured: holding a second, prefetched entry per lane and starting it
+// without waiting was 13-20% slower on the extend and shadow batches: the
+// prefetch's HBM round trip then sits in front of every following node
+// fetch on the wave's in-order memory counter.)
 constexpr uint32_t kRefill = 16;
 constexpr uint32_t kPool = 256;
 constexpr uint32_t kPoolRounds = 2;
 
-// Wave-uniform pool of queue indices [pool, pool_end) for per-lane refill
-// (see trace_queue).  dry: the queue has no unreserved entries left.
+// queue entry of a plain ray batch
+struct RayItem {
+  Ray r;
+};
+
+// Wave-uniform pool of queue indices [pool, pool_end) (see trace_queue).
+// dry: the queue has no unreserved entries left.
 struct QueuePool {
   uint32_t pool, pool_end, psize, n;
   bool dry;
@@ -487,11 +512,14 @@ struct QueuePool {
     const uint32_t first_round = 64u * gridDim.x;
     psize = n > first_round ? min(kPool, max(32u, (n - first_round) / (gridDim.x * kPoolRounds) + 1u)) : 32u;
   }
-  __device__ __forceinline__ bool done(uint32_t nidle) const { return nidle == 64 && pool_end == pool && dry; }
-  __device__ __forceinline__ bool can_refill(uint32_t nidle) const {
-    return nidle >= kRefill && (pool_end > pool || !dry);
+  // indices left to hand out (in the pool or still reservable)
+  __device__ __forceinline__ bool more() const { return pool_end > pool || !dry; }
+  // k_finish's refill rule: at least `least` idle lanes and indices left
+  __device__ __forceinline__ bool done(uint32_t nidle) const { return nidle == 64 && !more(); }
+  __device__ __forceinline__ bool can_refill(uint32_t nidle, uint32_t least = 16) const {
+    return nidle >= least && more();
   }
-  // Hands idle lanes (ballot `idle`, nidle of them) the next indices: the
+  // Hands the lanes of ballot `idle` (nidle of them) the next indices: the
   // rest of the pool, then a new reservation from *fetch.  Returns this
   // lane's index (n for none; only meaningful where `is_idle`).
   __device__ __forceinline__ uint32_t take(uint64_t idle, uint32_t nidle, bool is_idle, uint32_t *fetch) {
@@ -525,29 +553,32 @@ struct QueuePool {
 template <bool ANY, bool COUNT, class Stack, class Load, class Done>
 __device__ __forceinline__ void trace_queue(const DevScene &sc, uint32_t n, uint32_t *fetch, const Stack &stk,
                                             Load load, Done done) {
+  using Item = decltype(load(0u));
   TravCounts cnt{0, 0};
   Trav t;
   bool active = false, ovf = false;
   uint32_t idx = 0;
+  Item cur{};
   QueuePool qp;
   qp.init(n);
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (qp.done(nidle)) break;
-    if (qp.can_refill(nidle)) {
+    if (qp.can_refill(nidle, kRefill)) {
       const uint32_t i = qp.take(idle, nidle, !active, fetch);
       if (!active && i < n) {
         idx = i;
         ovf = false;
         cnt = TravCounts{0, 0};
-        active = trav_begin<ANY>(sc, load(idx), t);
-        if (!active) done(idx, t.best, false, cnt);
+        cur = load(idx);
+        active = trav_begin<ANY>(sc, cur.r, t);
+        if (!active) done(idx, cur, t, false, cnt);
       }
     }
     if (active) {
       active = trav_step<ANY, COUNT>(sc, t, stk, cnt, ovf);
-      if (!active) done(idx, t.best, ovf, cnt);
+      if (!active) done(idx, cur, t, ovf, cnt);
     }
   }
 }

@@ -18,20 +18,21 @@ __global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *_
   auto load = [&](uint32_t i) {
     const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
     const float4 a = rp[0], b = rp[1];
-    Ray r;
-    r.o = mk(a.x, a.y, a.z);
-    r.tmax = a.w;
-    r.d = mk(b.x, b.y, b.z);
-    r.skip = __float_as_uint(b.w);
-    return r;
+    RayItem it;
+    it.r.o = mk(a.x, a.y, a.z);
+    it.r.tmax = a.w;
+    it.r.d = mk(b.x, b.y, b.z);
+    it.r.skip = __float_as_uint(b.w);
+    return it;
   };
-  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &c) {
+  auto done = [&](uint32_t i, const RayItem &, const Trav &tv, bool ovf, const TravCounts &c) {
+    const Hit &h = tv.best;
     if (ovf) atomicOr(err, 1u);
     if (ANY) {
       occl[i] = h.obj != LAM_NONE ? 1u : 0u;
     } else {
-      uint32_t tri = LAM_NONE;
-      if (h.slot != LAM_NONE) tri = __float_as_uint(sc.tris[3 * h.slot + 2].w);
+      // input index of the hit triangle (kept by the traversal: no fetch)
+      const uint32_t tri = h.slot != LAM_NONE ? tv.orig : LAM_NONE;
       float4 o;
       o.x = h.obj == LAM_NONE ? kFltMax : h.t;
       o.y = __uint_as_float(tri);

@@ -101,14 +101,15 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
   uint32_t nrays = 0;
   auto load = [&](uint32_t i) {
     const float4 a = rays[2 * size_t(i)], b = rays[2 * size_t(i) + 1];
-    Ray r;
-    r.o = mk(a.x, a.y, a.z);
-    r.tmax = a.w;
-    r.d = mk(b.x, b.y, b.z);
-    r.skip = LAM_NONE;
-    return r;
+    RayItem it;
+    it.r.o = mk(a.x, a.y, a.z);
+    it.r.tmax = a.w;
+    it.r.d = mk(b.x, b.y, b.z);
+    it.r.skip = LAM_NONE;
+    return it;
   };
-  auto done = [&](uint32_t i, Hit h, bool ovf, const TravCounts &rc) {
+  auto done = [&](uint32_t i, const RayItem &, const Trav &tv, bool ovf, const TravCounts &rc) {
+    Hit h = tv.best;
     if (ovf) {  // cannot happen under the host-checked stack bound
       atomicOr(ctrl + 15, 1u);
       h.obj = LAM_NONE;
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
     bool push_next = false, push_shadow = false;
     uint32_t p = 0;
     Ray nxt, shd;
+    float spend[H];  // the shadow ray's NEE term
     const float4 a = next_a, b = next_b;
     const Hit hit = next_hit;
     if (q < n) {
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
           }
           if (ls.shadow) {
 #pragma unroll
-            for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + p] = pend[k];
+            for (int k = 0; k < H; k++) spend[k] = pend[k];
             push_shadow = true;
             shd = ls.sray;
           } else {
@@ -258,6 +260,8 @@ __global__ __launch_bounds__(kShadeBlock, H == 4 ? 3 : 1) void k_shade(DevScene 
       sr[2 * size_t(slot[1])] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
       sr[2 * size_t(slot[1]) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
       ws.spath[slot[1]] = p;
+#pragma unroll
+      for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + slot[1]] = spend[k];
     }
   }
 }
@@ -436,6 +440,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
     bool push_next = false, push_shadow = false, live = false, guided = false, want_rec = false;
     uint32_t p = 0, mat_id = 0;
     Ray nxt, shd;
+    float spend[H];  // the shadow ray's NEE term
     uint64_t key = 0;
     Lam<H> lam;
     float beta[H];
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       }
       if (ls.shadow) {
 #pragma unroll
-        for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + p] = pend[k];
+        for (int k = 0; k < H; k++) spend[k] = pend[k];
         push_shadow = true;
         shd = ls.sray;
       } else {
@@ -653,6 +658,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       sr[2 * size_t(slot[1])] = make_float4(shd.o.x, shd.o.y, shd.o.z, shd.tmax);
       sr[2 * size_t(slot[1]) + 1] = make_float4(shd.d.x, shd.d.y, shd.d.z, __uint_as_float(shd.skip));
       ws.spath[slot[1]] = p;
+#pragma unroll
+      for (int k = 0; k < H; k++) ws.pend[size_t(k) * cap + slot[1]] = spend[k];
     }
   }
 }
@@ -698,25 +705,34 @@ __global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32
   const size_t cap = ws.cap;
   TravCounts c{0, 0};
   uint32_t nrays = 0;
+  // the entry's ray, its path slot and NEE term (queue-indexed, so all are
+  // loaded ahead with the ray)
+  struct Item {
+    Ray r;
+    uint32_t p;
+    float pe[H];
+  };
   auto load = [&](uint32_t i) {
     const float4 a = sr[2 * size_t(i)], b = sr[2 * size_t(i) + 1];
-    Ray r;
-    r.o = mk(a.x, a.y, a.z);
-    r.tmax = a.w;
-    r.d = mk(b.x, b.y, b.z);
-    r.skip = __float_as_uint(b.w);
-    return r;
-  };
-  auto done = [&](uint32_t i, const Hit &h, bool ovf, const TravCounts &rc) {
-    if (ovf) atomicOr(ctrl + 15, 1u);
-    if (h.obj == LAM_NONE && !ovf) {
-      const uint32_t p = ws.spath[i];
-      float pe[H];
+    Item it;
+    it.r.o = mk(a.x, a.y, a.z);
+    it.r.tmax = a.w;
+    it.r.d = mk(b.x, b.y, b.z);
+    it.r.skip = __float_as_uint(b.w);
+    it.p = ws.spath[i];
 #pragma unroll
-      for (int k = 0; k < H; k++) pe[k] = ws.pend[size_t(k) * cap + p];
-      add_to<H>(ws.L, cap, p, pe);
+    for (int k = 0; k < H; k++) it.pe[k] = ws.pend[size_t(k) * cap + i];
+    return it;
+  };
+  auto done = [&](uint32_t i, const Item &it, const Trav &tv, bool ovf, const TravCounts &rc) {
+    if (ovf) atomicOr(ctrl + 15, 1u);
+    if (tv.best.obj == LAM_NONE && !ovf) {
+      // L += NEE term: one shadow ray per path and iteration, so the add
+      // is the sum L + pe exactly; the atomic form needs no wait
+#pragma unroll
+      for (int k = 0; k < H; k++) unsafeAtomicAdd(ws.L + size_t(k) * cap + it.p, it.pe[k]);
       if (REC) {  // the vertex's training record: its NEE term reached L
-        const uint32_t r = rec_of_slot[p];
+        const uint32_t r = rec_of_slot[it.p];
         if (r < rec_cap) rec[4 * size_t(r) + 3].y = 1.f;
       }
     }
