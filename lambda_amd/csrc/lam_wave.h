@@ -49,19 +49,24 @@ __host__ __device__ inline TabLds tab_lds(const DevScene &sc) {
 }
 
 // Copies the scene's read-only tables into the block's dynamic LDS and
-// points sc at the copies: the coefficient spectra (STAGE >= 1), then the
-// materials, lights and objects (STAGE 2), so that the shading of a hit
-// waits on one dependent global fetch (the triangle records) instead of a
-// chain of object -> material -> light -> light object fetches.  Every
-// thread of the block calls it (one barrier); NT threads per block.
+// points sc at the copies: STAGE bit 1 the coefficient spectra, bit 2 the
+// materials, lights and objects (after the spectra when both), so that the
+// shading of a hit waits on one dependent global fetch (the triangle
+// records) instead of a chain of object -> material -> light -> light
+// object fetches.  Every thread of the block calls it (one barrier); NT
+// threads per block.
+constexpr int kStageSpectra = 1, kStageTables = 2;
 template <int STAGE, uint32_t NT>
 __device__ __forceinline__ void stage_tables(DevScene &sc) {
   extern __shared__ float4 lds_tab[];
   const TabLds t = tab_lds(sc);
-  const float4 *sp = reinterpret_cast<const float4 *>(sc.spectra);
-  for (uint32_t i = threadIdx.x; i < t.spec4; i += NT) lds_tab[i] = sp[i];
-  if constexpr (STAGE >= 2) {
-    float4 *mat = lds_tab + t.spec4;
+  float4 *mat = lds_tab;
+  if constexpr (STAGE & kStageSpectra) {
+    const float4 *sp = reinterpret_cast<const float4 *>(sc.spectra);
+    for (uint32_t i = threadIdx.x; i < t.spec4; i += NT) lds_tab[i] = sp[i];
+    mat += t.spec4;
+  }
+  if constexpr (STAGE & kStageTables) {
     float4 *lt = mat + t.mat4;
     float4 *ob = lt + t.light4;
     // materials are 8-byte aligned in HBM: copied as uint2
@@ -72,12 +77,30 @@ __device__ __forceinline__ void stage_tables(DevScene &sc) {
     for (uint32_t i = threadIdx.x; i < t.light4; i += NT) lt[i] = ls[i];
     const float4 *os = reinterpret_cast<const float4 *>(sc.objects);
     for (uint32_t i = threadIdx.x; i < t.obj4; i += NT) ob[i] = os[i];
-    sc.materials = reinterpret_cast<const lam_material *>(mat);
-    sc.lights = reinterpret_cast<const lam_light *>(lt);
-    sc.objects = reinterpret_cast<const lam_object *>(ob);
   }
   __syncthreads();
-  sc.spectra = reinterpret_cast<const float *>(lds_tab);
+  if constexpr (STAGE & kStageSpectra) sc.spectra = reinterpret_cast<const float *>(lds_tab);
+  if constexpr (STAGE & kStageTables) {
+    sc.materials = reinterpret_cast<const lam_material *>(mat);
+    sc.lights = reinterpret_cast<const lam_light *>(mat + t.mat4);
+    sc.objects = reinterpret_cast<const lam_object *>(mat + t.mat4 + t.light4);
+  }
+}
+
+// Staging choice for a shade launch and its dynamic LDS bytes: spectra and
+// tables together when they fit kTabLdsBytes, else the tables alone, else
+// (spectra_alone) the spectra when they fit kSpecLdsBytes, else nothing.
+struct StagePlan {
+  int stage;
+  uint32_t bytes;
+};
+inline StagePlan stage_plan(const DevScene &sc, bool spectra_alone) {
+  const TabLds t = tab_lds(sc);
+  const uint32_t spec = t.spec4 * 16u, tab = (t.mat4 + t.light4 + t.obj4) * 16u;
+  if (spec + tab <= kTabLdsBytes) return {kStageSpectra | kStageTables, spec + tab};
+  if (tab <= kTabLdsBytes) return {kStageTables, tab};
+  if (spectra_alone && spec <= kSpecLdsBytes) return {kStageSpectra, spec};
+  return {0, 0u};
 }
 
 // L[k] += v[k] over the H wavelengths of slot p, all loads first: the

@@ -133,10 +133,9 @@ __global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__rest
 // 168 VGPRs without spills (2 waves at the default allocation), and the
 // latency-bound kernel gains ~10% from the third wave.  H = 16 needs the
 // whole register file (1 wave).
-// STAGE: 1 = the coefficient spectra (n_spectra x grid_n floats, at most
-// kSpecLdsBytes), 2 = the spectra, materials, lights and objects (at most
-// kTabLdsBytes together) are staged in LDS at the start of the block and
-// every lookup of the block reads them there; 0 = none.
+// STAGE (stage_tables): bit 1 the coefficient spectra, bit 2 the materials,
+// lights and objects are staged in LDS at the start of the block and every
+// lookup of the block reads them there (stage_plan picks what fits).
 // The ray and hit of the thread's next queue entry are loaded at the end of
 // the current one, so that their round trip overlaps its stores and the
 // block's barrier (loads and stores share one in-order wait counter, so a
@@ -423,18 +422,32 @@ __device__ __forceinline__ void load_mix(const uint4 *src, float *raw) {
   }
 }
 
-template <int H, bool TRAIN>
+template <int H, bool TRAIN, int STAGE>
 __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
                                                       GuideDev gd, uint32_t s, uint32_t qsel,
                                                       uint32_t *__restrict__ ctrl) {
   if (ctrl[CTRL_TAIL]) return;
+  if constexpr (STAGE > 0) stage_tables<STAGE, kShadeBlock>(sc);
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const uint32_t pass = ctrl[CTRL_PASS];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
   const size_t cap = ws.cap;
   const uint32_t stride = gridDim.x * kShadeBlock;
   __shared__ BlockQueues<kShadeBlock / 64, TRAIN ? 3 : 2> sq;
   uint32_t par = 0;
+  // the next entry's ray and hit, loaded at the end of the current one (as
+  // k_shade)
+  float4 next_a = make_float4(0.f, 0.f, 0.f, 0.f), next_b = next_a;
+  Hit next_hit{0.f, LAM_NONE, LAM_NONE, LAM_NONE};
+  auto fetch = [&](uint32_t q) {
+    if (q < n) {
+      next_a = rin[2 * size_t(q)];
+      next_b = rin[2 * size_t(q) + 1];
+      next_hit = ws.hits[q];
+    }
+  };
+  fetch(blockIdx.x * kShadeBlock + threadIdx.x);
   for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
     const uint32_t q = bbase + threadIdx.x;
     bool push_next = false, push_shadow = false, live = false, guided = false, want_rec = false;
@@ -447,14 +460,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
     Surf sf;
     f3 o = mk(0.f, 0.f, 1.f);
     float gin[kGIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float4 a = next_a, b = next_b;
+    const Hit hit = next_hit;
     if (q < n) {
-      const float4 a = rin[2 * size_t(q)], b = rin[2 * size_t(q) + 1];
       const f3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
       p = __float_as_uint(b.w);
-      const Hit hit = ws.hits[q];
       if (hit.obj != LAM_NONE) {
         uint32_t pixel, sample;
-        path_id(ids, ctrl, p, pixel, sample);
+        path_id_of(ids, pass, p, pixel, sample);
         key = lam_path_key(pa.seed, pixel, sample);
         lam_for<H>(sc, key, lam);
 #pragma unroll
@@ -638,6 +651,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_guided(DevScene sc, PathA
       block_append(sq, par, {push_next, push_shadow, want_rec}, {next_count, ctrl + CTRL_SHADOW, gd.rec_count}, slot);
     else
       block_append(sq, par, {push_next, push_shadow}, {next_count, ctrl + CTRL_SHADOW}, slot);
+    fetch(q + stride);
     if constexpr (TRAIN) {
       const uint32_t r = slot[TRAIN ? 2 : 0];
       if (live) gd.rec_of_slot[p] = (want_rec && r < gd.rec_cap) ? r : LAM_NONE;
@@ -862,8 +876,7 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
                                  hipStream_t st, WaveProfile *prof, const GuideDev *gd, bool train) {
   const uint32_t b256 = (n + 255) / 256;
   // (rounded up to whole float4s, as the staging copies them)
-  const uint32_t spec_bytes = (sc.n_spectra * sc.grid_n + 3u) / 4u * 16u;
-  const uint32_t tab_bytes = tab_lds(sc).total4() * 16u;
+  const StagePlan plain = stage_plan(sc, true), guided = stage_plan(sc, false);
   auto ev = [&](int slot) {
     if (prof) (void)hipEventRecord(prof->ev[slot], st);
   };
@@ -898,21 +911,31 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     if (gv)
       hipLaunchKernelGGL((k_guide_queue<H>), dim3(wl.guide_blocks), dim3(256), 0, st, sc, pa, ids, ws, *gv, s, q,
                          ctrl);
-    if (!gv && tab_bytes <= kTabLdsBytes)
-      hipLaunchKernelGGL((k_shade<H, 2>), dim3(wl.shade_blocks), dim3(kShadeBlock), tab_bytes, st, sc, pa, ids,
-                         ws, s, q, ctrl);
-    else if (!gv && spec_bytes <= kSpecLdsBytes)
-      hipLaunchKernelGGL((k_shade<H, 1>), dim3(wl.shade_blocks), dim3(kShadeBlock), spec_bytes, st, sc, pa, ids,
-                         ws, s, q, ctrl);
-    else if (!gv)
-      hipLaunchKernelGGL((k_shade<H, 0>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, s,
-                         q, ctrl);
-    else if (train)
-      hipLaunchKernelGGL((k_shade_guided<H, true>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws, *gv,
-                         s, q, ctrl);
-    else
-      hipLaunchKernelGGL((k_shade_guided<H, false>), dim3(wl.shade_blocks), dim3(kShadeBlock), 0, st, sc, pa, ids, ws,
-                         *gv, s, q, ctrl);
+    if (!gv) {
+      const dim3 g(wl.shade_blocks), b(kShadeBlock);
+      switch (plain.stage) {
+        case 3: hipLaunchKernelGGL((k_shade<H, 3>), g, b, plain.bytes, st, sc, pa, ids, ws, s, q, ctrl); break;
+        case 2: hipLaunchKernelGGL((k_shade<H, 2>), g, b, plain.bytes, st, sc, pa, ids, ws, s, q, ctrl); break;
+        case 1: hipLaunchKernelGGL((k_shade<H, 1>), g, b, plain.bytes, st, sc, pa, ids, ws, s, q, ctrl); break;
+        default: hipLaunchKernelGGL((k_shade<H, 0>), g, b, 0, st, sc, pa, ids, ws, s, q, ctrl); break;
+      }
+    } else {
+      const dim3 g(wl.shade_blocks), b(kShadeBlock);
+      const uint32_t lb = guided.bytes;
+      if (train) {
+        switch (guided.stage) {
+          case 3: hipLaunchKernelGGL((k_shade_guided<H, true, 3>), g, b, lb, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+          case 2: hipLaunchKernelGGL((k_shade_guided<H, true, 2>), g, b, lb, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+          default: hipLaunchKernelGGL((k_shade_guided<H, true, 0>), g, b, 0, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+        }
+      } else {
+        switch (guided.stage) {
+          case 3: hipLaunchKernelGGL((k_shade_guided<H, false, 3>), g, b, lb, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+          case 2: hipLaunchKernelGGL((k_shade_guided<H, false, 2>), g, b, lb, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+          default: hipLaunchKernelGGL((k_shade_guided<H, false, 0>), g, b, 0, st, sc, pa, ids, ws, *gv, s, q, ctrl); break;
+        }
+      }
+    }
     if (prof) (void)hipEventRecord(prof->shd0[s], st);
     if (gv && train)
       hipLaunchKernelGGL((k_shadow<H, false, true>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, ctrl, nullptr,
