@@ -316,11 +316,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_adj_pending(DevScene sc, PathAr
   grad_lds_flush(gl, dv);
 }
 
-template <int H>
+// STAGE: scene tables staged in LDS after the gradient sums (stage_tables);
+// the next entry's ray and hit are loaded after the block append (as
+// k_shade)
+template <int H, int STAGE>
 __global__ __launch_bounds__(kShadeBlock) void k_adj_shade(DevScene sc, PathArgs pa, PathIds ids, WaveState ws,
                                                            ReplayDev rd, DiffDev dv, uint32_t s, uint32_t qsel,
                                                            uint32_t *__restrict__ ctrl) {
   const uint32_t n = ctrl[CTRL_COUNT0 + qsel];
+  const uint32_t pass = ctrl[CTRL_PASS];
   const float4 *__restrict__ rin = reinterpret_cast<const float4 *>(ws.ray[qsel]);
   float4 *__restrict__ rout = reinterpret_cast<float4 *>(ws.ray[qsel ^ 1u]);
   const size_t cap = ws.cap;
@@ -328,23 +332,34 @@ __global__ __launch_bounds__(kShadeBlock) void k_adj_shade(DevScene sc, PathArgs
   __shared__ BlockQueues<kShadeBlock / 64, 2> sq;
   extern __shared__ float gl[];
   if (blockIdx.x * kShadeBlock >= n) return;
+  if constexpr (STAGE > 0) stage_tables<STAGE, kShadeBlock>(sc, (dv.n_params + 3u) / 4u);
   grad_lds_init(gl, dv);
   uint32_t par = 0;
+  float4 next_a = make_float4(0.f, 0.f, 0.f, 0.f), next_b = next_a;
+  Hit next_hit{0.f, LAM_NONE, LAM_NONE, LAM_NONE};
+  auto fetch = [&](uint32_t q) {
+    if (q < n) {
+      next_a = rin[2 * size_t(q)];
+      next_b = rin[2 * size_t(q) + 1];
+      next_hit = ws.hits[q];
+    }
+  };
+  fetch(blockIdx.x * kShadeBlock + threadIdx.x);
   for (uint32_t bbase = blockIdx.x * kShadeBlock; bbase < n; bbase += stride, par ^= 1u) {
     const uint32_t q = bbase + threadIdx.x;
     bool push_next = false, push_shadow = false;
     uint32_t p = 0;
     Ray nxt, shd;
     Lam<H> lam;
+    const float4 ra = next_a, rb = next_b;
+    const Hit hit = next_hit;
     if (q < n) {
-      const float4 ra = rin[2 * size_t(q)], rb = rin[2 * size_t(q) + 1];
       const f3 ro = mk(ra.x, ra.y, ra.z), rdir = mk(rb.x, rb.y, rb.z);
       p = __float_as_uint(rb.w);
       uint32_t pixel, sample;
-      path_id(ids, ctrl, p, pixel, sample);
+      path_id_of(ids, pass, p, pixel, sample);
       const uint64_t key = lam_path_key(pa.seed, pixel, sample);
       lam_for<H>(sc, key, lam);
-      const Hit hit = ws.hits[q];
       if (hit.obj != LAM_NONE && s <= pa.max_scat) {
         const Surf sf = surface(sc, hit, ro, rdir);
         const f3 o = neg(normalized(rdir));
@@ -435,6 +450,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_adj_shade(DevScene sc, PathArgs
     }
     uint32_t slot[2];
     block_append(sq, par, {push_next, push_shadow}, {ctrl + CTRL_COUNT0 + (qsel ^ 1u), ctrl + CTRL_SHADOW}, slot);
+    fetch(q + stride);
     if (push_next) {
       rout[2 * size_t(slot[0])] = make_float4(nxt.o.x, nxt.o.y, nxt.o.z, kFltMax);
       rout[2 * size_t(slot[0]) + 1] = make_float4(nxt.d.x, nxt.d.y, nxt.d.z, __uint_as_float(p));
@@ -537,6 +553,10 @@ static hipError_t enqueue_replay_t(const DevScene &sc, const PathArgs &pa, const
                                    const float *dloss, float inv_n, const WaveLaunch &wl, hipStream_t st) {
   const uint32_t b256 = (n + 255) / 256;
   const size_t gl_bytes = size_t(dv.n_params) * 4;
+  // the replay shade stages the scene tables after the gradient sums when
+  // both fit 64 KB
+  const size_t gl4 = (size_t(dv.n_params) + 3) / 4 * 16;
+  const StagePlan plan = gl4 < 64 * 1024 ? stage_plan(sc, false, uint32_t(64 * 1024 - gl4)) : StagePlan{0, 0u};
   hipLaunchKernelGGL((k_adj_raygen<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, rd, n, ctrl, dloss, inv_n);
   uint32_t q = 0;
   const uint32_t iters = pa.max_scat + 2;
@@ -545,8 +565,13 @@ static hipError_t enqueue_replay_t(const DevScene &sc, const PathArgs &pa, const
     e = enqueue_iter_extend(sc, ws, ctrl, q, wl, st);
     hipLaunchKernelGGL((k_adj_pending<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), gl_bytes, st, sc, pa, ids, ws,
                        rd, dv, q, ctrl);
-    hipLaunchKernelGGL((k_adj_shade<H>), dim3(wl.shade_blocks), dim3(kShadeBlock), gl_bytes, st, sc, pa, ids, ws, rd,
-                       dv, s, q, ctrl);
+    const dim3 g(wl.shade_blocks), b(kShadeBlock);
+    const size_t sb = gl4 + plan.bytes;
+    switch (plan.stage) {
+      case 3: hipLaunchKernelGGL((k_adj_shade<H, 3>), g, b, sb, st, sc, pa, ids, ws, rd, dv, s, q, ctrl); break;
+      case 2: hipLaunchKernelGGL((k_adj_shade<H, 2>), g, b, sb, st, sc, pa, ids, ws, rd, dv, s, q, ctrl); break;
+      default: hipLaunchKernelGGL((k_adj_shade<H, 0>), g, b, gl_bytes, st, sc, pa, ids, ws, rd, dv, s, q, ctrl); break;
+    }
     hipLaunchKernelGGL((k_adj_shadow<H>), dim3(wl.trace_blocks), dim3(64), 0, st, sc, ws, rd, ctrl);
     q ^= 1u;
   }

@@ -3,6 +3,7 @@
 // replay passes): the control-block words, path identities and the
 // block-aggregated queue appends.
 #pragma once
+#include <algorithm>
 #include "lam_launch.h"
 #include "lam_shade.h"
 
@@ -57,8 +58,9 @@ __host__ __device__ inline TabLds tab_lds(const DevScene &sc) {
 // threads per block.
 constexpr int kStageSpectra = 1, kStageTables = 2;
 template <int STAGE, uint32_t NT>
-__device__ __forceinline__ void stage_tables(DevScene &sc) {
-  extern __shared__ float4 lds_tab[];
+__device__ __forceinline__ void stage_tables(DevScene &sc, uint32_t off4 = 0) {
+  extern __shared__ float4 lds_dyn[];
+  float4 *lds_tab = lds_dyn + off4;  // after off4 float4s the caller keeps
   const TabLds t = tab_lds(sc);
   float4 *mat = lds_tab;
   if constexpr (STAGE & kStageSpectra) {
@@ -94,12 +96,12 @@ struct StagePlan {
   int stage;
   uint32_t bytes;
 };
-inline StagePlan stage_plan(const DevScene &sc, bool spectra_alone) {
+inline StagePlan stage_plan(const DevScene &sc, bool spectra_alone, uint32_t budget = kTabLdsBytes) {
   const TabLds t = tab_lds(sc);
   const uint32_t spec = t.spec4 * 16u, tab = (t.mat4 + t.light4 + t.obj4) * 16u;
-  if (spec + tab <= kTabLdsBytes) return {kStageSpectra | kStageTables, spec + tab};
-  if (tab <= kTabLdsBytes) return {kStageTables, tab};
-  if (spectra_alone && spec <= kSpecLdsBytes) return {kStageSpectra, spec};
+  if (spec + tab <= budget) return {kStageSpectra | kStageTables, spec + tab};
+  if (tab <= budget) return {kStageTables, tab};
+  if (spectra_alone && spec <= std::min(budget, kSpecLdsBytes)) return {kStageSpectra, spec};
   return {0, 0u};
 }
 
