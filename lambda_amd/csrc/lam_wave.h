@@ -145,11 +145,10 @@ __device__ __forceinline__ void path_id_of(const PathIds &ids, uint32_t pass, ui
     return;
   }
   uint32_t q = 0;
-  if (ids.npix)
-    while (slot >= ids.npix) {  // a few samples per pass at most
-      slot -= ids.npix;
-      q++;
-    }
+  if (ids.npix) {  // (a uniform divisor: one division sequence per lane)
+    q = slot / ids.npix;
+    slot -= q * ids.npix;
+  }
   pixel = slot + ids.pixel_offset;
   sample = pass + q;
 }

@@ -854,9 +854,18 @@ __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds 
   }
   const uint32_t npx = ids.npix ? ids.npix : n;
   if (t >= npx) return;
-  for (uint32_t p = t; p < n; p += npx) {
+  const uint32_t pass = ids.pixel ? 0u : ctrl[CTRL_PASS];
+  // slot p = q npx + t: pixel t (+ offset), sample pass + q (path_id
+  // without its per-slot division loop)
+  for (uint32_t p = t, q = 0; p < n; p += npx, q++) {
     uint32_t pixel, sample;
-    path_id(ids, ctrl, p, pixel, sample);
+    if (ids.pixel) {
+      pixel = ids.pixel[p];
+      sample = ids.sample[p];
+    } else {
+      pixel = t + ids.pixel_offset;
+      sample = pass + q;
+    }
     const uint64_t key = lam_path_key(pa.seed, pixel, sample);
     Lam<H> lam;
     const float inv_om = lam_for<H>(sc, key, lam);
