@@ -63,3 +63,13 @@ def lights_scene(tmp_path_factory):
     from tests.scene_text import all_lights_scene_text
     d = tmp_path_factory.mktemp("lights")
     return la.Scene.from_text(all_lights_scene_text(str(d)))
+
+
+@pytest.fixture(scope="session")
+def crowd_scene(tmp_path_factory):
+    """the mixed scene plus 520 small mesh instances (a large object table,
+    small spectra)"""
+    import lambda_amd as la
+    from tests.scene_text import crowd_scene_text
+    d = tmp_path_factory.mktemp("crowd")
+    return la.Scene.from_text(crowd_scene_text(str(d)))

@@ -58,3 +58,24 @@ def all_lights_scene_text(d):
         {"name": "sky", "type": "ambient", "radiance": {"val": 0.08}},
     ]
     return json.dumps(s)
+
+
+def crowd_scene_text(d, n=520):
+    """the mixed scene plus n small meshes ("pebbles", a frequency-1
+    geodesic sphere each, one shared grey material) scattered over the
+    floor: the object table outgrows the shade kernels' LDS budget (n x 112
+    bytes) while the spectra stay small, so the plain shade kernel stages
+    the spectra alone"""
+    import json
+    from lambda_amd import scenes
+    V, F, N = scenes.geodesic_sphere(1, seed=5)
+    scenes.write_obj(os.path.join(d, "pebble.obj"), V, F, N)
+    s = json.loads(mixed_scene_text(d))
+    rng = np.random.default_rng(11)
+    for k in range(n):
+        x, y = rng.uniform(-14.0, 14.0), rng.uniform(-12.0, 18.0)
+        r = float(rng.uniform(0.3, 0.8))
+        s["objects"].append({"name": f"p{k}", "type": "trimesh", "source": os.path.join(d, "pebble.obj"),
+                             "transform": {"scale": [r, r, r], "translate": [float(x), float(y), r]},
+                             "material": "grey"})
+    return json.dumps(s)

@@ -213,6 +213,48 @@ def test_render_matches_oracle_same_samples(ctx, request, scene_name, hero, flag
     assert relmse < 1e-8, relmse
 
 
+def _shade_stage(scene):
+    """what the shade kernels stage in LDS for `scene` (stage_plan in
+    lam_wave.h): 3 spectra and tables, 2 the tables, 1 the spectra (the
+    plain shade kernel only), 0 nothing"""
+    D = scene.desc
+    spec = (D.n_spectra * D.grid.n + 3) // 4 * 16
+    tab = ((D.n_materials * 56 + 15) // 16 + D.n_lights * 3 + D.n_objects * 7) * 16
+    if spec + tab <= 52 * 1024:
+        return 3
+    if tab <= 52 * 1024:
+        return 2
+    return 1 if spec <= 48 * 1024 else 0
+
+
+def test_stage_choice_of_the_test_scenes(box_scene, mixed_scene, crowd_scene, field_scene):
+    """the parity scenes cover the shade kernels' LDS staging variants (the
+    glossy interior's tables alone and the stress field's none are covered
+    by the full-size tests)"""
+    assert _shade_stage(box_scene) == 3 and _shade_stage(mixed_scene) == 3 and _shade_stage(field_scene) == 3
+    assert _shade_stage(crowd_scene) == 1
+
+
+@pytest.mark.parametrize("hero", [4, 16])
+def test_spectra_only_staging_parity(ctx, crowd_scene, hero):
+    """a large object table (530 objects) beside small spectra: the shade
+    kernel stages the spectra alone; paths and images against the oracle"""
+    ctx.upload(crowd_scene)
+    orc = ob.Oracle(crowd_scene)
+    npix = crowd_scene.width * crowd_scene.height
+    rng = np.random.default_rng(17)
+    pix = rng.integers(0, npix, 8192).astype(np.uint32)
+    smp = rng.integers(0, 64, 8192).astype(np.uint32)
+    pr = la.params(spp=1, max_scatterings=8, sample_p=0.8, hero=hero, seed=9)
+    g, o = ctx.trace_paths(pr, pix, smp), orc.trace_paths(pr, pix, smp)
+    assert (o > 0).mean() > 0.05
+    _check_paths(g, o)
+    pr = la.params(spp=4, max_scatterings=8, sample_p=0.8, hero=hero, seed=1)
+    gi, oi = ctx.render(pr), orc.render(pr)
+    relmse = np.mean((gi - oi) ** 2) / np.mean(oi ** 2)
+    assert relmse < 1e-8, relmse
+
+
 def test_tail_handoff_is_exact(ctx, tmp_path):
     """262,144 paths: the wavefront runs the first bounces and hands the
     short queue to the finish kernel; the image must equal the pure
