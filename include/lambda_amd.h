@@ -374,6 +374,13 @@ int lam_extend_dev_timed(lam_ctx *ctx, const lam_ray *d_rays, size_t n,
  * comparison with the CPU oracle. */
 int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t n, float *out);
 
+/* Test hook (no counterpart in the reference): the device BVH builder's
+ * stable radix sort of (key, value) pairs by the low `bits` (1..32) bits of
+ * the keys, on host arrays of n <= 2^28 pairs; keys_out/vals_out receive the
+ * sorted pairs. */
+int lam_debug_radix_sort(lam_ctx *ctx, const uint32_t *keys, const uint32_t *vals, size_t n, uint32_t bits,
+                         uint32_t *keys_out, uint32_t *vals_out);
+
 /* ---- progressive rendering and checkpoints ------------------------------
  * No counterpart in the reference (progress/ETA output is listed as future
  * work in its README.md:24).  A render with LAM_FLAG_ACCUMULATE adds passes

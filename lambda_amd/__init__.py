@@ -220,6 +220,7 @@ def lib():
         "lam_tone_map": (C.c_int, [vp, C.c_float, C.POINTER(C.c_uint8)]),
         "lam_set_wave_config": (C.c_int, [vp, C.POINTER(WaveConfig)]),
         "lam_debug_math": (C.c_int, [vp, C.c_int, fp, fp, sz, fp]),
+        "lam_debug_radix_sort": (C.c_int, [vp, u32p, u32p, sz, C.c_uint32, u32p, u32p]),
         "lamh_load_nls": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
         "lamh_load_nls_text": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, sz]),
         "lamh_scene_free": (None, [vp]),
@@ -824,6 +825,20 @@ class Context:
         out = np.zeros_like(a)
         self._check(self._lib.lam_debug_math(self._h, int(fn), _fptr(a), _fptr(b), len(a), _fptr(out)))
         return out
+
+    def debug_radix_sort(self, keys, vals, bits=32):
+        """the device BVH builder's stable radix sort of (keys, vals) by the
+        low `bits` bits of the keys (test hook)"""
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        v = np.ascontiguousarray(vals, dtype=np.uint32)
+        ko, vo = np.zeros_like(k), np.zeros_like(v)
+        self._check(self._lib.lam_debug_radix_sort(self._h, _u32ptr(k), _u32ptr(v), len(k), int(bits),
+                                                   _u32ptr(ko), _u32ptr(vo)))
+        return ko, vo
+
+
+def _u32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
 def host_trace(scene, rays, any_hit=False):

@@ -1625,6 +1625,31 @@ int lam_debug_math(lam_ctx *ctx, int fn, const float *a, const float *b, size_t 
   return LAM_OK;
 }
 
+int lam_debug_radix_sort(lam_ctx *ctx, const uint32_t *keys, const uint32_t *vals, size_t n, uint32_t bits,
+                         uint32_t *keys_out, uint32_t *vals_out) {
+  if (!ctx) return LAM_E_INVALID;
+  if (n == 0) return LAM_OK;
+  if (!keys || !vals || !keys_out || !vals_out) return fail(ctx, LAM_E_INVALID, "NULL buffer");
+  if (bits < 1 || bits > 32) return fail(ctx, LAM_E_INVALID, "bits must be 1..32");
+  if (n > (size_t(1) << 28)) return fail(ctx, LAM_E_LIMIT, "batch too large");
+  HIPC(hipSetDevice(ctx->device));
+  int rc;
+  // (slot 3 holds the film of the last render: not used here)
+  const int slots[4] = {0, 1, 2, 4};
+  for (int i : slots)
+    if ((rc = ensure_scratch(ctx, i, n * 4))) return rc;
+  if ((rc = ensure_scratch(ctx, 5, lam::radix_sort_scratch(uint32_t(n))))) return rc;
+  uint32_t *kin = static_cast<uint32_t *>(ctx->scratch[0]), *vin = static_cast<uint32_t *>(ctx->scratch[1]);
+  uint32_t *kout = static_cast<uint32_t *>(ctx->scratch[2]), *vout = static_cast<uint32_t *>(ctx->scratch[4]);
+  HIPC(hipMemcpyAsync(kin, keys, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(vin, vals, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(lam::radix_sort_pairs(kin, vin, kout, vout, uint32_t(n), bits, ctx->scratch[5], ctx->stream));
+  HIPC(hipMemcpyAsync(keys_out, kout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipMemcpyAsync(vals_out, vout, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return LAM_OK;
+}
+
 int lam_get_timing(const lam_ctx *ctx, lam_timing *out) {
   if (!ctx || !out) return LAM_E_INVALID;
   *out = ctx->timing;

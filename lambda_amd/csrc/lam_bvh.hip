@@ -13,7 +13,6 @@
 // host, milliseconds here).  Images do not depend on the tree: the closest
 // hit and its tie-break are exact for any conservative BVH.
 #include <cfloat>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "lam_launch.h"
 
@@ -437,13 +436,179 @@ __global__ void k_root(uint32_t n, uint32_t *node_ref, uint32_t *node_acc, uint3
   }
 }
 
+// ---- stable LSD radix sort of (key, value) pairs, 8 bits per pass ----------
+// Per pass over a tile partition of kRsTile keys per block: k_rs_hist counts
+// the tile's digits (LDS) into hist[digit * tiles + tile]; an exclusive
+// scan over that digit-major array (k_scan_local / k_scan_top /
+// k_scan_add) gives every (digit, tile) its first output index; k_rs_scatter
+// re-reads the tile in rounds of 256 keys and writes each key to its
+// digit's next slot in tile order: within a wave the rank among equal
+// digits comes from eight ballots (the lanes whose digit matches bit for
+// bit), across the block's four waves from per-wave counts in LDS.  Keys
+// keep their input order within a digit, so the sort is stable and its
+// output is the one stable order of (key, index) pairs.
+constexpr uint32_t kRsTile = 2048;  // keys per block (8 rounds of 256)
+constexpr uint32_t kScanBlock = 1024;  // elements per block of the scan (256 threads x 4)
+
+__device__ __forceinline__ uint32_t rs_digit(uint32_t key, uint32_t shift) { return (key >> shift) & 0xFFu; }
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
+                                                 uint32_t tiles, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kRsTile;
+  for (uint32_t i = base + threadIdx.x; i < min(base + kRsTile, n); i += 256u)
+    atomicAdd(&cnt[rs_digit(keys[i], shift)], 1u);
+  __syncthreads();
+  hist[size_t(threadIdx.x) * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan, level 1: each block scans kScanBlock elements in place
+// and leaves their total in sums[block]
+__global__ __launch_bounds__(256) void k_scan_local(uint32_t *__restrict__ a, uint32_t n, uint32_t *__restrict__ sums) {
+  __shared__ uint32_t part[256];
+  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+  uint32_t v[4], t = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v[k] = base + k < n ? a[base + k] : 0u;
+    t += v[k];
+  }
+  part[threadIdx.x] = t;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off *= 2) {  // inclusive Hillis-Steele over the 256 thread totals
+    const uint32_t x = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - t;  // exclusive prefix of this thread's four
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (base + k < n) a[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) sums[blockIdx.x] = part[255];
+}
+// level 2: one block turns the block totals into exclusive offsets
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t *__restrict__ sums, uint32_t m) {
+  __shared__ uint32_t part[256];
+  uint32_t carry = 0;
+  for (uint32_t c = 0; c < m; c += 256) {
+    const uint32_t i = c + threadIdx.x;
+    const uint32_t v = i < m ? sums[i] : 0u;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off *= 2) {
+      const uint32_t x = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+      __syncthreads();
+      part[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (i < m) sums[i] = carry + part[threadIdx.x] - v;
+    const uint32_t tot = part[255];
+    __syncthreads();
+    carry += tot;
+  }
+}
+// level 3: add each block's offset
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *__restrict__ a, uint32_t n,
+                                                  const uint32_t *__restrict__ sums) {
+  const uint32_t add = sums[blockIdx.x];
+  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4u;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (base + k < n) a[base + k] += add;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                    uint32_t n, uint32_t shift, uint32_t tiles,
+                                                    const uint32_t *__restrict__ offs, uint32_t *__restrict__ kout,
+                                                    uint32_t *__restrict__ vout) {
+  __shared__ uint32_t next[256];     // next output index of each digit for this tile
+  __shared__ uint32_t wcnt[4][256];  // this round's digit counts per wave
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  next[threadIdx.x] = offs[size_t(threadIdx.x) * tiles + blockIdx.x];
+  const uint32_t base = blockIdx.x * kRsTile;
+  for (uint32_t r = 0; r < kRsTile; r += 256) {
+    const uint32_t i = base + r + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u, val = valid ? vin[i] : 0u;
+    const uint32_t d = rs_digit(key, shift);
+#pragma unroll
+    for (int k = 0; k < 4; k++) wcnt[wave][lane * 4 + k] = 0;
+    __syncthreads();
+    // the lanes of this wave with the same digit: match all eight bits
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t lrank = __popcll(peers & below);
+    if (valid && lrank == 0) wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = next[d] + lrank;
+      for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
+      if (pos < n) {  // always, when the counts are consistent
+        kout[pos] = key;
+        vout[pos] = val;
+      }
+    }
+    __syncthreads();
+    next[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
+    __syncthreads();  // the counts are read before the next round clears them
+  }
+}
+
 }  // namespace
 
+// scratch of radix_sort_pairs for n keys
+size_t radix_sort_scratch(uint32_t n) {
+  const size_t tiles = (size_t(n) + kRsTile - 1) / kRsTile;
+  const size_t hn = 256 * tiles;
+  return (hn * 4 + 255) / 256 * 256 + ((hn + kScanBlock - 1) / kScanBlock * 4 + 255) / 256 * 256 +
+         2 * ((size_t(n) * 4 + 255) / 256 * 256);
+}
+// Sorts (keys, vals) by the low `bits` bits of the keys, stable; the result
+// is in (kout, vout); kin and vin are left unchanged.
+hipError_t radix_sort_pairs(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, uint32_t n,
+                            uint32_t bits, void *scratch, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const uint32_t tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t hn = 256u * tiles;
+  const uint32_t sb = (hn + kScanBlock - 1) / kScanBlock;
+  char *p = static_cast<char *>(scratch);
+  auto take = [&](size_t bytes) {
+    void *r = p;
+    p += (bytes + 255) / 256 * 256;
+    return static_cast<uint32_t *>(r);
+  };
+  uint32_t *hist = take(size_t(hn) * 4), *sums = take(size_t(sb) * 4);
+  uint32_t *tk = take(size_t(n) * 4), *tv = take(size_t(n) * 4);
+  const uint32_t passes = (bits + 7) / 8;
+  // ping-pong so that the last pass writes (kout, vout)
+  const uint32_t *ik = kin, *iv = vin;
+  for (uint32_t ps = 0; ps < passes; ps++) {
+    const bool to_out = ((passes - 1 - ps) % 2) == 0;
+    uint32_t *ok = to_out ? kout : tk, *ov = to_out ? vout : tv;
+    hipLaunchKernelGGL(k_rs_hist, dim3(tiles), dim3(256), 0, st, ik, n, 8u * ps, tiles, hist);
+    hipLaunchKernelGGL(k_scan_local, dim3(sb), dim3(256), 0, st, hist, hn, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, sums, sb);
+    hipLaunchKernelGGL(k_scan_add, dim3(sb), dim3(256), 0, st, hist, hn, sums);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(256), 0, st, ik, iv, n, 8u * ps, tiles, hist, ok, ov);
+    ik = ok;
+    iv = ov;
+  }
+  return hipGetLastError();
+}
+
 size_t gpu_bvh_scratch_bytes(uint32_t n) {
-  size_t sort_bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, static_cast<uint32_t *>(nullptr),
-                                  static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
-                                  static_cast<uint32_t *>(nullptr), n, 0, 30);
+  const size_t sort_bytes = radix_sort_scratch(n);
   const size_t m = n > 1 ? n - 1 : 1;
   return (size_t(n) * 32)           // pbox
          + size_t(n) * 16           // keys, vals (in and out)
@@ -486,10 +651,8 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
   uint32_t *partial = static_cast<uint32_t *>(take(kRedBlocks * 24));
   float *bounds = static_cast<float *>(take(32));
   uint32_t *stats = static_cast<uint32_t *>(take(32));
-  size_t sort_bytes = 0;
-  hipError_t e = rocprim::radix_sort_pairs(nullptr, sort_bytes, keys, skeys, vals, perm, n, 0, 30, stream);
-  if (e != hipSuccess) return e;
-  void *sort_tmp = take(sort_bytes);
+  void *sort_tmp = take(radix_sort_scratch(n));
+  hipError_t e = hipSuccess;
 
   const uint32_t b256 = (n + 255) / 256;
   const uint32_t nb = std::min(kRedBlocks, b256);
@@ -497,7 +660,8 @@ hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, stream, partial, nb, bounds);
   hipLaunchKernelGGL(k_morton, dim3(b256), dim3(256), 0, stream, pbox, n, bounds, keys, vals);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = rocprim::radix_sort_pairs(sort_tmp, sort_bytes, keys, skeys, vals, perm, n, 0, 30, stream);
+  // Morton codes in leaf order, the triangle of each leaf beside
+  e = radix_sort_pairs(keys, vals, skeys, perm, n, 30, sort_tmp, stream);
   if (e != hipSuccess) return e;
   if (n >= 2) {
     if ((e = launch_zero(arrive, m * 4, stream)) != hipSuccess) return e;

@@ -152,6 +152,12 @@ size_t gpu_bvh_scratch_bytes(uint32_t n);
 // Synchronises the stream (the collapse reads each level's node count).
 hipError_t gpu_build_bvh8(const float4 *tri, const float4 *shd, uint32_t n, void *scratch, Node8d *nodes,
                           float4 *tri_out, float4 *shd_out, GpuBvhStats *st, hipStream_t stream);
+// the build's stable LSD radix sort of (key, value) pairs by the low `bits`
+// bits of the keys: scratch of radix_sort_scratch(n) bytes; kin/vin are
+// left unchanged, the sorted pairs go to kout/vout
+size_t radix_sort_scratch(uint32_t n);
+hipError_t radix_sort_pairs(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, uint32_t n,
+                            uint32_t bits, void *scratch, hipStream_t st);
 
 // ---- path-replay adjoint (lam_adjoint.hip) --------------------------------------
 constexpr uint32_t kMaxDiffParams = 8192;  // gradient slots held in LDS per block

@@ -344,6 +344,22 @@ def test_deep_bvh_uploads_and_renders(ctx, tmp_path):
     _check_paths(ctx.trace_paths(p1, pix, smp), orc.trace_paths(p1, pix, smp))
 
 
+@pytest.mark.parametrize("n,bits,span", [(1, 30, 8), (5, 30, 4), (2047, 30, 1 << 30), (2048, 8, 256),
+                                         (2049, 13, 1 << 13), (100_003, 30, 1 << 30), (100_003, 30, 500),
+                                         (1_000_000, 32, 1 << 32), (1_000_000, 30, 1 << 12)])
+def test_radix_sort_is_the_stable_sort(ctx, n, bits, span):
+    """the device BVH builder's radix sort against numpy's stable argsort:
+    partial and whole tiles, few and many distinct keys (equal keys keep
+    their input order), keys wider than `bits` (the high bits are ignored)"""
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, span, n, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)[::-1].copy()
+    ko, vo = ctx.debug_radix_sort(keys, vals, bits)
+    sort_key = keys & np.uint32((1 << bits) - 1) if bits < 32 else keys
+    o = np.argsort(sort_key, kind="stable")
+    assert np.array_equal(ko, keys[o]) and np.array_equal(vo, vals[o])
+
+
 @pytest.mark.parametrize("scene_name", ["box_scene", "field_scene", "mixed_scene", "sheets"])
 def test_gpu_bvh_same_hits_and_images(ctx, request, scene_name):
     """the BVH8 built on the device (linear BVH, LAM_UPLOAD_GPU_BVH) gives the
