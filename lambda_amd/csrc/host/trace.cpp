@@ -32,7 +32,7 @@ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 constexpr float kBias = 0x1.6a09e6p-12f;
 constexpr float kEps = 1.1920928955078125e-07f;
-constexpr float kRobust = 1.0000003576278687f;
+constexpr float kRobustQ = 1.000000476837158203125f;  // 1 + 2^-21
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 struct Hit {
@@ -131,9 +131,10 @@ bool trace(const Scene &S, const lam_ray &ray, bool any, Hit &best, uint32_t &no
     }
   }
   if (S.bvh.nodes.empty()) return true;
-  const float ix = u.x != 0.f ? 1.f / u.x : FLT_MAX;
-  const float iy = u.y != 0.f ? 1.f / u.y : FLT_MAX;
-  const float iz = u.z != 0.f ? 1.f / u.z : FLT_MAX;
+  // inverse direction clamped to +-2^64, as trav_begin does
+  const float ix = std::fmin(std::fmax(u.x != 0.f ? 1.f / u.x : FLT_MAX, -0x1p64f), 0x1p64f);
+  const float iy = std::fmin(std::fmax(u.y != 0.f ? 1.f / u.y : FLT_MAX, -0x1p64f), 0x1p64f);
+  const float iz = std::fmin(std::fmax(u.z != 0.f ? 1.f / u.z : FLT_MAX, -0x1p64f), 0x1p64f);
   const V3 mr = cross(o, u);
   uint32_t stack[128];
   int sp = 0;
@@ -145,17 +146,38 @@ bool trace(const Scene &S, const lam_ray &ray, bool any, Hit &best, uint32_t &no
       nodes++;
       float tk[8];
       uint32_t ck[8];
+      // the kernel's slab test in the node's quantized frame (trav_step in
+      // lam_device.h): t = q (s ix) + (origin - o) ix per plane, one fused
+      // multiply-add from b - E (near) or b + E (far), the far distance
+      // widened by 1 + 2^-21
+      const float ixv[3] = {ix, iy, iz}, rov[3] = {o.x, o.y, o.z};
+      const uint8_t ev[3] = {nd.ex, nd.ey, nd.ez};
+      float av[3], bn[3], bf[3];
+      bool pos[3];
+      for (int j = 0; j < 3; j++) {
+        // scale 2^(e-127) as the kernel reads it: the exponent field alone
+        // (e = 0 reads as 0)
+        const uint32_t sbits = uint32_t(ev[j]) << 23;
+        float sj;
+        std::memcpy(&sj, &sbits, 4);
+        const float b = (nd.origin[j] - rov[j]) * ixv[j];
+        const float e = (std::fabs(b) + (std::fabs(nd.origin[j]) + 255.f * sj) * std::fabs(ixv[j])) * 0x1p-22f;
+        av[j] = sj * ixv[j];
+        bn[j] = b - e;
+        bf[j] = b + e;
+        pos[j] = ixv[j] > 0.f;
+      }
       for (int k = 0; k < 8; k++) {
-        float lo[3], hi[3];
-        lamh::decode_child_box(nd, k, lo, hi);
-        const float tx0 = (lo[0] - o.x) * ix, tx1 = (hi[0] - o.x) * ix;
-        const float ty0 = (lo[1] - o.y) * iy, ty1 = (hi[1] - o.y) * iy;
-        const float tz0 = (lo[2] - o.z) * iz, tz1 = (hi[2] - o.z) * iz;
-        const float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)),
-                                   std::fmax(std::fmin(tz0, tz1), 0.f));
-        const float tf = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)),
-                                   std::fmin(std::fmax(tz0, tz1), best.t));
-        const bool hit = k < nd.nchild && tn <= tf * kRobust;
+        float tnj[3], tfj[3];
+        for (int j = 0; j < 3; j++) {
+          const float qn = float(pos[j] ? nd.qlo[j][k] : nd.qhi[j][k]);
+          const float qf = float(pos[j] ? nd.qhi[j][k] : nd.qlo[j][k]);
+          tnj[j] = std::fma(qn, av[j], bn[j]);
+          tfj[j] = std::fma(qf, av[j], bf[j]);
+        }
+        const float tn = std::fmax(std::fmax(tnj[0], tnj[1]), std::fmax(tnj[2], 0.f));
+        const float tf = std::fmin(std::fmin(tfj[0], tfj[1]), std::fmin(tfj[2], best.t));
+        const bool hit = k < nd.nchild && tn <= tf * kRobustQ;
         tk[k] = hit ? std::fmin(tn, FLT_MAX) : kInf;
         ck[k] = nd.child[k];
       }

@@ -205,8 +205,9 @@ __device__ __forceinline__ float analytic_t(const lam_object &ob, f3 o, f3 u) {
 
 __device__ __forceinline__ float scale_of(uint32_t e) { return __uint_as_float(e << 23); }
 
-// robust slab-test widening (Ize 2013): 1 + 2*gamma(3), gamma(n)=n*2^-24/(1-n*2^-24)
-constexpr float kRobust = 1.0000003576278687f;
+// the relative widening of the quantized-frame slab test: 1 + 2^-21 (covers
+// the relative roundings of tn and tf and of the widened far distance)
+constexpr float kRobustQ = 1.000000476837158203125f;
 
 __device__ __forceinline__ void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb) {
   const bool s = tb < ta;
@@ -290,10 +291,13 @@ __device__ __forceinline__ bool trav_begin(const DevScene &sc, const Ray &r, Tra
     }
   }
   if (sc.n_nodes == 0) return false;
-  // ray constructor inv_u (cg.hpp:357-361)
-  t.ix = r.d.x != 0.f ? 1.f / r.d.x : kFltMax;
-  t.iy = r.d.y != 0.f ? 1.f / r.d.y : kFltMax;
-  t.iz = r.d.z != 0.f ? 1.f / r.d.z : kFltMax;
+  // ray constructor inv_u (cg.hpp:357-361), clamped to +-2^64 for the
+  // quantized-frame slab test (trav_step): the slab distances then stay
+  // finite, and an axis the ray does not move along tests o_j against the
+  // planes up to the test's widening
+  t.ix = fminf(fmaxf(r.d.x != 0.f ? 1.f / r.d.x : kFltMax, -0x1p64f), 0x1p64f);
+  t.iy = fminf(fmaxf(r.d.y != 0.f ? 1.f / r.d.y : kFltMax, -0x1p64f), 0x1p64f);
+  t.iz = fminf(fmaxf(r.d.z != 0.f ? 1.f / r.d.z : kFltMax, -0x1p64f), 0x1p64f);
   t.mr = cross(r.o, r.d);
   t.sp = 0;
   t.cur = 0;  // root is always an interior record
@@ -323,38 +327,55 @@ __device__ __forceinline__ bool trav_step(const DevScene &sc, Trav &t, const Sta
     const uint32_t qlo[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
     const uint32_t qhi[6] = {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
     const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    // the near and far planes of each axis by the sign of the ray's inverse
-    // direction: for ix > 0, (lo - o) ix <= (hi - o) ix (rounding is
-    // monotone), so the slab's min / max are these two values exactly, and
-    // the test below computes the same tn and tf as with min / max of both
-    bool pos[3] = {t.ix > 0.f, t.iy > 0.f, t.iz > 0.f};
+    // The slab test in the node's quantized frame: the plane of byte q on
+    // axis j is origin_j + q s_j, reached at t = q (s_j ix_j) + (origin_j -
+    // o_j) ix_j: one fused multiply-add per plane (s_j ix_j is exact, s_j a
+    // power of two).  The rounding of b_j = (origin_j - o_j) ix_j is absolute
+    // in t, and so is the rounding the decode origin_j + q s_j had where the
+    // builder checked the boxes conservative: E_j = 2^-22 (|b_j| + (|origin_j|
+    // + 255 s_j) |ix_j|) bounds both (and the rounding of b_j -+ E_j), so the
+    // near planes start from b_j - E_j and the far planes from b_j + E_j; the
+    // relative roundings of the fused multiply-adds and of the comparison
+    // take the factor 1 + 2^-21.  No child box the ray enters is culled
+    // (ix is clamped to +-2^64: for a vanishing direction component the
+    // slab distances shrink, which culls less).  Near and far planes per
+    // axis by the sign of ix (t is monotone in q).  The host traversal
+    // (host/trace.cpp) computes the same values.
+    const f3 o = t.r.o;
+    const float ixv[3] = {t.ix, t.iy, t.iz}, orv[3] = {h0.x, h0.y, h0.z}, sv[3] = {sx, sy, sz};
+    const float rov[3] = {o.x, o.y, o.z};
+    const float kInf = __builtin_inff();
+    float av[3], bn[3], bf[3];
     uint32_t qn[6], qf[6];
 #pragma unroll
-    for (int j = 0; j < 6; j++) {
-      qn[j] = pos[j >> 1] ? qlo[j] : qhi[j];
-      qf[j] = pos[j >> 1] ? qhi[j] : qlo[j];
+    for (int j = 0; j < 3; j++) {
+      const float ix = ixv[j];
+      const float b = (orv[j] - rov[j]) * ix;
+      const float e = (fabsf(b) + (fabsf(orv[j]) + 255.f * sv[j]) * fabsf(ix)) * 0x1p-22f;
+      av[j] = sv[j] * ix;
+      bn[j] = b - e;
+      bf[j] = b + e;
+      const bool pos = ix > 0.f;
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        qn[2 * j + w] = pos ? qlo[2 * j + w] : qhi[2 * j + w];
+        qf[2 * j + w] = pos ? qhi[2 * j + w] : qlo[2 * j + w];
+      }
     }
-    const float kInf = __builtin_inff();
-    const f3 o = t.r.o;
     float tk[8];
     uint32_t ck[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      // origin + q * 2^e: the product is exact (8-bit q, power-of-two
-      // scale), so the fused form rounds once, exactly as the sum does
       const int w = k >> 2, sh = (k & 3) * 8;
-      const float nx = __builtin_fmaf(float((qn[0 + w] >> sh) & 0xffu), sx, h0.x);
-      const float ny = __builtin_fmaf(float((qn[2 + w] >> sh) & 0xffu), sy, h0.y);
-      const float nz = __builtin_fmaf(float((qn[4 + w] >> sh) & 0xffu), sz, h0.z);
-      const float fx = __builtin_fmaf(float((qf[0 + w] >> sh) & 0xffu), sx, h0.x);
-      const float fy = __builtin_fmaf(float((qf[2 + w] >> sh) & 0xffu), sy, h0.y);
-      const float fz = __builtin_fmaf(float((qf[4 + w] >> sh) & 0xffu), sz, h0.z);
-      const float tnx = (nx - o.x) * t.ix, tfx = (fx - o.x) * t.ix;
-      const float tny = (ny - o.y) * t.iy, tfy = (fy - o.y) * t.iy;
-      const float tnz = (nz - o.z) * t.iz, tfz = (fz - o.z) * t.iz;
+      const float tnx = __builtin_fmaf(float((qn[0 + w] >> sh) & 0xffu), av[0], bn[0]);
+      const float tny = __builtin_fmaf(float((qn[2 + w] >> sh) & 0xffu), av[1], bn[1]);
+      const float tnz = __builtin_fmaf(float((qn[4 + w] >> sh) & 0xffu), av[2], bn[2]);
+      const float tfx = __builtin_fmaf(float((qf[0 + w] >> sh) & 0xffu), av[0], bf[0]);
+      const float tfy = __builtin_fmaf(float((qf[2 + w] >> sh) & 0xffu), av[1], bf[1]);
+      const float tfz = __builtin_fmaf(float((qf[4 + w] >> sh) & 0xffu), av[2], bf[2]);
       const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
       const float tf = fminf(fminf(tfx, tfy), fminf(tfz, t.best.t));
-      const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobust);
+      const bool hit = (uint32_t(k) < nchild) && (tn <= tf * kRobustQ);
       tk[k] = hit ? fminf(tn, kFltMax) : kInf;
       ck[k] = ch[k];
     }

@@ -10,7 +10,7 @@ namespace lam {
 // wavefront's extend and shadow kernels (split stacks, per-lane refill)
 // ---------------------------------------------------------------------------
 template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(64) void k_trace_rays(DevScene sc, const lam_ray *__restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(64, 6) void k_trace_rays(DevScene sc, const lam_ray *__restrict__ rays, uint32_t n,
                                                    lam_hit *__restrict__ hits, uint32_t *__restrict__ occl,
                                                    uint32_t *__restrict__ counts, uint32_t *__restrict__ err,
                                                    uint32_t *__restrict__ fetch, uint32_t *__restrict__ spill) {

@@ -91,7 +91,7 @@ __global__ void k_pass_next(uint32_t *__restrict__ ctrl, uint32_t stride) {
 
 // ---- extend: persistent closest hit ---------------------------------------------------
 template <bool COUNT>
-__global__ __launch_bounds__(64) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
+__global__ __launch_bounds__(64, 6) void k_extend(DevScene sc, const float4 *__restrict__ rays, Hit *__restrict__ hits,
                                                uint32_t *__restrict__ ctrl, uint32_t qsel, uint32_t *__restrict__ spill,
                                                unsigned long long *__restrict__ counts) {
   __shared__ uint32_t stack[kLdsStack * 64];
@@ -707,7 +707,7 @@ hipError_t guide_weights(int hero, const WaveState &ws, const GuideDev &gd, hipS
 
 // ---- shadow: persistent any-hit, then the pending NEE term -----------------------------
 template <int H, bool COUNT, bool REC = false>
-__global__ __launch_bounds__(64) void k_shadow(DevScene sc, WaveState ws, uint32_t *__restrict__ ctrl,
+__global__ __launch_bounds__(64, H == 4 ? 6 : 1) void k_shadow(DevScene sc, WaveState ws, uint32_t *__restrict__ ctrl,
                                                unsigned long long *__restrict__ counts,
                                                float4 *__restrict__ rec = nullptr,
                                                const uint32_t *__restrict__ rec_of_slot = nullptr,

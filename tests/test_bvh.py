@@ -119,3 +119,30 @@ def test_ties_go_to_lowest_object(tmp_path):
         assert np.all(h["obj"] == 0) and np.all(h["t"] == np.float32(3.0))
     rays["skip"] = 0
     assert np.all(o.closest(rays)["obj"] == 1)
+
+
+@pytest.mark.parametrize("scene_name", ["box_scene", "mixed_scene", "field_scene", "crowd_scene"])
+def test_host_traversal_of_the_product_bvh_is_exact(request, scene_name):
+    """lamh_trace_counts (the kernels' BVH8 traversal, quantized-frame slab
+    test and all, restated on the host) finds the oracle's brute-force
+    closest hits and occlusion on random rays, rays from inside the scene,
+    axis-parallel rays (zero direction components) and short shadow rays:
+    its culling keeps every box a ray enters"""
+    scene = request.getfixturevalue(scene_name)
+    o = Oracle(scene)
+    rng = np.random.default_rng(21)
+    v = scene.tri_v().reshape(-1, 3)
+    rays = random_rays(rng, 6000, v.min(0) - 1, v.max(0) + 1)
+    axis = random_rays(rng, 3000, v.min(0), v.max(0))
+    d = np.zeros((3000, 3), np.float32)
+    d[np.arange(3000), rng.integers(0, 3, 3000)] = rng.choice([-1.0, 1.0], 3000)
+    two = rng.random(3000) < 0.5
+    d[two, (np.argmax(np.abs(d[two]), axis=1) + 1) % 3] = rng.uniform(-1, 1, two.sum())
+    axis["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([rays, axis])
+    h, _ = la.host_trace(scene, rays)
+    b = o.closest(rays, brute=True)
+    assert np.array_equal(h.view(np.uint32), b.view(np.uint32))
+    rays["tmax"] = rng.uniform(0, 10, len(rays)).astype(np.float32)
+    occ, _ = la.host_trace(scene, rays, any_hit=True)
+    assert np.array_equal(occ, o.occluded(rays, brute=True))
