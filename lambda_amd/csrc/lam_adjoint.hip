@@ -223,7 +223,7 @@ __device__ __forceinline__ void adjoint_weights(const DevScene &sc, const float 
 #pragma unroll
   for (int k = 0; k < H; k++) {
     float X = 0.f, Y = 0.f, Z = 0.f;
-    splat_xyz(sc, lam.l[k], 1.f, X, Y, Z);
+    splat_xyz(sc, lam.l[k], 1.f, X, Y, Z, 1.f / sc.grid_delta);
     a[k] = (gx * X + gy * Y + gz * Z) * inv_n * sc.grid_delta / prob_l;
   }
 }

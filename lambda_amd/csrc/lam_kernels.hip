@@ -114,7 +114,7 @@ __global__ __launch_bounds__(64) void k_render_mega(DevScene sc, PathArgs pa, ui
     if (ovf) atomicOr(err, 1u);
     float X = 0.f, Y = 0.f, Z = 0.f;
 #pragma unroll
-    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z);
+    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z, 1.f / sc.grid_delta);
     film_add(film, pixel, X, Y, Z);
   }
 }

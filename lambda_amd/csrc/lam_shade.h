@@ -662,16 +662,37 @@ __device__ void path_loop(const DevScene &sc, uint32_t max_scat, float sample_p,
 // addHeroλ (cg.hpp:306-320) projected onto the CMFs: the splat weights of
 // wavelength l on the coefficient grid, with the reference's ordering of
 // (1-t) and t, folded into XYZ.
-__device__ __forceinline__ void splat_xyz(const DevScene &sc, float l, float s, float &X, float &Y, float &Z) {
+// i = the number of grid points p_c = inf + c Δ (float arithmetic) with
+// p_c <= l, capped at n.  p_c does not decrease with c, so i is the one
+// count with "i = 0 or p_{i-1} <= l" (not A) and "i = n or p_i > l" (not B),
+// whatever the start of the search.  The start is a guess from
+// (l - inf) / Δ (inv_d: any approximation of 1/Δ); one step fixes a guess
+// that is off by one, and the loops, which stop only where neither A nor B
+// holds, run for the rest.  A NaN l counts 0 grid points.
+__device__ __forceinline__ uint32_t splat_index(const DevScene &sc, float l, float inv_d) {
+  const int n = int(sc.grid_n);
+  const float inf = sc.grid_inf, D = sc.grid_delta;
+  const float g = (l - inf) * inv_d;
+  int ci = g >= 0.f ? int(fminf(g, float(n))) + 1 : 0;
+  if (ci > n) ci = n;
+  auto A = [&](int c) { return c > 0 && !(inf + float(c - 1) * D <= l); };
+  auto B = [&](int c) { return c < n && inf + float(c) * D <= l; };
+  if (A(ci))
+    ci--;
+  else if (B(ci))
+    ci++;
+  if (A(ci) || B(ci)) {
+    while (A(ci)) ci--;
+    while (B(ci)) ci++;
+  }
+  return uint32_t(ci);
+}
+
+__device__ __forceinline__ void splat_xyz(const DevScene &sc, float l, float s, float &X, float &Y, float &Z,
+                                          float inv_d) {
   const uint32_t n = sc.grid_n;
   const float inf = sc.grid_inf, D = sc.grid_delta;
-  // i = number of grid points inf+iΔ <= l, capped at n
-  float g = (l - inf) / D;
-  int ci = g >= 0.f ? int(fminf(g, float(n))) + 1 : 0;
-  if (ci > int(n)) ci = int(n);
-  while (ci > 0 && !(inf + float(ci - 1) * D <= l)) ci--;
-  while (ci < int(n) && inf + float(ci) * D <= l) ci++;
-  const uint32_t i = uint32_t(ci);
+  const uint32_t i = splat_index(sc, l, inv_d);
   const float l1 = inf + float(i) * D;
   float t = (l1 - l) / D;
   if (i == 0) t = 1.f;
@@ -690,10 +711,9 @@ __device__ __forceinline__ void splat_xyz(const DevScene &sc, float l, float s, 
   }
 }
 
-// Kahan-compensated accumulation of one sample per pixel (6 floats/px)
-__device__ __forceinline__ void film_add(float *__restrict__ film, uint32_t pixel, float X, float Y, float Z) {
-  float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;  // 8 floats: s.xyz c.xyz pad
-  float4 s = f[0], c = f[1];
+// Kahan-compensated accumulation of one sample per pixel (6 floats/px:
+// sums s.xyz, compensations c.xyz, in 8 floats)
+__device__ __forceinline__ void kahan_add(float4 &s, float4 &c, float X, float Y, float Z) {
   float v[3] = {X, Y, Z};
   float *sp[3] = {&s.x, &s.y, &s.z};
   float *cp[3] = {&c.x, &c.y, &c.z};
@@ -704,6 +724,11 @@ __device__ __forceinline__ void film_add(float *__restrict__ film, uint32_t pixe
     *cp[q] = (t - *sp[q]) - y;
     *sp[q] = t;
   }
+}
+__device__ __forceinline__ void film_add(float *__restrict__ film, uint32_t pixel, float X, float Y, float Z) {
+  float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;
+  float4 s = f[0], c = f[1];
+  kahan_add(s, c, X, Y, Z);
   f[0] = s;
   f[1] = c;
 }

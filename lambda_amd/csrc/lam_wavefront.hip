@@ -854,26 +854,38 @@ __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds 
   }
   const uint32_t npx = ids.npix ? ids.npix : n;
   if (t >= npx) return;
-  const uint32_t pass = ids.pixel ? 0u : ctrl[CTRL_PASS];
-  // slot p = q npx + t: pixel t (+ offset), sample pass + q (path_id
-  // without its per-slot division loop)
-  for (uint32_t p = t, q = 0; p < n; p += npx, q++) {
-    uint32_t pixel, sample;
-    if (ids.pixel) {
-      pixel = ids.pixel[p];
-      sample = ids.sample[p];
-    } else {
-      pixel = t + ids.pixel_offset;
-      sample = pass + q;
-    }
+  const float inv_d = 1.f / sc.grid_delta;
+  auto sample_xyz = [&](uint32_t p, uint32_t pixel, uint32_t sample, float &X, float &Y, float &Z) {
     const uint64_t key = lam_path_key(pa.seed, pixel, sample);
     Lam<H> lam;
     const float inv_om = lam_for<H>(sc, key, lam);
-    float X = 0.f, Y = 0.f, Z = 0.f;
+    X = Y = Z = 0.f;
 #pragma unroll
-    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z);
-    film_add(film, pixel, X, Y, Z);
+    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z, inv_d);
+  };
+  if (ids.pixel) {  // slots from lists
+    for (uint32_t p = t; p < n; p += npx) {
+      const uint32_t pixel = ids.pixel[p];
+      float X, Y, Z;
+      sample_xyz(p, pixel, ids.sample[p], X, Y, Z);
+      film_add(film, pixel, X, Y, Z);
+    }
+    return;
   }
+  // slot p = q npx + t: pixel t (+ offset), sample pass + q.  The pixel's
+  // sums and compensations stay in registers over its m samples (the same
+  // operations in the same order as m separate film_add calls).
+  const uint32_t pass = ctrl[CTRL_PASS];
+  const uint32_t pixel = t + ids.pixel_offset;
+  float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;
+  float4 s = f[0], c = f[1];
+  for (uint32_t p = t, q = 0; p < n; p += npx, q++) {
+    float X, Y, Z;
+    sample_xyz(p, pixel, pass + q, X, Y, Z);
+    kahan_add(s, c, X, Y, Z);
+  }
+  f[0] = s;
+  f[1] = c;
 }
 
 // ---------------------------------------------------------------------------

@@ -69,3 +69,23 @@ def test_device_tone_map_matches_host(ctx, box_scene):
     d = np.abs(dev.astype(np.int32) - host)
     assert d.max() <= 1 and np.mean(d > 0) < 1e-3, (d.max(), np.mean(d > 0))
     assert dev.mean() > 10
+
+
+def test_returned_images_own_their_memory(ctx, box_scene):
+    """Rendered images live in page-locked buffers that are reused once
+    released: an image kept by the caller is never overwritten by a later
+    render, and a released buffer comes back for the next render."""
+    ctx.upload(box_scene)
+    p = lambda seed: la.params(spp=2, max_scatterings=4, sample_p=0.8, hero=4, seed=seed)
+    a = ctx.render(p(1))
+    a_copy = a.copy()
+    frames = [ctx.render(p(2 + i)) for i in range(6)]  # more than the pool keeps
+    assert np.array_equal(a, a_copy)
+    assert len({f.ctypes.data for f in frames} | {a.ctypes.data}) == 7
+    assert not any(np.array_equal(f, a) for f in frames)
+    b_ref = ctx.render(p(2))
+    assert np.array_equal(b_ref, frames[0])
+    del frames, b_ref
+    c = ctx.render(p(1))
+    assert np.array_equal(c, a)
+    assert c.flags.writeable and c.dtype == np.float32 and c.shape == a.shape
