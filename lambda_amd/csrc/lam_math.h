@@ -12,12 +12,19 @@
 //
 // Methods (textbook, no vendor code):
 //   sin/cos  Cody-Waite reduction by pi/2 (two-part constant) + Taylor
-//            series to degree 17 on [-pi/4, pi/4];
-//   log2     frexp split, ln m = 2 atanh((m-1)/(m+1)) series;
-//   exp2     2^y = 2^n * exp(r ln2), |r| <= 1/2, Taylor to degree 15;
+//            series to degree 11 / 12 on [-pi/4, pi/4] (truncation below
+//            1e-11 relative);
+//   log2     frexp split, ln m = 2 atanh((m-1)/(m+1)) series to f^19,
+//            |f| <= 0.172 (truncation below 1e-17: pow scales log2's error
+//            by |y|, up to 2048 here);
+//   exp2     2^y = 2^n * exp(r ln2), |r| <= 1/2, Taylor to degree 11
+//            (truncation below 1e-14);
 //   pow      exp2(y log2 x) with C99 special cases;
 //   atan2    two half-angle reductions atan t = 2 atan(t/(1+sqrt(1+t^2)))
-//            then the odd Taylor series.
+//            then the odd Taylor series to t^15, t <= tan(pi/16)
+//            (truncation below 1e-13).
+// Each truncation is far below half a float ulp, so the rounded results
+// stay within 1 ulp of libm; shorter series cost fewer double operations.
 // RNG: splitmix64 finalizer (Steele, Lea & Flood 2014) keyed by
 // (seed, pixel, sample) and a per-draw dimension index.
 #pragma once
@@ -35,10 +42,7 @@
 
 LAM_HD double lam_sin_series(double r) {
   const double r2 = r * r;
-  double p = -1.0 / 355687428096000.0;     /* -1/17! */
-  p = p * r2 + 1.0 / 1307674368000.0;      /* 1/15! */
-  p = p * r2 - 1.0 / 6227020800.0;         /* -1/13! */
-  p = p * r2 + 1.0 / 39916800.0;           /* 1/11! */
+  double p = 1.0 / 39916800.0;             /* 1/11! */
   p = p * r2 - 1.0 / 362880.0;             /* -1/9! */
   p = p * r2 + 1.0 / 5040.0;               /* 1/7! */
   p = p * r2 - 1.0 / 120.0;
@@ -47,10 +51,7 @@ LAM_HD double lam_sin_series(double r) {
 }
 LAM_HD double lam_cos_series(double r) {
   const double r2 = r * r;
-  double p = 1.0 / 6402373705728000.0;     /* 1/18! */
-  p = p * r2 - 1.0 / 20922789888000.0;     /* -1/16! */
-  p = p * r2 + 1.0 / 87178291200.0;        /* 1/14! */
-  p = p * r2 - 1.0 / 479001600.0;          /* -1/12! */
+  double p = -1.0 / 479001600.0;           /* -1/12! */
   p = p * r2 + 1.0 / 3628800.0;            /* 1/10! */
   p = p * r2 - 1.0 / 40320.0;              /* -1/8! */
   p = p * r2 + 1.0 / 720.0;
@@ -106,9 +107,7 @@ LAM_HD double lam_log2_pos(double x) {
   }
   const double f = (m - 1.0) / (m + 1.0);
   const double f2 = f * f;
-  double p = 1.0 / 23.0;
-  p = p * f2 + 1.0 / 21.0;
-  p = p * f2 + 1.0 / 19.0;
+  double p = 1.0 / 19.0;
   p = p * f2 + 1.0 / 17.0;
   p = p * f2 + 1.0 / 15.0;
   p = p * f2 + 1.0 / 13.0;
@@ -126,12 +125,8 @@ LAM_HD double lam_exp2_d(double y) {
   if (y < -1100.0) return 0.0;
   const double n = floor(y + 0.5);
   const double a = (y - n) * 0.69314718055994530942; /* r ln 2 */
-  double p = 1.0 / 1307674368000.0;         /* 1/15! */
-  p = p * a + 1.0 / 87178291200.0;          /* 1/14! */
-  p = p * a + 1.0 / 6227020800.0;
-  p = p * a + 1.0 / 479001600.0;
-  p = p * a + 1.0 / 39916800.0;
-  p = p * a + 1.0 / 3628800.0;
+  double p = 1.0 / 39916800.0;             /* 1/11! */
+  p = p * a + 1.0 / 3628800.0;              /* 1/10! */
   p = p * a + 1.0 / 362880.0;
   p = p * a + 1.0 / 40320.0;
   p = p * a + 1.0 / 5040.0;
@@ -190,11 +185,7 @@ LAM_HD double lam_atan_pos(double t) { /* t >= 0, finite */
   t = t / (1.0 + sqrt(1.0 + t * t));
   t = t / (1.0 + sqrt(1.0 + t * t));
   const double t2 = t * t;
-  double p = -1.0 / 23.0;
-  p = p * t2 + 1.0 / 21.0;
-  p = p * t2 - 1.0 / 19.0;
-  p = p * t2 + 1.0 / 17.0;
-  p = p * t2 - 1.0 / 15.0;
+  double p = -1.0 / 15.0;
   p = p * t2 + 1.0 / 13.0;
   p = p * t2 - 1.0 / 11.0;
   p = p * t2 + 1.0 / 9.0;

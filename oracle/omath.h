@@ -6,10 +6,12 @@
  * constructions, evaluated in double from correctly rounded IEEE operations
  * and rounded once to float (the oracle is compiled with -ffp-contract=off):
  *   sin/cos: Cody-Waite reduction by pi/2 with fdlibm's two-part pi/2,
- *            Taylor series on [-pi/4, pi/4] (to x^17 / x^18);
- *   log2:    x = m 2^e, ln m = 2 atanh((m-1)/(m+1)) (odd series to f^23);
- *   exp2:    2^n * exp(r ln 2), Taylor to r^15;
- *   atan:    two half-angle reductions then the odd series to t^23.
+ *            Taylor series on [-pi/4, pi/4] (to x^11 / x^12);
+ *   log2:    x = m 2^e, ln m = 2 atanh((m-1)/(m+1)) (odd series to f^19);
+ *   exp2:    2^n * exp(r ln 2), Taylor to r^11;
+ *   atan:    two half-angle reductions then the odd series to t^15.
+ * Each series is truncated where its remainder is far below half a float
+ * ulp (1e-11 relative or less; log2 1e-17, as pow scales its error by |y|).
  * They are pinned against glibc in tests/test_oracle.py (test_math_*).
  * RNG: splitmix64 finalizer (Steele, Lea, Flood 2014), keyed by
  * (seed, pixel, sample) plus a per-draw dimension.
@@ -21,10 +23,7 @@
 
 static inline double om_sin_poly(double r) {
   double r2 = r * r;
-  double p = -1.0 / 355687428096000.0;
-  p = p * r2 + 1.0 / 1307674368000.0;
-  p = p * r2 - 1.0 / 6227020800.0;
-  p = p * r2 + 1.0 / 39916800.0;
+  double p = 1.0 / 39916800.0;
   p = p * r2 - 1.0 / 362880.0;
   p = p * r2 + 1.0 / 5040.0;
   p = p * r2 - 1.0 / 120.0;
@@ -33,10 +32,7 @@ static inline double om_sin_poly(double r) {
 }
 static inline double om_cos_poly(double r) {
   double r2 = r * r;
-  double p = 1.0 / 6402373705728000.0;
-  p = p * r2 - 1.0 / 20922789888000.0;
-  p = p * r2 + 1.0 / 87178291200.0;
-  p = p * r2 - 1.0 / 479001600.0;
+  double p = -1.0 / 479001600.0;
   p = p * r2 + 1.0 / 3628800.0;
   p = p * r2 - 1.0 / 40320.0;
   p = p * r2 + 1.0 / 720.0;
@@ -80,9 +76,7 @@ static inline double om_log2(double x) {
   }
   f = (m - 1.0) / (m + 1.0);
   f2 = f * f;
-  p = 1.0 / 23.0;
-  p = p * f2 + 1.0 / 21.0;
-  p = p * f2 + 1.0 / 19.0;
+  p = 1.0 / 19.0;
   p = p * f2 + 1.0 / 17.0;
   p = p * f2 + 1.0 / 15.0;
   p = p * f2 + 1.0 / 13.0;
@@ -100,11 +94,7 @@ static inline double om_exp2(double y) {
   if (y < -1100.0) return 0.0;
   n = floor(y + 0.5);
   a = (y - n) * 0.69314718055994530942;
-  p = 1.0 / 1307674368000.0;
-  p = p * a + 1.0 / 87178291200.0;
-  p = p * a + 1.0 / 6227020800.0;
-  p = p * a + 1.0 / 479001600.0;
-  p = p * a + 1.0 / 39916800.0;
+  p = 1.0 / 39916800.0;
   p = p * a + 1.0 / 3628800.0;
   p = p * a + 1.0 / 362880.0;
   p = p * a + 1.0 / 40320.0;
@@ -162,11 +152,7 @@ static inline double om_atan_pos(double t) {
   t = t / (1.0 + sqrt(1.0 + t * t));
   t = t / (1.0 + sqrt(1.0 + t * t));
   t2 = t * t;
-  p = -1.0 / 23.0;
-  p = p * t2 + 1.0 / 21.0;
-  p = p * t2 - 1.0 / 19.0;
-  p = p * t2 + 1.0 / 17.0;
-  p = p * t2 - 1.0 / 15.0;
+  p = -1.0 / 15.0;
   p = p * t2 + 1.0 / 13.0;
   p = p * t2 - 1.0 / 11.0;
   p = p * t2 + 1.0 / 9.0;
