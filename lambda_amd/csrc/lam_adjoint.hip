@@ -89,7 +89,8 @@ __device__ void thinfilm_f_grad(const DevScene &sc, const lam_material &m, const
     const float S = sqrtf(1.f - n01 * n01);
     const float cos_l = S * (1.f - ci * ci);
     const float arg = 4 * kPi * n1[k] * m.d * cos_l / L.l[k];
-    const float cos_d = lam_cosf(arg);
+    float sin_d, cos_d;
+    lam_sincosf(arg, &sin_d, &cos_d);
     const float a01 = 1.f - n1[k], b01 = 1.f + n1[k];
     const float r0_01 = a01 * a01 / (b01 * b01);
     const float a12 = n1[k] - n2[k], b12 = n1[k] + n2[k];
@@ -102,7 +103,6 @@ __device__ void thinfilm_f_grad(const DevScene &sc, const lam_material &m, const
     const float den = R01 * R12 + 1.f + ravg * cos_d;
     val[k] = num / den * c * pw;
     // tangents
-    const float sin_d = lam_sinf(arg);
     const float dcos_dd = -sin_d * (4 * kPi * n1[k] * cos_l / L.l[k]);
     const float dS = (S > 0.f) ? n01 * n01 * n01 / S : 0.f;  // d sqrt(1-1/n1^2) / d n1
     const float dcos_l = dS * Q;

@@ -97,6 +97,22 @@ LAM_HD float lam_cosf(float xf) {
   return (float)v;
 }
 
+/* sin and cos of one argument: one reduction and each series once (the
+ * same operations as lam_sinf and lam_cosf, so the same bits) */
+LAM_HD void lam_sincosf(float xf, float *sf, float *cf) {
+  const double x = (double)xf;
+  if (!(x - x == 0.0)) {
+    *sf = *cf = xf - xf;
+    return;
+  }
+  int q;
+  const double r = lam_reduce_pio2(x, &q);
+  const double s = lam_sin_series(r), c = lam_cos_series(r);
+  const double sv = (q & 1) ? c : s, cv = (q & 1) ? s : c;
+  *sf = (float)((q & 2) ? -sv : sv);
+  *cf = (float)(((q + 1) & 2) ? -cv : cv);
+}
+
 /* log2 of a positive finite double */
 LAM_HD double lam_log2_pos(double x) {
   int e;

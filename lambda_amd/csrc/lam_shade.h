@@ -99,8 +99,8 @@ __device__ __forceinline__ void sphere_surf(const lam_object &ob, f3 lp, f3 wu, 
   const float ct = n.z;
   const float st = sqrtf(1.f - ct * ct);
   const float phi = lam_atan2f(n.y, n.x);
-  const float cp = lam_cosf(phi);
-  const float spp = lam_sinf(phi);
+  float spp, cp;
+  lam_sincosf(phi, &spp, &cp);
   s.F.x = mk(-st * spp, st * cp, 0.f);
   s.F.y = mk(ct * cp, ct * spp, -st);
   s.F.z = n;
@@ -157,19 +157,25 @@ __device__ __forceinline__ f3 cos_hemi(float x0, float x1) {
   const float ct = sqrtf(cc);
   const float st = sqrtf(1.f - cc);
   const float phi = 2.f * kPi * x1;
-  return mk(st * lam_cosf(phi), st * lam_sinf(phi), ct);
+  float sp, cp;
+  lam_sincosf(phi, &sp, &cp);
+  return mk(st * cp, st * sp, ct);
 }
 __device__ __forceinline__ f3 unif_hemi(float x0, float x1) {
   const float ct = 1.f - x0;
   const float st = sqrtf(1.f - ct * ct);
   const float phi = 2.f * kPi * x1;
-  return mk(st * lam_cosf(phi), st * lam_sinf(phi), ct);
+  float sp, cp;
+  lam_sincosf(phi, &sp, &cp);
+  return mk(st * cp, st * sp, ct);
 }
 __device__ __forceinline__ f3 blinn_h(float alpha, float x0, float x1) {
   const float ct = lam_powf(1.f - x0, 1.f / (alpha + 1.f));
   const float st = sqrtf(1.f - ct * ct);
   const float phi = 2.f * kPi * x1;
-  return mk(st * lam_cosf(phi), st * lam_sinf(phi), ct);
+  float sp, cp;
+  lam_sincosf(phi, &sp, &cp);
+  return mk(st * cp, st * sp, ct);
 }
 
 // ---- materials ------------------------------------------------------------------
@@ -462,7 +468,9 @@ __device__ void sample_light(const DevScene &sc, const lam_light &lt, const Lam<
     const float ct = fmax_ref(0.f, 1.f - xa * sr);
     const float st = sqrtf(1.f - ct * ct);
     const float phi = 2 * kPi * xb;
-    const f3 w = add(add(mul(mul(B.x, st), lam_cosf(phi)), mul(mul(B.y, st), lam_sinf(phi))), mul(B.z, ct));
+    float sp, cp;
+    lam_sincosf(phi, &sp, &cp);
+    const f3 w = add(add(mul(mul(B.x, st), cp), mul(mul(B.y, st), sp)), mul(B.z, ct));
     ls.dir = w;
     ls.sray.tmax = sqrtf(dist2);
     ls.sray.skip = lt.object;
