@@ -251,9 +251,8 @@ def render_kernel_roofline(ctx, bounces, p, hero, spp):
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms": round(ms, 3), "rays": int(rays),
                      "nodes_per_ray": round(nodes / max(rays, 1), 3), "tris_per_ray": round(tris / max(rays, 1), 3),
                      "rays_per_s": round(rays / (ms * 1e-3), 1) if ms > 0 else 0.0}
-    m = ctx.wave_config()["samples_per_pass"]
-    out["workload"] = (f"{spp}-spp profiled render (one lane, no graph): passes of "
-                       f"{max(1, min(m, spp // 2))} samples of every pixel")
+    out["workload"] = (f"{spp}-spp profiled render (one lane, no graph): {t.passes} passes of "
+                       f"{t.pass_samples} samples of every pixel")
     return out
 
 
@@ -648,7 +647,7 @@ def main():
     samples = w * h * spp * args.steps
     log(f"render: {args.steps} frames of {spp} spp in {wall:.3f}s wall, {stream_ms:.1f} ms on stream; "
         f"last frame mean {img.mean():.4g} finite {np.isfinite(img).all()}")
-    m_pass = max(1, min(ctx.wave_config()["samples_per_pass"], spp // 2))
+    m_pass, n_pass = tm.pass_samples, tm.passes  # the timed frames' pass shape
     out = {
         "metric": "samples/s (plain method, 1 GPU)", "value": round(samples / wall, 1), "unit": "samples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": warm,
@@ -657,7 +656,7 @@ def main():
         "config": {"workload": "icosphere field: 64 x 15,680-tri instances (1,003,520 tris), ground plane, "
                                "2 area lights, 1920x1080, 64 spp, 32 bounces, p=0.8, hero 4 on 360-830 nm; "
                                "one step = one frame",
-                   "width": w, "height": h, "spp_per_step": spp, "samples_per_pass": m_pass,
+                   "width": w, "height": h, "spp_per_step": spp, "samples_per_pass": m_pass, "passes": n_pass,
                    "bounces": bounces, "bounce_prob": p, "triangles": scene.n_tris},
         "render_rays": {"rays_per_s": round((tm.extend_rays + tm.shadow_rays) / wall, 1),
                         "extend_rays_per_sample": round(tm.extend_rays / samples, 4),

@@ -212,6 +212,10 @@ typedef struct {
    * kernels (the finish kernel's rays are not among them) */
   uint64_t extend_nodes, extend_tris, extend_traced;
   uint64_t shadow_nodes, shadow_tris, shadow_traced;
+  /* the wavefront passes of the last lam_render / lam_render_guided (its
+   * first chunk, after any training passes): samples of every pixel per
+   * pass and the number of passes */
+  uint32_t pass_samples, passes;
 } lam_timing;
 
 /* ---- context ---------------------------------------------------------- */
@@ -427,9 +431,10 @@ typedef struct {
   uint32_t finish_per_cu;  /* finish-kernel blocks (64 lanes) per CU: 1..32 (default 10) */
   uint32_t shade_per_cu;   /* shade-kernel blocks (256 lanes) per CU: 1..8 (default 3) */
   uint32_t samples_per_pass; /* at most this many samples of every pixel one wavefront pass traces together:
-                                1..32 (default 32); a render of spp samples takes min(this, spp / 2) per pass, at
-                                most 64 Mi paths per pass (frames of at most 64 Mi pixels); the film still adds each
-                                pixel's samples in sample order */
+                                1..32 (default 32), at most 64 Mi paths per pass (frames of at most 64 Mi pixels);
+                                a render of spp samples takes ceil(spp / g) per pass, g >= 2 passes rounded up to
+                                a multiple of the lanes, so that every lane traces beside the others; the film still
+                                adds each pixel's samples in sample order */
 } lam_wave_config;
 int lam_get_wave_config(const lam_ctx *ctx, lam_wave_config *out);
 int lam_set_wave_config(lam_ctx *ctx, const lam_wave_config *cfg);

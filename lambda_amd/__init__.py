@@ -162,7 +162,8 @@ class Timing(C.Structure):
                 ("guide_dropped", C.c_uint64), ("guide_zero_pdf", C.c_uint64),
                 ("guide_loss_first", C.c_double), ("guide_loss_last", C.c_double),
                 ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64), ("extend_traced", C.c_uint64),
-                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("shadow_traced", C.c_uint64)]
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("shadow_traced", C.c_uint64),
+                ("pass_samples", C.c_uint32), ("passes", C.c_uint32)]
 
 
 def params(spp=4, max_scatterings=4, sample_p=0.5, hero=16, seed=0, spp_offset=0, flags=0):

@@ -429,6 +429,31 @@ def test_wave_config_keeps_the_image(ctx, box_scene):
     assert ctx.wave_config() == default
 
 
+@pytest.mark.parametrize("spp,cfg,flags,shape", [
+    (64, {}, 0, (22, 3)), (64, dict(lanes=2), 0, (32, 2)), (64, {}, la.LAM_FLAG_NO_PIPE, (32, 2)),
+    (128, {}, 0, (22, 6)), (256, {}, 0, (29, 9)), (7, {}, 0, (3, 3)), (2, {}, 0, (1, 2)), (1, {}, 0, (1, 1)),
+    (64, dict(samples_per_pass=16), 0, (11, 6)), (64, dict(lanes=1), 0, (32, 2))])
+def test_pass_shape(ctx, box_scene, spp, cfg, flags, shape):
+    """a render of spp samples goes in g >= 2 passes of at most
+    samples_per_pass samples, g a multiple of the lanes; the image does not
+    depend on it"""
+    ctx.upload(box_scene)
+    default = ctx.wave_config()
+    pr = la.params(spp=spp, max_scatterings=2, sample_p=0.8, hero=4, seed=5)
+    try:
+        ctx.set_wave_config(**{**default, **cfg})
+        img = ctx.render(la.params(spp=spp, max_scatterings=2, sample_p=0.8, hero=4, seed=5, flags=flags))
+        t = ctx.timing()
+        assert (t.pass_samples, t.passes) == shape
+    finally:
+        ctx.set_wave_config(**default)
+    ctx.set_wave_config(**{**default, "lanes": 1, "samples_per_pass": 1})
+    try:
+        assert np.array_equal(img, ctx.render(pr))
+    finally:
+        ctx.set_wave_config(**default)
+
+
 def test_pipelined_chunks_are_exact(ctx, tmp_path):
     """a frame of more than one chunk of path slots (4,410,000 pixels > 4 Mi):
     both lanes per chunk, the same image as one lane"""
