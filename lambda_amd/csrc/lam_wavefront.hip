@@ -841,7 +841,12 @@ __global__ __launch_bounds__(64) void k_finish(DevScene sc, PathArgs pa, PathIds
 // One thread per pixel of the pass: its m samples (slots i, i + npix, ...)
 // are added to the film in sample order, so the film receives the sums of a
 // one-sample-per-pass render in the same order.
-template <int H>
+// STAGE: the colour-matching table is copied to LDS first (every splat
+// reads six of its entries).  A pixel's next sample's radiance is loaded
+// before the current sample is splatted, so that its round trip overlaps
+// the splat's arithmetic.
+constexpr uint32_t kFilmLdsBytes = 32 * 1024;
+template <int H, int STAGE>
 __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds ids, WaveState ws, uint32_t n,
                                               const uint32_t *__restrict__ ctrl, float *__restrict__ film,
                                               float *__restrict__ out) {
@@ -852,22 +857,32 @@ __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds 
     for (int k = 0; k < H; k++) out[size_t(t) * H + k] = ws.L[size_t(k) * ws.cap + t];
     return;
   }
+  if constexpr (STAGE) {
+    extern __shared__ float cmf_lds[];
+    for (uint32_t i = threadIdx.x; i < 3 * sc.grid_n; i += blockDim.x) cmf_lds[i] = sc.cmf[i];
+    __syncthreads();
+    sc.cmf = cmf_lds;
+  }
   const uint32_t npx = ids.npix ? ids.npix : n;
   if (t >= npx) return;
   const float inv_d = 1.f / sc.grid_delta;
-  auto sample_xyz = [&](uint32_t p, uint32_t pixel, uint32_t sample, float &X, float &Y, float &Z) {
+  const size_t cap = ws.cap;
+  auto sample_xyz = [&](const float (&L)[H], uint32_t pixel, uint32_t sample, float &X, float &Y, float &Z) {
     const uint64_t key = lam_path_key(pa.seed, pixel, sample);
     Lam<H> lam;
     const float inv_om = lam_for<H>(sc, key, lam);
     X = Y = Z = 0.f;
 #pragma unroll
-    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], ws.L[size_t(k) * ws.cap + p] / inv_om, X, Y, Z, inv_d);
+    for (int k = 0; k < H; k++) splat_xyz(sc, lam.l[k], L[k] / inv_om, X, Y, Z, inv_d);
   };
   if (ids.pixel) {  // slots from lists
     for (uint32_t p = t; p < n; p += npx) {
       const uint32_t pixel = ids.pixel[p];
+      float L[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) L[k] = ws.L[size_t(k) * cap + p];
       float X, Y, Z;
-      sample_xyz(p, pixel, ids.sample[p], X, Y, Z);
+      sample_xyz(L, pixel, ids.sample[p], X, Y, Z);
       film_add(film, pixel, X, Y, Z);
     }
     return;
@@ -879,13 +894,32 @@ __global__ __launch_bounds__(256) void k_film(DevScene sc, PathArgs pa, PathIds 
   const uint32_t pixel = t + ids.pixel_offset;
   float4 *f = reinterpret_cast<float4 *>(film) + size_t(pixel) * 2;
   float4 s = f[0], c = f[1];
+  float L[H];
+#pragma unroll
+  for (int k = 0; k < H; k++) L[k] = ws.L[size_t(k) * cap + t];
   for (uint32_t p = t, q = 0; p < n; p += npx, q++) {
+    float Ln[H];
+    const uint32_t pn = p + npx;
+#pragma unroll
+    for (int k = 0; k < H; k++) Ln[k] = pn < n ? ws.L[size_t(k) * cap + pn] : 0.f;
     float X, Y, Z;
-    sample_xyz(p, pixel, pass + q, X, Y, Z);
+    sample_xyz(L, pixel, pass + q, X, Y, Z);
     kahan_add(s, c, X, Y, Z);
+#pragma unroll
+    for (int k = 0; k < H; k++) L[k] = Ln[k];
   }
   f[0] = s;
   f[1] = c;
+}
+
+template <int H>
+static void launch_film(const DevScene &sc, const PathArgs &pa, const PathIds &ids, const WaveState &ws, uint32_t n,
+                        const uint32_t *ctrl, float *film, float *out, uint32_t blocks, hipStream_t st) {
+  const uint32_t bytes = 3u * sc.grid_n * 4u;
+  if (!out && bytes <= kFilmLdsBytes)
+    hipLaunchKernelGGL((k_film<H, 1>), dim3(blocks), dim3(256), bytes, st, sc, pa, ids, ws, n, ctrl, film, out);
+  else
+    hipLaunchKernelGGL((k_film<H, 0>), dim3(blocks), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -971,8 +1005,7 @@ static hipError_t enqueue_pass_t(const DevScene &sc, const PathArgs &pa, const P
     q ^= 1u;
   }
   ev(2);
-  if (film || out)
-    hipLaunchKernelGGL((k_film<H>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, out);
+  if (film || out) launch_film<H>(sc, pa, ids, ws, n, ctrl, film, out, b256, st);
   ev(3);
   return hipGetLastError();
 }
@@ -992,9 +1025,9 @@ hipError_t enqueue_film(int hero, const DevScene &sc, const PathArgs &pa, uint32
   const PathIds ids{nullptr, nullptr, pixel_offset, m > 1 ? n / m : 0u};
   const uint32_t b256 = (n / m + 255) / 256;
   if (hero == 16)
-    hipLaunchKernelGGL((k_film<16>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
+    launch_film<16>(sc, pa, ids, ws, n, ctrl, film, nullptr, b256, st);
   else
-    hipLaunchKernelGGL((k_film<4>), dim3(b256), dim3(256), 0, st, sc, pa, ids, ws, n, ctrl, film, nullptr);
+    launch_film<4>(sc, pa, ids, ws, n, ctrl, film, nullptr, b256, st);
   return hipGetLastError();
 }
 
