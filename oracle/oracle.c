@@ -1233,6 +1233,10 @@ static void add_hero(const orc_scene *s, real *v, const olam *L, const real *sv)
   }
 }
 
+/* per-pixel spectral accumulator of orc_render: grids of up to this many
+ * bins (32 KB of stack per thread in the FP64 build) */
+#define ORC_RENDER_MAX_BINS 4096
+
 /* Renderer::render (src/renderer.cpp:14-61): OpenMP over pixels, a spectral
  * film per pixel, coefλ2linRGB with the row flip.  rows: render only rows
  * i with i % row_step == row_phase (bounded CPU-baseline samples); others
@@ -1245,7 +1249,7 @@ int orc_render(const orc_scene *s, const lam_render_params *p, float *linrgb, fl
   int H = (int)p->hero;
   uint32_t nb = d->grid.n;
   if (H != 4 && H != 16) return -1;
-  if (nb > 512) return -1;
+  if (nb > ORC_RENDER_MAX_BINS) return -1;
   if (row_step < 1) row_step = 1;
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
@@ -1255,11 +1259,11 @@ int orc_render(const orc_scene *s, const lam_render_params *p, float *linrgb, fl
 #pragma omp parallel for collapse(2) schedule(dynamic, 16)
   for (i = 0; i < h; i++)
     for (j = 0; j < w; j++) {
-      real acc[512], X = 0, Y = 0, Z = 0, N = (real)(float)((uint64_t)p->spp * (uint64_t)H), r, g, b;
+      real acc[ORC_RENDER_MAX_BINS], X = 0, Y = 0, Z = 0, N = (real)(float)((uint64_t)p->spp * (uint64_t)H), r, g, b;
       uint32_t k, q;
       size_t o;
       if (i % row_step != row_phase) continue;
-      for (q = 0; q < nb && q < 512; q++) acc[q] = 0;
+      for (q = 0; q < nb; q++) acc[q] = 0;
       for (k = 0; k < p->spp; k++) {
         opath pa;
         olam L;

@@ -255,6 +255,32 @@ def test_spectra_only_staging_parity(ctx, crowd_scene, hero):
     assert relmse < 1e-8, relmse
 
 
+@pytest.mark.parametrize("hero", [4, 16])
+def test_fine_spectral_grid_parity(ctx, tmp_path, hero):
+    """a 3,000-bin grid: the colour-matching table (36 KB) is too large for
+    the film kernel's LDS copy and the spectra for the shade kernels', so
+    both read them from HBM; paths and images against the oracle"""
+    from tests.scene_text import mixed_scene_text
+    text = mixed_scene_text(str(tmp_path)).rstrip()
+    scene = la.Scene.from_text(text[:-1] + ',"spectral_grid":{"n":3000,"inf":380,"sup":780}}')
+    assert scene.desc.grid.n == 3000 and 3 * 3000 * 4 > 32 * 1024
+    assert _shade_stage(scene) in (0, 2)
+    ctx.upload(scene)
+    orc = ob.Oracle(scene)
+    npix = scene.width * scene.height
+    rng = np.random.default_rng(23)
+    pix = rng.integers(0, npix, 4096).astype(np.uint32)
+    smp = rng.integers(0, 64, 4096).astype(np.uint32)
+    pr = la.params(spp=1, max_scatterings=8, sample_p=0.8, hero=hero, seed=3)
+    g, o = ctx.trace_paths(pr, pix, smp), orc.trace_paths(pr, pix, smp)
+    assert (o > 0).mean() > 0.05
+    _check_paths(g, o)
+    pr = la.params(spp=6, max_scatterings=8, sample_p=0.8, hero=hero, seed=2)
+    gi, oi = ctx.render(pr), orc.render(pr)
+    relmse = np.mean((gi - oi) ** 2) / np.mean(oi ** 2)
+    assert relmse < 1e-8, relmse
+
+
 def test_tail_handoff_is_exact(ctx, tmp_path):
     """262,144 paths: the wavefront runs the first bounces and hands the
     short queue to the finish kernel; the image must equal the pure
